@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/golden.json: the fixtures the oracle and the GPU library are pinned on.
+
+Run in a container that holds the reference tree (default /root/reference) after
+`make -C oracle ref`.  Three kinds of fixture are collected, all data (inputs + outputs):
+
+  e0      the printed values of the reference's own E0 output (E0_resultado.txt)
+  lotes   time-series rows of committed runs in the reference's Lotes/ tree (inputs parsed from
+          the case directory: operador.txt, data_in.txt, velocity.txt, inclinacao.txt)
+  ref     outputs of the reference's own functions, built from its sources by oracle/Makefile
+          (oracle/_ref/harness_*): SHA-256 of the post-collision populations / force field at
+          chosen steps and the full-precision diagnostics, for small boxes and all operators,
+          with the "nofma" (-ffp-contract=off) build and with the optimised "fma" build.
+
+Usage:  python3 tests/golden/make_golden.py [--ref /root/reference]
+"""
+import argparse
+import hashlib
+import json
+import os
+import re
+import subprocess
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REFBIN = os.path.join(ROOT, "oracle", "_ref")
+
+OPS = ["P1", "P3 -0.4", "P3 0.5", "P3 2", "F1", "F2", "F3 1.569988"]
+
+# small parity cases: (label, mode, args)
+SMALL = []
+for op in OPS:
+    A = "0.225" if op[0] == "P" else "0.1"
+    SMALL.append((f"drop24x20 {op}", "drop",
+                  dict(nx=24, ny=20, R=6, width=3.45, op=op, tau=0.8, A=A, beta=0.8, steps=10, diag=4,
+                       dump="0,1,9")))
+SMALL.append(("drop sharp 26x22 P3 0.5 v0", "drop",
+              dict(nx=26, ny=22, R=7, width=-1, op="P3 0.5", tau=1.0, A=0.225, beta=0.8, v0=0.05, steps=10,
+                   diag=3, dump="0,9")))
+SMALL.append(("drop sharp 26x22 F2 v0", "drop",
+              dict(nx=26, ny=22, R=7, width=-1, op="F2", tau=0.6, A=0.1, beta=0.8, v0=0.05, steps=10,
+                   diag=3, dump="0,9")))
+SMALL.append(("incl 30 P3 0.5 (2,1)", "incl",
+              dict(nx=30, ny=30, p=2, q=1, width=3.45, op="P3 0.5", tau=1.0, A=0.225, beta=0.8, steps=10,
+                   diag=3, dump="0,9")))
+SMALL.append(("incl 30 F1 (1,1)", "incl",
+              dict(nx=30, ny=30, p=1, q=1, width=3.45, op="F1", tau=1.0, A=0.1, beta=0.8, steps=10,
+                   diag=3, dump="0,9")))
+
+
+def sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def run_harness(kind, mode, args):
+    exe = os.path.join(REFBIN, f"harness_{'incl' if mode == 'incl' else 'drop'}_{kind}")
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [exe, f"mode={mode}", f"out={d}"] + [f"{k}={v}" for k, v in args.items()]
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+        lines = open(os.path.join(d, "diag.txt")).read().splitlines()
+        names = lines[0][2:].split()
+        diag = [dict(zip(names, [float(v) for v in l.split()])) for l in lines[1:]]
+        hashes = {}
+        for st in [int(s) for s in str(args.get("dump", "")).split(",") if s]:
+            ent = {"R": sha(os.path.join(d, f"popR_{st}.bin")), "B": sha(os.path.join(d, f"popB_{st}.bin"))}
+            fp = os.path.join(d, f"force_{st}.bin")
+            if os.path.exists(fp):
+                ent["F"] = sha(fp)
+            hashes[str(st)] = ent
+    return diag, hashes
+
+
+def parse_e0(path):
+    txt = open(path).read()
+    errs = [m.group(1) for m in re.finditer(r"erro =\s+(\S+)\s+(ok|FALHOU)", txt)]
+    m4 = []
+    for line in txt.splitlines():
+        if re.match(r"^\s+P[13].*\s+-?\d\.\d{6}\s", line) and "lambda" not in line:
+            m4.append([float(v) for v in re.findall(r"-?\d+\.\d{6}", line)])
+    return {"erro": errs, "m4": m4, "tudo_certo": "TUDO CERTO" in txt}
+
+
+def read_case(d):
+    dat = open(os.path.join(d, "data_in.txt")).read().split()
+    vals = {dat[i].rstrip(":"): dat[i + 1] for i in range(1, len(dat) - 1, 2)}
+    case = {
+        "op": open(os.path.join(d, "operador.txt")).read().strip(),
+        "tau": float(vals["tau_R"]), "A": float(vals["fat_R_B"]), "beta": float(vals["recoll"]),
+    }
+    vf = os.path.join(d, "velocity.txt")
+    case["v0"] = float(open(vf).read().split()[0]) if os.path.exists(vf) else 0.0
+    inc = os.path.join(d, "inclinacao.txt")
+    if os.path.exists(inc):
+        p, q = open(inc).read().split()[:2]
+        case["pq"] = [int(p), int(q)]
+    res = open(os.path.join(d, "resumo.dat")).read().splitlines()
+    cab = res[0].lstrip("#").split()
+    row = res[1].split()
+    case["nx"] = int(float(row[cab.index("nx" if "nx" in cab else "N")]))
+    return case
+
+
+def table(path, steps):
+    lines = open(path).read().splitlines()
+    head = lines[0].lstrip("#").split()
+    rows = {}
+    for l in lines[1:]:
+        v = l.split()
+        if int(float(v[0])) in steps:
+            rows[str(int(float(v[0])))] = v
+    return {"header": head, "rows": rows}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    a = ap.parse_args()
+    out = {"e0": parse_e0(os.path.join(a.ref, "E0_resultado.txt")), "lotes": {}, "ref": []}
+
+    lotes = [
+        ("E2/P3 chi=0.5/tau=1/sigma=0.1", ["galilean.dat", "laplace.dat", "iso.dat"], [0, 500, 1000]),
+        ("E2/F2/tau=1/sigma=0.1", ["galilean.dat", "laplace.dat", "iso.dat"], [0, 500, 1000]),
+        ("E4v/P1/v0=0.1", ["galilean.dat", "laplace.dat", "iso.dat"], [0, 500, 1000]),
+        ("E1/F3/R=10", ["galilean.dat", "laplace.dat", "iso.dat"], [0, 500, 1000]),
+        ("E3chi/chi=0.5", ["inclinada.dat"], [0, 500]),
+        ("E3/F1/pq=1,0", ["inclinada.dat"], [0, 500]),
+    ]
+    for rel, files, steps in lotes:
+        d = os.path.join(a.ref, "Lotes", rel)
+        ent = {"case": read_case(d), "tables": {}}
+        for f in files:
+            ent["tables"][f] = table(os.path.join(d, f), set(steps))
+        out["lotes"][rel] = ent
+
+    for label, mode, args in SMALL:
+        diag_n, hash_n = run_harness("nofma", mode, args)
+        diag_f, _ = run_harness("fma", mode, args)
+        out["ref"].append({"label": label, "mode": mode, "args": args, "diag_nofma": diag_n,
+                           "hash_nofma": hash_n, "diag_fma": diag_f})
+
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", os.path.join(HERE, "golden.json"))
+
+
+if __name__ == "__main__":
+    main()
