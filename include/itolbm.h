@@ -1,0 +1,161 @@
+/*
+ * itolbm: C ABI of the MI355X colour-gradient LBM step (D3Q19 slab, one node in z).
+ *
+ * The reference has no C ABI: its hot path is the step loop of its drivers
+ * (Imm_OPS.cpp:246-372, Imm_INCL.cpp:358-442) calling, per site, colisao_comum
+ * (Operadores_tensao.cpp:340-373), emite_mediadores / propag_site_med / propag_site
+ * (LBM_functions.cpp:1183-1337) and, for the force operators, calcula_forca
+ * (Operadores_tensao.cpp:202-332), followed every PASSOS_DIAG steps by the diagnostics
+ * calc_galilean / calc_perfil_logistico / modos_interface (Imm_OPS.cpp:289-353) or
+ * mede_inclinada (Imm_INCL.cpp:85-189).  The entry points below replace that loop; each cites
+ * the reference code it stands in for.  A maintainer binds them from C++ (the reference's
+ * language) with a plain #include, or from Python with ctypes.
+ *
+ * Conventions: every function returns 0 on success and a negative LBM_E* code on failure, with a
+ * message in lbm_last_error(ctx); no exception crosses the ABI; one host thread per context.
+ * Host population arrays use the reference layout [site][19], site = x + nx * y
+ * (Imm_OPS.cpp:158-181, read_geo numbering of an all-fluid box LBM_functions.cpp:280-354).
+ */
+#ifndef ITOLBM_H
+#define ITOLBM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* operator ids: Operadores_tensao.cpp:45-49 */
+enum { LBM_OP_P1 = 1, LBM_OP_P3 = 3, LBM_OP_F1 = 11, LBM_OP_F2 = 12, LBM_OP_F3 = 13 };
+
+/* arithmetic / storage forms */
+enum {
+    LBM_PREC_F64 = 0 /* FP64 compute and storage: the drop-in form */
+};
+
+/* which population state to transfer */
+enum {
+    LBM_STATE_POST_STREAM = 0,    /* state at the start of a step (the reference's inif_R after the swap) */
+    LBM_STATE_POST_COLLISION = 1  /* post-collision, pre-stream (what the reference's diagnostics read) */
+};
+
+/* lbm_step flags */
+enum { LBM_STEP_RECORD_AUX = 1 /* the last step keeps the mediator momentum and force for diagnostics */ };
+
+/* error codes */
+enum {
+    LBM_OK = 0,
+    LBM_E_ARG = -1,    /* invalid argument */
+    LBM_E_HIP = -2,    /* HIP runtime error */
+    LBM_E_STATE = -3,  /* call not valid in the current state (e.g. diagnostics without recorded aux) */
+    LBM_E_NOMEM = -4
+};
+
+/* One case: the parameters the reference reads from data_in.txt / operador.txt
+ * (read_data LBM_functions.cpp:10-268, le_operador Operadores_tensao.cpp:66-86). */
+typedef struct {
+    int nx, ny;        /* periodic box nx x ny x 1 */
+    int op;            /* LBM_OP_* */
+    double chi;        /* P3 chi (default 0.5) */
+    double xi;         /* F3 xi (default 1.569988) */
+    double tau;        /* tau_R = tau_B */
+    double A;          /* fat_R_B: amplitude A (P class) or sigma (F class) */
+    double beta;       /* recoll */
+} lbm_case_desc;
+
+/* Drop measurements: the DROP fields (Definitions.cpp:251-339) set by calc_laplace_tension /
+ * calc_drop_radius / calc_perfil_logistico (Other_functions.cpp:60-554), the GI_DATA fields of
+ * calc_galilean (Galilean_diagnostics.cpp:7-105) and the MODOS of modos_interface
+ * (Operadores_tensao.cpp:380-449). */
+typedef struct {
+    double raio, raio_grad, raio_linha, p_in, p_out, delta_p, sigma, u_max, u_rms;
+    double massa_R, massa_B, qx, qy, qz, sd_in, sd_out, sigma_linha;
+    double K_perfil, r_perfil, esp_perfil, res_perfil;
+    int n_perfil, n_in, n_out;
+    double x0, y0, area, p_centro, p_canto;
+    double mean_x, mean_y, mean_z, drop_x, drop_y, max_ref, rms_ref, max_mean, rms_mean;
+    double max_lab, rms_lab, min_f, min_rho, min_phi, max_phi, axis_ratio;
+    double center_dx, center_dy, x_unwrapped;
+    long long negative_total, negative_color;
+    double R0, a2, a4, a8, rmin, rmax;
+    int ok;       /* calc_galilean's return value */
+    int n_ruim;   /* non-finite or non-positive sites */
+} lbm_drop_diag;
+
+/* Inclined-interface measurements: INCL of Imm_INCL.cpp:62-70, filled as mede_inclinada. */
+typedef struct {
+    double sigma_pop, int_grad, K, res, d0, u_max, u_rms, un_max, ut_max;
+    double massa_R, massa_B, qx, qy, SFx, SFy, min_f;
+    long long neg_total, neg_cor;
+    int n_ruim, n_ajuste, ok;
+} lbm_incl_diag;
+
+/* Allocates device state for n_cases cases (Imm_OPS.cpp:158-181 allocations).  Only LBM_PREC_F64. */
+int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx** out);
+
+/* Frees everything (the reference never frees). */
+int lbm_destroy(lbm_ctx* ctx);
+
+/* Last error message of ctx ("" if none); a static message if ctx is NULL. */
+const char* lbm_last_error(const lbm_ctx* ctx);
+
+/* Number of device bytes held by ctx. */
+long long lbm_device_bytes(const lbm_ctx* ctx);
+
+/* Sets the state at the start of the next step from post-stream populations, as the reference's
+ * initial_conditions_drop / Imm_INCL init fill inif_R / inif_B (Initial_conditions.cpp:37-169,
+ * Imm_INCL.cpp:309-334). */
+int lbm_upload_state(lbm_ctx* ctx, int case_id, const double* fR, const double* fB);
+
+/* Initial drop: initial_conditions_drop (Initial_conditions.cpp:37-169), EQ_CLASSICO branch, for a
+ * cylinder of radius R centred at the integer site (x0, y0), rho_R = rho_B = 1, velocity (v0, 0, 0),
+ * logistic profile of 10-90 % width `width` (k = 2 ln 9 / width, Imm_OPS.cpp:197) or a sharp step
+ * if width <= 0.  Computed on the host with libm, uploaded as the state at the next step. */
+int lbm_init_drop(lbm_ctx* ctx, int case_id, double R, double width, int x0, int y0, double v0);
+
+/* Initial band of Imm_INCL.cpp:309-334: rest equilibrium rho = 1, red fraction logistic (or sharp if
+ * width <= 0, 1/2 on the interface sites) in the signed distance to the nearest interface of the
+ * band 0 <= (p x + q y) mod M < M/2, M = gcd(p nx, q ny) / gcd(p, q) (= N for an N x N box). */
+int lbm_init_band(lbm_ctx* ctx, int case_id, int p, int q, double width);
+
+/* Copies populations to the host: LBM_STATE_POST_COLLISION gives what the reference holds in
+ * inif_R / inif_B between collide and swap (Imm_OPS.cpp:289-364), LBM_STATE_POST_STREAM what it
+ * holds after the swap. */
+int lbm_download_state(lbm_ctx* ctx, int case_id, int which, double* fR, double* fB);
+
+/* Advances every case by n_steps time steps (Imm_OPS.cpp:246-287 + swap :368-371), stream-ordered
+ * and asynchronous.  flags: LBM_STEP_RECORD_AUX keeps the last step's mediator momentum and
+ * class-F force for the diagnostics below. */
+int lbm_step(lbm_ctx* ctx, int n_steps, int flags);
+
+/* Advances one case only. */
+int lbm_step_case(lbm_ctx* ctx, int case_id, int n_steps, int flags);
+
+/* Waits for all queued work. */
+int lbm_sync(lbm_ctx* ctx);
+
+/* Queues all later work of ctx on the caller's HIP stream (a hipStream_t, default priority);
+ * NULL returns to the context's own stream.  Waits for work already queued first.  No
+ * reference counterpart: it lets a caller order the steps with its own work and time them. */
+int lbm_set_stream(lbm_ctx* ctx, void* hip_stream);
+
+/* The last recorded step's mediator momentum gm = -grad(phi) (momentum of inif_m) and force field
+ * (ini_force), each [site][3]; either pointer may be NULL. */
+int lbm_download_aux(lbm_ctx* ctx, int case_id, double* gm, double* F);
+
+/* Drop diagnostics on the post-collision state of the last step (needs LBM_STEP_RECORD_AUX for the
+ * force operators): calc_galilean + calc_perfil_logistico + modos_interface as Imm_OPS.cpp:295-300,
+ * with v0, step and the initial centre (x_ini, y_ini) as the reference passes them.  FP64,
+ * reduced on the device. */
+int lbm_diag_drop(lbm_ctx* ctx, int case_id, double v0, int step, double x_ini, double y_ini,
+                  lbm_drop_diag* out);
+
+/* Inclined-interface diagnostics (mede_inclinada) for normal (p, q); needs LBM_STEP_RECORD_AUX.
+ * profile (optional, NULL to skip) receives nx*ny rows of (d, c_R, rho, u_x, u_y) in site order,
+ * the values perfil.dat is written from (Imm_INCL.cpp:101,162). */
+int lbm_diag_incl(lbm_ctx* ctx, int case_id, int p, int q, lbm_incl_diag* out, double* profile);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

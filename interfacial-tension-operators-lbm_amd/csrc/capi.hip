@@ -1,0 +1,748 @@
+// C ABI (include/itolbm.h): context, device memory, state transfer, stepping and the host side
+// of the diagnostics.  Host arithmetic here is compiled with -ffp-contract=off and keeps the
+// reference's expressions, so the closing formulas of the diagnostics round as the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "diag_kernels.h"
+#include "itolbm.h"
+#include "lattice.h"
+#include "step_kernels.h"
+
+using namespace itolbm;
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+constexpr double CS2 = 1. / 3.;
+constexpr int DIAG_THREADS = 256;
+constexpr int N_ANG = 256;
+
+struct CaseState {
+    lbm_case_desc d{};
+    CaseConst cc{};
+    long long n = 0, S = 0;
+    double* pop[2] = {nullptr, nullptr};   // each [colour][19][S]
+    int cur = 0;                            // buffer holding the latest post-collision state
+    double* aux = nullptr;                  // [6][S] gm, F
+    bool aux_valid = false;
+    int ly = 8, nstrip = 1, nseg = 1;
+    // diagnostics scratch (allocated on first use)
+    double* gal = nullptr;
+    double* dsum = nullptr;
+    double* partial = nullptr;
+    double* hist = nullptr;
+    unsigned long long* histn = nullptr;
+    double* tab = nullptr;                  // cosx[nx] sinx[nx] cosy[ny] siny[ny] ct[256] st[256] r[256]
+    double* prof = nullptr;
+    int n_casca = 0;
+    int nblk = 1;
+};
+
+}  // namespace
+
+struct lbm_ctx {
+    std::vector<CaseState> cases;
+    std::string err;
+    hipStream_t stream = nullptr;      // stream in use
+    hipStream_t own_stream = nullptr;  // the context's own stream
+    long long bytes = 0;
+};
+
+namespace {
+
+int fail(lbm_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, LBM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+int dev_alloc(lbm_ctx* c, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(c, LBM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->bytes += (long long)bytes;
+    return LBM_OK;
+}
+
+bool valid_op(int op) { return op == OP_P1 || op == OP_P3 || op == OP_F1 || op == OP_F2 || op == OP_F3; }
+
+// Constants of one case, formed with the reference's expressions (coef_B Operadores_tensao.cpp:94-109,
+// calcula_forca :239-241, source LBM_functions.cpp:1641, coll_BGK Collision_operators.cpp:24).
+CaseConst make_const(const lbm_case_desc& d)
+{
+    CaseConst c{};
+    c.op = d.op;
+    c.nx = d.nx;
+    c.ny = d.ny;
+    c.one_over_tau = 1.0 / d.tau;
+    c.A = d.A;
+    if (d.op == OP_P1) {
+        c.B0 = W0 / 3.0; c.B1 = W1 / 3.0; c.B2 = W2 / 3.0;
+    } else {
+        const double chi = d.chi;
+        const double den = 6.0 * chi + 24.0;
+        c.B0 = -(2.0 + 2.0 * chi) / (3.0 * chi + 12.0);
+        c.B1 = chi / den;
+        c.B2 = 1.0 / den;
+    }
+    c.beta = d.beta;
+    const double sigma = d.A;
+    c.hs = 0.5 * sigma;
+    c.kf = 0.75 * sigma * d.xi;
+    c.bf = 3.0 * sigma / (8.0 * d.xi);
+    const double factor = (1. - 1. / (2. * d.tau));
+    c.sfac0 = factor * W0 * 3.0;
+    c.sfac1 = factor * W1 * 3.0;
+    c.sfac2 = factor * W2 * 3.0;
+    return c;
+}
+
+int default_rows(int nstrip, int ny)
+{
+    if (const char* e = std::getenv("ITOLBM_LY")) {
+        const int v = std::atoi(e);
+        if (v > 0) return v;
+    }
+    // aim at >= 2048 wavefronts (8 per CU) with segments of 4..64 rows
+    long long ly = (long long)nstrip * ny / 2048;
+    if (ly < 4) ly = 4;
+    if (ly > 64) ly = 64;
+    return (int)std::min<long long>(ly, std::max(ny, 1));
+}
+
+const int CXh[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+const int CYh[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1};
+
+inline long long nbr(int x, int y, int dx, int dy, int nx, int ny)
+{
+    int xx = (x + dx) % nx, yy = (y + dy) % ny;
+    if (xx < 0) xx += nx;
+    if (yy < 0) yy += ny;
+    return (long long)xx + (long long)nx * yy;
+}
+
+int check_case(lbm_ctx* c, int id)
+{
+    if (!c) return LBM_E_ARG;
+    if (id < 0 || id >= (int)c->cases.size()) return fail(c, LBM_E_ARG, "case id out of range");
+    return LBM_OK;
+}
+
+int ensure_aux(lbm_ctx* c, CaseState& k)
+{
+    if (k.aux) return LBM_OK;
+    int r = dev_alloc(c, (void**)&k.aux, sizeof(double) * 6 * (size_t)k.S);
+    if (r) return r;
+    HIPCHK(c, hipMemsetAsync(k.aux, 0, sizeof(double) * 6 * (size_t)k.S, c->stream));
+    return LBM_OK;
+}
+
+int ensure_diag(lbm_ctx* c, CaseState& k)
+{
+    if (k.gal) return LBM_OK;
+    const int nx = k.d.nx, ny = k.d.ny;
+    k.nblk = (int)std::min<long long>((k.n + DIAG_THREADS - 1) / DIAG_THREADS, 1024);
+    k.n_casca = (int)(std::sqrt((double)(nx * nx + ny * ny + 1)) / 0.5) + 2;
+    const int np = std::max(std::max(P1_NP, P2_NP), I1_NP);
+    int r;
+    if ((r = dev_alloc(c, (void**)&k.gal, sizeof(double) * 4 * (size_t)k.S))) return r;
+    if ((r = dev_alloc(c, (void**)&k.dsum, sizeof(double) * 2 * (size_t)k.S))) return r;
+    if ((r = dev_alloc(c, (void**)&k.partial, sizeof(double) * (size_t)k.nblk * np))) return r;
+    if ((r = dev_alloc(c, (void**)&k.hist, sizeof(double) * (size_t)k.n_casca))) return r;
+    if ((r = dev_alloc(c, (void**)&k.histn, sizeof(unsigned long long) * (size_t)k.n_casca))) return r;
+    const size_t ntab = 2 * (size_t)nx + 2 * (size_t)ny + 3 * N_ANG;
+    if ((r = dev_alloc(c, (void**)&k.tab, sizeof(double) * ntab))) return r;
+    // trigonometric tables from the host's libm, as the reference evaluates them
+    // (Other_functions.cpp:72-73,175-176; Operadores_tensao.cpp:419)
+    std::vector<double> t(ntab, 0.0);
+    const double tx = 2.0 * PI / (double)nx, ty = 2.0 * PI / (double)ny;
+    for (int x = 0; x < nx; ++x) { t[x] = std::cos(tx * x); t[nx + x] = std::sin(tx * x); }
+    for (int y = 0; y < ny; ++y) { t[2 * nx + y] = std::cos(ty * y); t[2 * nx + ny + y] = std::sin(ty * y); }
+    for (int kk = 0; kk < N_ANG; ++kk) {
+        const double th = 2.0 * M_PI * kk / N_ANG;
+        t[2 * nx + 2 * ny + kk] = std::cos(th);
+        t[2 * nx + 2 * ny + N_ANG + kk] = std::sin(th);
+    }
+    HIPCHK(c, hipMemcpy(k.tab, t.data(), sizeof(double) * ntab, hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+template <int NP>
+int reduce_partials(lbm_ctx* c, const CaseState& k, std::vector<double>& out, int ns, int nxm, int nn)
+{
+    std::vector<double> h((size_t)k.nblk * NP);
+    HIPCHK(c, hipMemcpyAsync(h.data(), k.partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out.assign(NP, 0.0);
+    for (int j = 0; j < ns; ++j) {
+        double s = 0.0;
+        for (int b = 0; b < k.nblk; ++b) s = s + h[(size_t)b * NP + j];
+        out[j] = s;
+    }
+    for (int j = ns; j < ns + nxm; ++j) {
+        double m = h[j];
+        for (int b = 1; b < k.nblk; ++b) { const double v = h[(size_t)b * NP + j]; m = (m < v) ? v : m; }
+        out[j] = m;
+    }
+    for (int j = ns + nxm; j < ns + nxm + nn; ++j) {
+        double m = h[j];
+        for (int b = 1; b < k.nblk; ++b) { const double v = h[(size_t)b * NP + j]; m = (v < m) ? v : m; }
+        out[j] = m;
+    }
+    return LBM_OK;
+}
+
+inline double gi_image(double dx, double length) { return dx - length * std::floor(dx / length + 0.5); }
+
+long long gcdll(long long a, long long b)
+{
+    if (a < 0) a = -a;
+    if (b < 0) b = -b;
+    while (b) { const long long t = a % b; a = b; b = t; }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_last_error(const lbm_ctx* ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->err.c_str();
+}
+
+long long lbm_device_bytes(const lbm_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx** out)
+{
+    if (!out) return LBM_E_ARG;
+    *out = nullptr;
+    if (!cases || n_cases < 1) return LBM_E_ARG;
+    if (precision != LBM_PREC_F64) return LBM_E_ARG;
+    lbm_ctx* c = new (std::nothrow) lbm_ctx();
+    if (!c) return LBM_E_NOMEM;
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) {
+        delete c;
+        return LBM_E_HIP;
+    }
+    if (hipStreamCreate(&c->own_stream) != hipSuccess) {
+        delete c;
+        return LBM_E_HIP;
+    }
+    c->stream = c->own_stream;
+    c->cases.resize(n_cases);
+    for (int i = 0; i < n_cases; ++i) {
+        const lbm_case_desc& d = cases[i];
+        if (d.nx < 1 || d.ny < 1 || !valid_op(d.op) || !(d.tau > 0.0) || !std::isfinite(d.A) ||
+            !std::isfinite(d.beta) || (d.op == OP_F3 && !(d.xi != 0.0)) ||
+            (long long)d.nx * d.ny > (1LL << 31) / Q) {
+            lbm_destroy(c);
+            return LBM_E_ARG;
+        }
+        CaseState& k = c->cases[i];
+        k.d = d;
+        k.cc = make_const(d);
+        k.n = (long long)d.nx * d.ny;
+        k.S = (k.n + 63) / 64 * 64;
+        int ns, nseg;
+        step_tile_geometry(d.op, d.nx, d.ny, 1, &ns, &nseg);
+        k.ly = default_rows(ns, d.ny);
+        step_tile_geometry(d.op, d.nx, d.ny, k.ly, &k.nstrip, &k.nseg);
+        for (int b = 0; b < 2; ++b) {
+            if (dev_alloc(c, (void**)&k.pop[b], sizeof(double) * 2 * Q * (size_t)k.S) ||
+                hipMemsetAsync(k.pop[b], 0, sizeof(double) * 2 * Q * (size_t)k.S, c->stream) != hipSuccess) {
+                lbm_destroy(c);
+                return LBM_E_NOMEM;
+            }
+        }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        lbm_destroy(c);
+        return LBM_E_HIP;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx* c)
+{
+    if (!c) return LBM_E_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (CaseState& k : c->cases) {
+        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.partial, k.hist, k.tab, k.prof})
+            if (p) (void)hipFree(p);
+        if (k.histn) (void)hipFree(k.histn);
+    }
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return LBM_OK;
+}
+
+int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
+{
+    if (int r = check_case(c, id)) return r;
+    if (!fR || !fB) return fail(c, LBM_E_ARG, "null population array");
+    CaseState& k = c->cases[id];
+    const int nx = k.d.nx, ny = k.d.ny;
+    std::vector<double> h(2 * Q * (size_t)k.S, 0.0);
+    // device holds "post-collision at own site": f*_i(x) = f_i(x + c_i) for a post-stream state f
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            const long long site = (long long)x + (long long)nx * y;
+            for (int i = 0; i < Q; ++i) {
+                const long long src = nbr(x, y, CXh[i], CYh[i], nx, ny);
+                h[(size_t)i * k.S + site] = fR[(size_t)src * Q + i];
+                h[(size_t)(Q + i) * k.S + site] = fB[(size_t)src * Q + i];
+            }
+        }
+    HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    k.aux_valid = false;
+    return LBM_OK;
+}
+
+namespace {
+
+// dist_eq (LBM_functions.cpp:924-944), host side
+void host_feq(double* feq, double vx, double vy, double vz, double rho)
+{
+    static const double Ch[Q][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {0, -1, 0},
+                                    {0, 0, 1},  {0, 0, -1},  {1, 1, 0},  {-1, -1, 0}, {1, -1, 0},
+                                    {-1, 1, 0}, {1, 0, 1},   {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1},
+                                    {0, -1, -1}, {0, 1, 1},  {0, -1, 1}, {0, 1, -1}};
+    const double v[3] = {vx, vy, vz};
+    const double vquad = (vx * vx + vy * vy + vz * vz);
+    const double o = 3.0;
+    for (int i = 0; i < Q; i++) {
+        double cv = 0.0;
+        for (int a = 0; a < 3; a++) cv = cv + Ch[i][a] * v[a];
+        const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+        feq[i] = w * rho * (1.0 + cv * o + 0.5 * cv * cv * o * o - 0.5 * vquad * o);
+    }
+}
+
+}  // namespace
+
+int lbm_init_drop(lbm_ctx* c, int id, double R, double width, int x0, int y0, double v0)
+{
+    if (int r = check_case(c, id)) return r;
+    if (!std::isfinite(R) || !std::isfinite(v0) || !std::isfinite(width)) return fail(c, LBM_E_ARG, "non-finite argument");
+    const CaseState& k = c->cases[id];
+    const int nx = k.d.nx, ny = k.d.ny;
+    const double k_perfil = (width > 0.0) ? 2.0 * std::log(9.0) / width : -1.0;
+    const double rho_ini_R = 1.0, rho_ini_B = 1.0, delta_rho = 0.0;
+    std::vector<double> fR((size_t)k.n * Q), fB((size_t)k.n * Q);
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++) {
+            const double dx = (double)(x - x0), dy = (double)(y - y0), dz = 0.0;
+            const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            double phi;
+            if (k_perfil > 0.0) {
+                double arg = k_perfil * (r - R);
+                if (arg > 60.0) arg = 60.0;
+                if (arg < -60.0) arg = -60.0;
+                phi = 1.0 / (1.0 + std::exp(arg));
+            } else {
+                phi = (r < R) ? 1.0 : 0.0;
+            }
+            const double rho_tot = rho_ini_B + phi * (rho_ini_R - rho_ini_B) + phi * delta_rho;
+            const size_t p = (size_t)x + (size_t)nx * y;
+            host_feq(&fR[p * Q], v0, 0., 0., phi * rho_tot);
+            host_feq(&fB[p * Q], v0, 0., 0., (1.0 - phi) * rho_tot);
+        }
+    return lbm_upload_state(c, id, fR.data(), fB.data());
+}
+
+int lbm_init_band(lbm_ctx* c, int id, int p, int q, double width)
+{
+    if (int r = check_case(c, id)) return r;
+    if (p == 0 && q == 0) return fail(c, LBM_E_ARG, "p = q = 0");
+    const CaseState& k = c->cases[id];
+    const int nx = k.d.nx, ny = k.d.ny;
+    const long long M = gcdll((long long)p * nx, (long long)q * ny) / gcdll(p, q);
+    const double m = std::sqrt((double)(p * p + q * q));
+    const double kk = (width > 0) ? 2.0 * std::log(9.0) / width : -1.0;
+    std::vector<double> fR((size_t)k.n * Q), fB((size_t)k.n * Q);
+    for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+            long long s = ((long long)p * x + (long long)q * y) % M;
+            if (s < 0) s += M;
+            const double a = (double)s, b = 0.5 * (double)M - (double)s;
+            const double c1 = (double)s - 0.5 * (double)M, c2 = (double)(M - s);
+            const double dred = (2 * s < M) ? std::min(a, b) / m : -std::min(c1, c2) / m;
+            double cR;
+            if (kk > 0) cR = 1.0 / (1.0 + std::exp(-kk * dred));
+            else cR = (dred > 0) ? 1.0 : (dred < 0 ? 0.0 : 0.5);
+            const size_t pto = (size_t)x + (size_t)nx * y;
+            for (int i = 0; i < Q; i++) {
+                const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+                fR[pto * Q + i] = w * cR;
+                fB[pto * Q + i] = w * (1.0 - cR);
+            }
+        }
+    return lbm_upload_state(c, id, fR.data(), fB.data());
+}
+
+int lbm_download_state(lbm_ctx* c, int id, int which, double* fR, double* fB)
+{
+    if (int r = check_case(c, id)) return r;
+    if (!fR || !fB) return fail(c, LBM_E_ARG, "null population array");
+    if (which != LBM_STATE_POST_STREAM && which != LBM_STATE_POST_COLLISION)
+        return fail(c, LBM_E_ARG, "bad state selector");
+    CaseState& k = c->cases[id];
+    const int nx = k.d.nx, ny = k.d.ny;
+    std::vector<double> h(2 * Q * (size_t)k.S);
+    HIPCHK(c, hipMemcpyAsync(h.data(), k.pop[k.cur], sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            const long long site = (long long)x + (long long)nx * y;
+            for (int i = 0; i < Q; ++i) {
+                const long long src = (which == LBM_STATE_POST_STREAM) ? nbr(x, y, -CXh[i], -CYh[i], nx, ny) : site;
+                fR[(size_t)site * Q + i] = h[(size_t)i * k.S + src];
+                fB[(size_t)site * Q + i] = h[(size_t)(Q + i) * k.S + src];
+            }
+        }
+    return LBM_OK;
+}
+
+int lbm_step_case(lbm_ctx* c, int id, int n_steps, int flags)
+{
+    if (int r = check_case(c, id)) return r;
+    if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
+    if (n_steps == 0) return LBM_OK;
+    CaseState& k = c->cases[id];
+    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    if (rec) {
+        if (int r = ensure_aux(c, k)) return r;
+    }
+    for (int s = 0; s < n_steps; ++s) {
+        StepArgs a{};
+        a.inR = k.pop[k.cur];
+        a.inB = k.pop[k.cur] + Q * k.S;
+        a.outR = k.pop[1 - k.cur];
+        a.outB = k.pop[1 - k.cur] + Q * k.S;
+        a.aux = (rec && s == n_steps - 1) ? k.aux : nullptr;
+        a.S = k.S;
+        a.ly = k.ly;
+        a.nstrip = k.nstrip;
+        a.nseg = k.nseg;
+        HIPCHK(c, launch_step_f64(k.cc, a, c->stream));
+        k.cur ^= 1;
+    }
+    k.aux_valid = rec;
+    return LBM_OK;
+}
+
+int lbm_step(lbm_ctx* c, int n_steps, int flags)
+{
+    if (!c) return LBM_E_ARG;
+    for (int i = 0; i < (int)c->cases.size(); ++i)
+        if (int r = lbm_step_case(c, i, n_steps, flags)) return r;
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_ctx* c)
+{
+    if (!c) return LBM_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LBM_OK;
+}
+
+int lbm_set_stream(lbm_ctx* c, void* s)
+{
+    if (!c) return LBM_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return LBM_OK;
+}
+
+int lbm_download_aux(lbm_ctx* c, int id, double* gm, double* F)
+{
+    if (int r = check_case(c, id)) return r;
+    CaseState& k = c->cases[id];
+    if (!k.aux_valid) return fail(c, LBM_E_STATE, "no recorded step (use LBM_STEP_RECORD_AUX)");
+    std::vector<double> h(6 * (size_t)k.S);
+    HIPCHK(c, hipMemcpyAsync(h.data(), k.aux, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (long long p = 0; p < k.n; ++p)
+        for (int j = 0; j < 3; ++j) {
+            if (gm) gm[3 * p + j] = h[(size_t)j * k.S + p];
+            if (F) F[3 * p + j] = h[(size_t)(3 + j) * k.S + p];
+        }
+    return LBM_OK;
+}
+
+int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double y_ini, lbm_drop_diag* o)
+{
+    if (int r = check_case(c, id)) return r;
+    if (!o) return fail(c, LBM_E_ARG, "null output");
+    CaseState& k = c->cases[id];
+    const bool forced = k.d.op >= OP_F1;
+    if (forced && !k.aux_valid) return fail(c, LBM_E_STATE, "force operators need a step with LBM_STEP_RECORD_AUX");
+    if (int r = ensure_diag(c, k)) return r;
+    std::memset(o, 0, sizeof(*o));
+    const int nx = k.d.nx, ny = k.d.ny;
+    const double* fR = k.pop[k.cur];
+    const double* fB = k.pop[k.cur] + Q * k.S;
+    const double* F = forced ? k.aux + 3 * k.S : nullptr;
+
+    // ---- pass 1
+    DropPass1Args a1{};
+    a1.fR = fR; a1.fB = fB; a1.S = k.S; a1.nx = nx; a1.ny = ny; a1.F = F;
+    a1.cosx = k.tab; a1.sinx = k.tab + nx; a1.cosy = k.tab + 2 * nx; a1.siny = k.tab + 2 * nx + ny;
+    a1.gal = k.gal; a1.dsum = k.dsum; a1.partial = k.partial;
+    HIPCHK(c, launch_drop_pass1(a1, k.nblk, c->stream));
+    std::vector<double> r1;
+    if (int r = reduce_partials<P1_NP>(c, k, r1, P1_NS, P1_NX, P1_NN)) return r;
+    const double n_fluid = r1[P1_NFLUID];
+    const double tx = 2.0 * PI / (double)nx, ty = 2.0 * PI / (double)ny;
+    double x0 = std::atan2(r1[P1_CXS], r1[P1_CXC]) / tx;
+    if (x0 < 0.0) x0 += nx;
+    double y0 = std::atan2(r1[P1_CYS], r1[P1_CYC]) / ty;
+    if (y0 < 0.0) y0 += ny;
+    const double area = r1[P1_AREA];
+    const double raio = (area > 0.0) ? std::sqrt(area / (PI * (double)1)) : 0.0;
+    o->min_f = r1[P1_NS + P1_NX + 0];
+    o->min_rho = r1[P1_NS + P1_NX + 1];
+    o->min_phi = r1[P1_NS + P1_NX + 2];
+    o->max_phi = r1[P1_NS + 1];
+    o->negative_total = (long long)r1[P1_NEGT];
+    o->negative_color = (long long)r1[P1_NEGC];
+    o->axis_ratio = 1;
+    const double bad = r1[P1_BAD], mass = r1[P1_MASS], mass_r = r1[P1_MASSR];
+    o->n_ruim = (int)bad;
+    if (bad > 0 || !(mass > 0) || !(mass_r > 0)) {
+        o->ok = 0;
+        return LBM_OK;
+    }
+    o->mean_x = r1[P1_QX] / mass; o->mean_y = r1[P1_QY] / mass; o->mean_z = r1[P1_QZ] / mass;
+    o->drop_x = r1[P1_QRX] / mass_r; o->drop_y = r1[P1_QRY] / mass_r;
+
+    // ---- pass 2
+    const double r_in_max = 0.5 * raio;
+    double r_out_min = 2.0 * raio;
+    if (r_out_min < raio + 4.0 * 3.45) r_out_min = raio + 4.0 * 3.45;
+    const double r_caixa = 0.45 * (double)((nx < ny) ? nx : ny);
+    if (r_out_min > r_caixa) r_out_min = r_caixa;
+    int y_lin = (int)(y0 + 0.5);
+    if (y_lin >= ny) y_lin -= ny;
+    const int y_line = ((int)std::floor(y0 + 0.5)) % ny;
+    const int corner_x = ((int)std::floor(x0 + 0.5) + nx / 2) % nx;
+    const int corner_y = ((int)std::floor(y0 + 0.5) + ny / 2) % ny;
+    DropPass2Args a2{};
+    a2.fR = fR; a2.fB = fB; a2.S = k.S; a2.nx = nx; a2.ny = ny; a2.gal = k.gal; a2.dsum = k.dsum;
+    a2.x0 = x0; a2.y0 = y0; a2.gx0 = x0; a2.gy0 = y0; a2.y_lin = y_lin; a2.y_line = y_line;
+    a2.r_in_max = r_in_max; a2.r_out_min = r_out_min;
+    a2.v0 = v0; a2.mean_x = o->mean_x; a2.mean_y = o->mean_y; a2.mean_z = o->mean_z;
+    a2.pc = corner_x + (long long)nx * corner_y;
+    a2.n_casca = k.n_casca; a2.hist = k.hist; a2.histn = k.histn; a2.partial = k.partial;
+    HIPCHK(c, hipMemsetAsync(k.hist, 0, sizeof(double) * k.n_casca, c->stream));
+    HIPCHK(c, hipMemsetAsync(k.histn, 0, sizeof(unsigned long long) * k.n_casca, c->stream));
+    HIPCHK(c, launch_drop_pass2(a2, k.nblk, c->stream));
+    std::vector<double> r2;
+    if (int r = reduce_partials<P2_NP>(c, k, r2, P2_NS, P2_NX, 0)) return r;
+
+    // calc_drop_radius closing (Other_functions.cpp:253-273)
+    o->x0 = x0; o->y0 = y0; o->area = area; o->raio = raio;
+    o->raio_grad = (r2[P2_SOMW] > 0.0) ? r2[P2_SOMWR] / r2[P2_SOMW] : 0.0;
+    o->massa_R = r1[P1_MASSA_R]; o->massa_B = r1[P1_MASSA_B];
+    o->qx = r1[P1_QXD]; o->qy = r1[P1_QYD]; o->qz = r1[P1_QZD];
+    // calc_laplace_tension closing (Other_functions.cpp:530-551)
+    const double n_in = r2[P2_NIN], n_out = r2[P2_NOUT], n_ctr = r2[P2_NCTR];
+    const double p_in = (n_in > 0) ? r2[P2_PIN] / n_in : 0.0;
+    const double p_out = (n_out > 0) ? r2[P2_POUT] / n_out : 0.0;
+    const double var_in = (n_in > 0) ? r2[P2_PIN2] / n_in - p_in * p_in : 0.0;
+    const double var_out = (n_out > 0) ? r2[P2_POUT2] / n_out - p_out * p_out : 0.0;
+    o->p_in = p_in; o->sd_in = (var_in > 0.0) ? std::sqrt(var_in) : 0.0;
+    o->p_out = p_out; o->sd_out = (var_out > 0.0) ? std::sqrt(var_out) : 0.0;
+    o->n_in = (int)n_in; o->n_out = (int)n_out;
+    o->delta_p = p_in - p_out;
+    o->sigma = 1.0 * raio * o->delta_p;
+    o->p_centro = (n_ctr > 0) ? r2[P2_PCTR] / n_ctr : 0.0;
+    // calc_galilean closing (Galilean_diagnostics.cpp:88-104)
+    const double nn = (double)k.n;
+    o->max_ref = r2[P2_NS + 0]; o->max_mean = r2[P2_NS + 1]; o->max_lab = r2[P2_NS + 2];
+    o->rms_ref = std::sqrt(r2[P2_SREF] / nn);
+    o->rms_mean = std::sqrt(r2[P2_SMEAN] / nn);
+    o->rms_lab = std::sqrt(r2[P2_SLAB] / nn);
+    const double weight = r2[P2_WEIGHT], sx = r2[P2_SX], sy = r2[P2_SY];
+    const double xx = r2[P2_SXX] / weight - (sx / weight) * (sx / weight);
+    const double yy = r2[P2_SYY] / weight - (sy / weight) * (sy / weight);
+    const double xy = r2[P2_SXY] / weight - (sx / weight) * (sy / weight), trace = xx + yy;
+    const double disc = std::sqrt((xx - yy) * (xx - yy) + 4 * xy * xy);
+    o->axis_ratio = (trace > disc) ? std::sqrt((trace + disc) / (trace - disc)) : INFINITY;
+    o->center_dx = gi_image(x0 - (x_ini + v0 * step), nx);
+    o->center_dy = gi_image(y0 - y_ini, ny);
+    o->x_unwrapped = x_ini + v0 * step + o->center_dx;
+    const double lw = r2[P2_LW], rw = r2[P2_RW];
+    o->raio_linha = (lw > 0 && rw > 0) ? 0.5 * (r2[P2_RX] / rw - r2[P2_LX] / lw) : 0;
+    o->p_canto = r2[P2_PCANTO];
+    o->sigma_linha = o->raio_linha * (o->p_centro - o->p_canto);
+    o->u_max = o->max_ref;
+    o->u_rms = o->rms_ref;
+    o->ok = std::isfinite(o->sigma) && std::isfinite(o->max_ref) && std::isfinite(o->axis_ratio);
+    (void)n_fluid;
+    if (!o->ok) return LBM_OK;
+
+    // ---- calc_perfil_logistico closing (Other_functions.cpp:381-445), from the shell histogram
+    {
+        std::vector<double> hs(k.n_casca);
+        std::vector<unsigned long long> hn(k.n_casca);
+        HIPCHK(c, hipMemcpyAsync(hs.data(), k.hist, sizeof(double) * k.n_casca, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(hn.data(), k.histn, sizeof(unsigned long long) * k.n_casca, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double eps = 1.0e-30, d_casca = 0.5;
+        double S0 = 0.0, Sr = 0.0, Sl = 0.0, Srr = 0.0, Srl = 0.0;
+        int n_usadas = 0;
+        for (int kk = 0; kk < k.n_casca; kk++) {
+            const double sn = (double)hn[kk];
+            if (sn < 1.0) continue;
+            const double phi = hs[kk] / sn;
+            if (phi <= 0.02 || phi >= 0.98) continue;
+            const double r = ((double)kk + 0.5) * d_casca;
+            const double l = std::log(phi / (1.0 - phi));
+            S0 += 1.0; Sr += r; Sl += l; Srr += r * r; Srl += r * l;
+            n_usadas++;
+        }
+        double K = 0.0, r_meio = 0.0, res_max = 0.0;
+        if (n_usadas >= 3) {
+            const double den = S0 * Srr - Sr * Sr;
+            if (std::fabs(den) > eps) {
+                const double a = (S0 * Srl - Sr * Sl) / den;
+                const double b = (Sl - a * Sr) / S0;
+                K = -a;
+                if (std::fabs(a) > eps) r_meio = -b / a;
+                for (int kk = 0; kk < k.n_casca; kk++) {
+                    const double sn = (double)hn[kk];
+                    if (sn < 1.0) continue;
+                    const double phi = hs[kk] / sn;
+                    if (phi <= 0.02 || phi >= 0.98) continue;
+                    const double r = ((double)kk + 0.5) * d_casca;
+                    const double dd = std::log(phi / (1.0 - phi)) - (a * r + b);
+                    if (std::fabs(dd) > res_max) res_max = std::fabs(dd);
+                }
+            }
+        }
+        o->K_perfil = K;
+        o->r_perfil = r_meio;
+        o->res_perfil = res_max;
+        o->esp_perfil = (K > eps) ? 2.0 * std::log(9.0) / K : 0.0;
+        o->n_perfil = n_usadas;
+    }
+
+    // ---- modos_interface (Operadores_tensao.cpp:382-449)
+    {
+        double* ct = k.tab + 2 * nx + 2 * ny;
+        ModesArgs am{};
+        am.dsum = k.dsum; am.S = k.S; am.nx = nx; am.ny = ny;
+        am.x0 = x0; am.y0 = y0; am.R_est = raio;
+        am.ct = ct; am.st = ct + N_ANG; am.r = ct + 2 * N_ANG;
+        HIPCHK(c, launch_drop_modes(am, c->stream));
+        std::vector<double> rr(N_ANG);
+        HIPCHK(c, hipMemcpyAsync(rr.data(), am.r, sizeof(double) * N_ANG, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double c2 = 0, s2 = 0, c4 = 0, s4 = 0, c8 = 0, s8 = 0, soma = 0, rmin = 1e30, rmax = 0;
+        for (int kk = 0; kk < N_ANG; kk++) {
+            const double th = 2.0 * M_PI * kk / N_ANG;
+            const double r = rr[kk];
+            soma += r;
+            c2 += r * std::cos(2 * th); s2 += r * std::sin(2 * th);
+            c4 += r * std::cos(4 * th); s4 += r * std::sin(4 * th);
+            c8 += r * std::cos(8 * th); s8 += r * std::sin(8 * th);
+            rmin = (r < rmin) ? r : rmin;
+            rmax = (rmax < r) ? r : rmax;
+        }
+        o->R0 = soma / N_ANG;
+        o->a2 = 2.0 * std::sqrt(c2 * c2 + s2 * s2) / N_ANG / o->R0;
+        o->a4 = 2.0 * std::sqrt(c4 * c4 + s4 * s4) / N_ANG / o->R0;
+        o->a8 = 2.0 * std::sqrt(c8 * c8 + s8 * s8) / N_ANG / o->R0;
+        o->rmin = rmin;
+        o->rmax = rmax;
+    }
+    return LBM_OK;
+}
+
+int lbm_diag_incl(lbm_ctx* c, int id, int p, int q, lbm_incl_diag* o, double* profile)
+{
+    if (int r = check_case(c, id)) return r;
+    if (!o) return fail(c, LBM_E_ARG, "null output");
+    if (p == 0 && q == 0) return fail(c, LBM_E_ARG, "p = q = 0");
+    CaseState& k = c->cases[id];
+    if (!k.aux_valid) return fail(c, LBM_E_STATE, "needs a step with LBM_STEP_RECORD_AUX");
+    if (int r = ensure_diag(c, k)) return r;
+    if (profile && !k.prof) {
+        if (int r = dev_alloc(c, (void**)&k.prof, sizeof(double) * 5 * (size_t)k.S)) return r;
+    }
+    std::memset(o, 0, sizeof(*o));
+    const int nx = k.d.nx, ny = k.d.ny;
+    const long long M = gcdll((long long)p * nx, (long long)q * ny) / gcdll(p, q);
+    const double m = std::sqrt((double)(p * p + q * q));
+    InclArgs a{};
+    a.fR = k.pop[k.cur]; a.fB = k.pop[k.cur] + Q * k.S; a.S = k.S; a.nx = nx; a.ny = ny;
+    a.F = (k.d.op >= OP_F1) ? k.aux + 3 * k.S : nullptr;
+    a.gm = k.aux;
+    a.p = p; a.q = q; a.M = M; a.m = m;
+    a.nxn = p / m; a.nyn = q / m; a.txn = -a.nyn; a.tyn = a.nxn;
+    a.fit_pass = 0;
+    a.prof = profile ? k.prof : nullptr;
+    a.partial = k.partial;
+    HIPCHK(c, launch_incl_pass(a, k.nblk, c->stream));
+    std::vector<double> r1;
+    if (int r = reduce_partials<I1_NP>(c, k, r1, I1_NS, I1_NX, I1_NN)) return r;
+    const double comprimento = 2.0 * ((double)nx * (double)ny / (double)M) * m;
+    o->n_ruim = (int)r1[I1_NRUIM];
+    o->neg_total = (long long)r1[I1_NEGT];
+    o->neg_cor = (long long)r1[I1_NEGC];
+    o->min_f = r1[I1_NS + I1_NX];
+    o->SFx = r1[I1_SFX]; o->SFy = r1[I1_SFY];
+    o->massa_R = r1[I1_MR]; o->massa_B = r1[I1_MB]; o->qx = r1[I1_QX]; o->qy = r1[I1_QY];
+    o->u_max = r1[I1_NS + 0]; o->un_max = r1[I1_NS + 1]; o->ut_max = r1[I1_NS + 2];
+    o->sigma_pop = r1[I1_SIGMA] / comprimento;
+    o->int_grad = r1[I1_IGRAD] / comprimento;
+    o->u_rms = std::sqrt(r1[I1_U2] / ((double)nx * (double)ny));
+    const int nfit = (int)r1[I1_NFIT];
+    o->n_ajuste = nfit;
+    if (nfit > 3) {
+        const double Sx = r1[I1_SX], Sy = r1[I1_SY], Sxx = r1[I1_SXX], Sxy = r1[I1_SXY];
+        const double K = (nfit * Sxy - Sx * Sy) / (nfit * Sxx - Sx * Sx);
+        const double aa = (Sy - K * Sx) / nfit;
+        o->K = K;
+        o->d0 = -aa / K;
+        a.fit_pass = 1; a.fa = aa; a.fK = K; a.prof = nullptr;
+        HIPCHK(c, launch_incl_pass(a, k.nblk, c->stream));
+        std::vector<double> r2;
+        if (int r = reduce_partials<I1_NP>(c, k, r2, I1_NS, I1_NX, I1_NN)) return r;
+        o->res = r2[I1_NS + 3];
+    }
+    o->ok = (o->n_ruim == 0) && std::isfinite(o->u_max);
+    if (profile) {
+        std::vector<double> h(5 * (size_t)k.S);
+        HIPCHK(c, hipMemcpyAsync(h.data(), k.prof, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (long long s = 0; s < k.n; ++s)
+            for (int j = 0; j < 5; ++j) profile[5 * s + j] = h[(size_t)j * k.S + s];
+    }
+    return LBM_OK;
+}
+
+}  // extern "C"

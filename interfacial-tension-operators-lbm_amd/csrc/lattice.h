@@ -1,0 +1,42 @@
+// D3Q19 lattice constants and shared parameter blocks for the device kernels.
+//
+// Ordering and weights follow the reference lattice (LBM_functions.cpp:366-497):
+//   i = 0 rest; 1..6 = +x -x +y -y +z -z; 7..18 = (1,1,0) (-1,-1,0) (1,-1,0) (-1,1,0) (1,0,1)
+//   (-1,0,-1) (1,0,-1) (-1,0,1) (0,-1,-1) (0,1,1) (0,-1,1) (0,1,-1);  w = 1/3, 1/18, 1/36.
+// With nz = 1 the z component of every velocity wraps onto the same plane, so a population
+// streams by (cx, cy) only.
+#pragma once
+
+#include <cstdint>
+
+namespace itolbm {
+
+constexpr int Q = 19;
+
+// HIP host and device both see these tables (constexpr, folded at compile time).
+constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1};
+constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1};
+
+constexpr double W0 = 1. / 3.;
+constexpr double W1 = 1. / 18.;
+constexpr double W2 = 1. / 36.;
+
+// operator ids of Operadores_tensao.cpp:45-49
+enum OpId { OP_P1 = 1, OP_P3 = 3, OP_F1 = 11, OP_F2 = 12, OP_F3 = 13 };
+
+// Everything one case's step needs on the device, computed once on the host so that every
+// constant is the same double the reference forms at run time.
+struct CaseConst {
+    int op;            // OpId
+    int nx, ny;
+    double one_over_tau;   // 1.0 / tau            (Collision_operators.cpp:24)
+    double A;              // fat_R_B              (P amplitude; sigma for F)
+    double B0, B1, B2;     // coef_B by |c|^2 = 0, 1, 2   (Operadores_tensao.cpp:94-109)
+    double beta;           // recoll
+    double hs;             // 0.5 * sigma          (F1 / F2)
+    double kf, bf;         // 0.75 sigma xi, 3 sigma / (8 xi)   (F3)
+    double sfac0, sfac1, sfac2;   // (1 - 1/(2 tau)) * w * 3   (Guo source, LBM_functions.cpp:1641-1669)
+};
+
+}  // namespace itolbm

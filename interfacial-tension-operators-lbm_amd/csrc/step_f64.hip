@@ -1,0 +1,519 @@
+// FP64 fused time step: pull-stream + colour field + mediator gradient + tension (P or F class)
+// + BGK (Guo forcing) + Latva-Kokko-Rothman recolouring, one kernel launch per step.
+//
+// Reference path: Imm_OPS.cpp:246-372 (mediator pass :252-267, class-F force :271, collide +
+// push-stream :275-287), with the per-site functions colisao_comum (Operadores_tensao.cpp:340-373),
+// coll_BGK (Collision_operators.cpp:11-49), calcula / dist_eq / source / recolloring
+// (LBM_functions.cpp:1099-1171, 889-945, 1602-1673, 1712-1746), perturbacao / coef_B
+// (Operadores_tensao.cpp:94-133) and calcula_forca (Operadores_tensao.cpp:202-332).
+//
+// Layout: populations of each colour are structure-of-arrays, f[i * S + x + nx * y], and hold the
+// POST-COLLISION values of the previous step at their own site.  The reference's push
+// (f_new[x + c_i] = f*(x)) is done here as a pull: site x reads f*_i(x - c_i).
+//
+// Work decomposition: one 64-lane wavefront owns a vertical strip of 64 consecutive columns, of
+// which the H edge lanes on each side are halo (H = 1 for P1/P3/F3, 2 for F1/F2), and marches down
+// `ly` rows.  Each lane pulls the 38 populations of its own column once per row; the colour field of
+// the rows in flight stays in registers and x-neighbours come from wavefront shuffles, so each
+// population is read from HBM once (halo columns/rows are re-read through L2) and written once.
+//
+// Arithmetic: every double expression keeps the reference's order of evaluation, so with
+// -ffp-contract=off the results equal the reference's bit for bit.  Terms whose lattice
+// coefficient is 0 are left out of the running sums: the reference adds them to sums that start at
+// +0.0, where adding a signed zero never changes the value.
+#include <hip/hip_runtime.h>
+
+#include "lattice.h"
+#include "step_kernels.h"
+
+namespace itolbm {
+
+namespace {
+
+constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0), correctly rounded
+
+__device__ __forceinline__ int wrapi(int v, int n)
+{
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of tiles so that
+// tiles sharing halo rows/columns also share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, idx = b >> 3;
+    return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+struct Lane {
+    int col, xm, xp;      // own column and its periodic x-neighbours
+    int lane;
+    bool interior;        // owns (stores) its column
+};
+
+// Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).
+__device__ __forceinline__ void load_row(const double* __restrict__ pR, const double* __restrict__ pB,
+                                         long long S, int nx, int ny, int y, const Lane& L,
+                                         double fR[Q], double fB[Q])
+{
+    const int rm = wrapi(y - 1, ny) * nx, r0 = wrapi(y, ny) * nx, rp = wrapi(y + 1, ny) * nx;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
+        const int rs = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
+        const long long idx = (long long)i * S + rs + xs;
+        fR[i] = pR[idx];
+        fB[i] = pB[idx];
+    }
+}
+
+// density (LBM_functions.cpp:1581-1590): sequential sum
+__device__ __forceinline__ double dens(const double f[Q])
+{
+    double r = f[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) r = r + f[i];
+    return r;
+}
+
+__device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }
+
+// c_i . a with the reference's left-to-right order, zero-coefficient terms left out
+template <int i>
+__device__ __forceinline__ double cdot(double a0, double a1, double a2)
+{
+    constexpr int c0 = CX[i], c1 = CY[i], c2 = CZ[i];
+    double s = 0.0;
+    bool first = true;
+    if constexpr (c0 != 0) { s = (c0 > 0) ? a0 : -a0; first = false; }
+    if constexpr (c1 != 0) { const double t = (c1 > 0) ? a1 : -a1; s = first ? t : s + t; first = false; }
+    if constexpr (c2 != 0) { const double t = (c2 > 0) ? a2 : -a2; s = first ? t : s + t; first = false; }
+    return s;
+}
+
+// Nine values of a field around a lane: v[dy + 1][dx + 1] = field(x + dx, y + dy).
+struct Nine {
+    double v[3][3];
+    __device__ __forceinline__ double at(int dx, int dy) const { return v[dy + 1][dx + 1]; }
+};
+
+__device__ __forceinline__ Nine gather9(double rm, double r0, double rp, int lane)
+{
+    Nine n;
+    const double rows[3] = {rm, r0, rp};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        n.v[k][0] = shfl(rows[k], lane - 1);
+        n.v[k][1] = rows[k];
+        n.v[k][2] = shfl(rows[k], lane + 1);
+    }
+    return n;
+}
+
+// Mediator momentum gm = sum_i c_i M_i, M_i(x) = w_i (3 phi(x - c_i)):
+// emite_mediadores + propag_site_med + momentum (LBM_functions.cpp:1286-1337, 1056-1087).
+__device__ __forceinline__ void mediator_gradient(const Nine& phi, double& gx, double& gy, double& gz)
+{
+    auto t = [&](int dx, int dy) { return phi.at(dx, dy) * 3.0; };
+    const double M1 = W1 * t(-1, 0), M2 = W1 * t(1, 0), M3 = W1 * t(0, -1), M4 = W1 * t(0, 1);
+    const double M5 = W1 * t(0, 0);
+    const double M7 = W2 * t(-1, -1), M8 = W2 * t(1, 1), M9 = W2 * t(-1, 1), M10 = W2 * t(1, -1);
+    const double M11 = W2 * t(-1, 0), M12 = W2 * t(1, 0), M15 = W2 * t(0, 1), M16 = W2 * t(0, -1);
+    // M13 = M11, M14 = M12, M17 = M15, M18 = M16, M6 = M5
+    gx = M1 - M2;
+    gx = gx + M7; gx = gx - M8; gx = gx + M9; gx = gx - M10;
+    gx = gx + M11; gx = gx - M12; gx = gx + M11; gx = gx - M12;
+    gy = M3 - M4;
+    gy = gy + M7; gy = gy - M8; gy = gy - M9; gy = gy + M10;
+    gy = gy - M15; gy = gy + M16; gy = gy - M15; gy = gy + M16;
+    gz = M5 - M5;
+    gz = gz + M11; gz = gz - M12; gz = gz - M11; gz = gz + M12;
+    gz = gz - M15; gz = gz + M16; gz = gz + M15; gz = gz - M16;
+}
+
+// Isotropic gradient of the phase field, g = 3 sum_{i>=1} w_i c_i phi(x + c_i)
+// (calcula_forca, Operadores_tensao.cpp:219-235).
+__device__ __forceinline__ void phase_gradient(const Nine& p, double& gx, double& gy, double& gz)
+{
+    const double ph1 = W1 * p.at(1, 0), ph2 = W1 * p.at(-1, 0), ph3 = W1 * p.at(0, 1), ph4 = W1 * p.at(0, -1);
+    const double ph5 = W1 * p.at(0, 0);
+    const double ph7 = W2 * p.at(1, 1), ph8 = W2 * p.at(-1, -1), ph9 = W2 * p.at(1, -1), ph10 = W2 * p.at(-1, 1);
+    const double ph11 = W2 * p.at(1, 0), ph12 = W2 * p.at(-1, 0), ph15 = W2 * p.at(0, -1), ph16 = W2 * p.at(0, 1);
+    double x = 0.0, y = 0.0, z = 0.0;
+    x = x + ph1; x = x - ph2; x = x + ph7; x = x - ph8; x = x + ph9; x = x - ph10;
+    x = x + ph11; x = x - ph12; x = x + ph11; x = x - ph12;
+    y = y + ph3; y = y - ph4; y = y + ph7; y = y - ph8; y = y - ph9; y = y + ph10;
+    y = y - ph15; y = y + ph16; y = y - ph15; y = y + ph16;
+    z = z + ph5; z = z - ph5; z = z + ph11; z = z - ph12; z = z - ph11; z = z + ph12;
+    z = z - ph15; z = z + ph16; z = z + ph15; z = z - ph16;
+    gx = 3.0 * x; gy = 3.0 * y; gz = 3.0 * z;
+}
+
+// Per-site collision: colisao_comum (Operadores_tensao.cpp:340-373) on pulled populations.
+// FORCED: class F (Guo forcing with F), otherwise class P (perturbation, no force).
+template <bool FORCED>
+__device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], double fB[Q], double rR, double rB,
+                                        double gx, double gy, double gz, double Fx, double Fy, double Fz)
+{
+    const double rho = rR + rB;
+    if (rho <= 1.0e-30) return;
+    const double cR = rR / rho;
+    const double cB = 1.0 - cR;
+    const double mod2 = gx * gx + gy * gy + gz * gz;
+    const double mod = sqrt(mod2);
+
+    double f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = fR[i] + fB[i];
+
+    // calcula (LBM_functions.cpp:1136-1169)
+    double rho2 = f[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) rho2 = rho2 + f[i];
+    double mx = f[1] - f[2];
+    mx = mx + f[7]; mx = mx - f[8]; mx = mx + f[9]; mx = mx - f[10];
+    mx = mx + f[11]; mx = mx - f[12]; mx = mx + f[13]; mx = mx - f[14];
+    double my = f[3] - f[4];
+    my = my + f[7]; my = my - f[8]; my = my - f[9]; my = my + f[10];
+    my = my - f[15]; my = my + f[16]; my = my - f[17]; my = my + f[18];
+    double mz = f[5] - f[6];
+    mz = mz + f[11]; mz = mz - f[12]; mz = mz - f[13]; mz = mz + f[14];
+    mz = mz - f[15]; mz = mz + f[16]; mz = mz + f[17]; mz = mz - f[18];
+    double vx = 0., vy = 0., vz = 0.;
+    if (rho2 != 0.0) {
+        const double inv = 1.0 / rho2;
+        vx = mx * inv; vy = my * inv; vz = mz * inv;
+    }
+
+    // coll_BGK (Collision_operators.cpp:28-47)
+    double Sx = 0.0, Sy = 0.0, Sz = 0.0, vF = 0.0;
+    if constexpr (FORCED) {
+        const double ax = Fx / rho, ay = Fy / rho, az = Fz / rho;
+        vx = vx + 0.5 * ax; vy = vy + 0.5 * ay; vz = vz + 0.5 * az;
+        Sx = ax * rho2; Sy = ay * rho2; Sz = az * rho2;   // F' = a rho'
+        vF = 0.0 + Sx * vx;
+        vF = vF + Sy * vy;
+        vF = vF + Sz * vz;
+    }
+    const double vquad = vx * vx + vy * vy + vz * vz;
+    const double K = 0.5 * vquad * 3.0;
+    const double wr0 = W0 * rho2, wr1 = W1 * rho2, wr2 = W2 * rho2;
+
+    double feq[Q];
+    feq[0] = wr0 * ((1.0 + 0.0) - K);
+    auto pair = [&](int i, int o, double wr, double cv) {
+        const double lin = cv * 3.0;
+        const double qd = 0.5 * cv * cv * 3.0 * 3.0;
+        feq[i] = wr * (1.0 + lin + qd - K);
+        feq[o] = wr * (1.0 - lin + qd - K);
+    };
+    pair(1, 2, wr1, vx);
+    pair(3, 4, wr1, vy);
+    pair(5, 6, wr1, vz);
+    pair(7, 8, wr2, vx + vy);
+    pair(9, 10, wr2, vx - vy);
+    pair(11, 12, wr2, vx + vz);
+    pair(13, 14, wr2, vx - vz);
+    pair(16, 15, wr2, vy + vz);
+    pair(18, 17, wr2, vy - vz);
+
+    const double oot = cc.one_over_tau;
+    if constexpr (FORCED) {
+        // source (LBM_functions.cpp:1641-1669), S_i = sfac_i ((cF + 3 cF cv) - vF)
+        double S[Q];
+        S[0] = cc.sfac0 * (0.0 - vF);
+#define ITOLBM_SRC(i, sf)                                                   \
+    {                                                                       \
+        const double cF = cdot<i>(Sx, Sy, Sz);                              \
+        const double cv = cdot<i>(vx, vy, vz);                              \
+        S[i] = sf * (cF + 3.0 * cF * cv - vF);                              \
+    }
+        ITOLBM_SRC(1, cc.sfac1) ITOLBM_SRC(2, cc.sfac1) ITOLBM_SRC(3, cc.sfac1) ITOLBM_SRC(4, cc.sfac1)
+        ITOLBM_SRC(5, cc.sfac1) ITOLBM_SRC(6, cc.sfac1) ITOLBM_SRC(7, cc.sfac2) ITOLBM_SRC(8, cc.sfac2)
+        ITOLBM_SRC(9, cc.sfac2) ITOLBM_SRC(10, cc.sfac2) ITOLBM_SRC(11, cc.sfac2) ITOLBM_SRC(12, cc.sfac2)
+        ITOLBM_SRC(13, cc.sfac2) ITOLBM_SRC(14, cc.sfac2) ITOLBM_SRC(15, cc.sfac2) ITOLBM_SRC(16, cc.sfac2)
+        ITOLBM_SRC(17, cc.sfac2) ITOLBM_SRC(18, cc.sfac2)
+#undef ITOLBM_SRC
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = f[i] + (feq[i] - f[i]) * oot + S[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = f[i] + (feq[i] - f[i]) * oot;
+
+        // perturbacao (Operadores_tensao.cpp:116-133): f_i += A|g| (w_i (c.g)^2/|g|^2 - B_i)
+        if (!(mod2 <= 1.0e-60) && cc.A != 0.0) {
+            const double Am = cc.A * mod;
+            f[0] += Am * (0.0 - cc.B0);
+            auto pert = [&](int i, int o, double w, double B, double cg) {
+                const double t = Am * (w * cg * cg / mod2 - B);
+                f[i] += t;
+                f[o] += t;
+            };
+            pert(1, 2, W1, cc.B1, gx);
+            pert(3, 4, W1, cc.B1, gy);
+            pert(5, 6, W1, cc.B1, gz);
+            pert(7, 8, W2, cc.B2, gx + gy);
+            pert(9, 10, W2, cc.B2, gx - gy);
+            pert(11, 12, W2, cc.B2, gx + gz);
+            pert(13, 14, W2, cc.B2, gx - gz);
+            pert(16, 15, W2, cc.B2, gy + gz);
+            pert(18, 17, W2, cc.B2, gy - gz);
+        }
+    }
+
+    // recolloring (LBM_functions.cpp:1712-1746)
+    const double fator = cc.beta * cR * cB;
+    fR[0] = cR * f[0];
+    fB[0] = cB * f[0];
+    const double fw1 = fator * W1 * rho, fw2 = fator * W2 * rho;
+    const double mod_d = mod * SQRT2;
+    auto recol = [&](int i, int o, double fw, double prod, double den) {
+        const double cosp = (mod != 0.0) ? prod / den : 0.0;
+        const double t = fw * cosp;
+        fR[i] = cR * f[i] - t;
+        fB[i] = cB * f[i] + t;
+        fR[o] = cR * f[o] - (-t);
+        fB[o] = cB * f[o] + (-t);
+    };
+    recol(1, 2, fw1, gx, mod);
+    recol(3, 4, fw1, gy, mod);
+    recol(5, 6, fw1, gz, mod);
+    recol(7, 8, fw2, gx + gy, mod_d);
+    recol(9, 10, fw2, gx - gy, mod_d);
+    recol(11, 12, fw2, gx + gz, mod_d);
+    recol(13, 14, fw2, gx - gz, mod_d);
+    recol(16, 15, fw2, gy + gz, mod_d);
+    recol(18, 17, fw2, gy - gz, mod_d);
+}
+
+__device__ __forceinline__ void store_row(double* __restrict__ oR, double* __restrict__ oB, long long S,
+                                          long long site, const double fR[Q], const double fB[Q])
+{
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        __builtin_nontemporal_store(fR[i], oR + (long long)i * S + site);
+        __builtin_nontemporal_store(fB[i], oB + (long long)i * S + site);
+    }
+}
+
+__device__ __forceinline__ double phase(double rR, double rB) { return (rR - rB) / (rR + rB); }
+
+__device__ __forceinline__ Lane make_lane(const CaseConst& cc, int strip, int H)
+{
+    Lane L;
+    L.lane = threadIdx.x;
+    const int WI = 64 - 2 * H;
+    const int xr = strip * WI + L.lane - H;
+    L.col = wrapi(xr, cc.nx);
+    L.xm = (L.col == 0) ? cc.nx - 1 : L.col - 1;
+    L.xp = (L.col == cc.nx - 1) ? 0 : L.col + 1;
+    L.interior = (L.lane >= H) && (L.lane < 64 - H) && (xr < cc.nx);
+    return L;
+}
+
+// ------------------------------------------------------------------ P1 / P3 and F3 (1-ring)
+template <bool F3>
+__global__ void __launch_bounds__(64) step_ring1(CaseConst cc, StepArgs a)
+{
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int strip = tile % a.nstrip, seg = tile / a.nstrip;
+    const Lane L = make_lane(cc, strip, 1);
+    const int nx = cc.nx, ny = cc.ny;
+    const int y0 = seg * a.ly;
+    const int y1 = min(y0 + a.ly, ny);
+    if (y0 >= ny) return;
+
+    double cR_[Q], cB_[Q], nR[Q], nB[Q];
+    load_row(a.inR, a.inB, a.S, nx, ny, y0 - 1, L, nR, nB);
+    double phi_m = phase(dens(nR), dens(nB));
+    load_row(a.inR, a.inB, a.S, nx, ny, y0, L, cR_, cB_);
+    double rRc = dens(cR_), rBc = dens(cB_);
+    double phi_c = phase(rRc, rBc);
+
+    for (int y = y0; y < y1; ++y) {
+        load_row(a.inR, a.inB, a.S, nx, ny, y + 1, L, nR, nB);
+        const double rRn = dens(nR), rBn = dens(nB);
+        const double phi_n = phase(rRn, rBn);
+
+        const Nine p9 = gather9(phi_m, phi_c, phi_n, L.lane);
+        double gx, gy, gz;
+        mediator_gradient(p9, gx, gy, gz);
+
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+        if constexpr (F3) {
+            // mu grad phi (Operadores_tensao.cpp:273-287, 325-330)
+            double g0, g1, g2;
+            phase_gradient(p9, g0, g1, g2);
+            const double ph = phi_c;
+            double lap = 0.0;
+#define ITOLBM_LAP(i) lap += (i < 7 ? W1 : W2) * (p9.at(CX[i], CY[i]) - ph);
+            ITOLBM_LAP(1) ITOLBM_LAP(2) ITOLBM_LAP(3) ITOLBM_LAP(4) ITOLBM_LAP(5) ITOLBM_LAP(6)
+            ITOLBM_LAP(7) ITOLBM_LAP(8) ITOLBM_LAP(9) ITOLBM_LAP(10) ITOLBM_LAP(11) ITOLBM_LAP(12)
+            ITOLBM_LAP(13) ITOLBM_LAP(14) ITOLBM_LAP(15) ITOLBM_LAP(16) ITOLBM_LAP(17) ITOLBM_LAP(18)
+#undef ITOLBM_LAP
+            lap *= 6.0;
+            const double mu = 4.0 * cc.bf * ph * (ph * ph - 1.0) - cc.kf * lap;
+            Fx = mu * g0; Fy = mu * g1; Fz = mu * g2;
+        }
+
+        if (L.interior) {
+            const long long site = (long long)y * nx + L.col;
+            collide<F3>(cc, cR_, cB_, rRc, rBc, gx, gy, gz, Fx, Fy, Fz);
+            store_row(a.outR, a.outB, a.S, site, cR_, cB_);
+            if (a.aux) {
+                a.aux[0 * a.S + site] = gx; a.aux[1 * a.S + site] = gy; a.aux[2 * a.S + site] = gz;
+                a.aux[3 * a.S + site] = Fx; a.aux[4 * a.S + site] = Fy; a.aux[5 * a.S + site] = Fz;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) { cR_[i] = nR[i]; cB_[i] = nB[i]; }
+        rRc = rRn; rBc = rBn;
+        phi_m = phi_c; phi_c = phi_n;
+    }
+}
+
+// ------------------------------------------------------------------ F1 / F2 (2-ring)
+struct Aux6 { double a[6]; };
+
+template <int OP>
+__device__ __forceinline__ Aux6 aux_of(const CaseConst& cc, double g0, double g1, double g2)
+{
+    Aux6 r;
+    const double m = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+    if constexpr (OP == OP_F1) {
+        const double inv = (m > 1.0e-12) ? 1.0 / m : 0.0;
+        r.a[0] = g0 * inv; r.a[1] = g1 * inv; r.a[2] = g2 * inv;
+        r.a[3] = r.a[4] = r.a[5] = 0.0;
+    } else {
+        const double h = cc.hs;
+        const double inv = (m > 1.0e-30) ? 1.0 / m : 0.0;
+        r.a[0] = h * (m - g0 * g0 * inv);
+        r.a[1] = h * (m - g1 * g1 * inv);
+        r.a[2] = h * (m - g2 * g2 * inv);
+        r.a[3] = -h * g0 * g1 * inv;
+        r.a[4] = -h * g0 * g2 * inv;
+        r.a[5] = -h * g1 * g2 * inv;
+    }
+    return r;
+}
+
+template <int OP>
+__global__ void __launch_bounds__(64) step_ring2(CaseConst cc, StepArgs a)
+{
+    constexpr int NA = (OP == OP_F1) ? 3 : 6;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int strip = tile % a.nstrip, seg = tile / a.nstrip;
+    const Lane L = make_lane(cc, strip, 2);
+    const int nx = cc.nx, ny = cc.ny;
+    const int y0 = seg * a.ly;
+    const int y1 = min(y0 + a.ly, ny);
+    if (y0 >= ny) return;
+
+    double fR[Q], fB[Q];
+    auto phi_row = [&](int y) {
+        load_row(a.inR, a.inB, a.S, nx, ny, y, L, fR, fB);
+        return phase(dens(fR), dens(fB));
+    };
+    double pa = phi_row(y0 - 2), pb = phi_row(y0 - 1), pc = phi_row(y0), pd = phi_row(y0 + 1);
+    double gm0, gm1, gm2, g00, g01, g02;
+    phase_gradient(gather9(pa, pb, pc, L.lane), gm0, gm1, gm2);     // g at row y0-1
+    phase_gradient(gather9(pb, pc, pd, L.lane), g00, g01, g02);     // g at row y0
+    Aux6 am = aux_of<OP>(cc, gm0, gm1, gm2), a0 = aux_of<OP>(cc, g00, g01, g02);
+
+    for (int y = y0; y < y1; ++y) {
+        const double pe = phi_row(y + 2);
+        double gp0, gp1, gp2;
+        phase_gradient(gather9(pc, pd, pe, L.lane), gp0, gp1, gp2);   // g at row y+1
+        const Aux6 ap = aux_of<OP>(cc, gp0, gp1, gp2);
+
+        // neighbour aux values: A[k][dy+1][dx+1]
+        double A[NA][3][3];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            const Nine n = gather9(am.a[k], a0.a[k], ap.a[k], L.lane);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) A[k][r][c] = n.v[r][c];
+        }
+        double Fx, Fy, Fz;
+        if constexpr (OP == OP_F1) {
+            // kappa = -div_iso(n) (Operadores_tensao.cpp:187-200, 300-304)
+            double d = 0.0;
+#define ITOLBM_DIV(i)                                                                               \
+    d += (i < 7 ? W1 : W2) * cdot<i>(A[0][CY[i] + 1][CX[i] + 1], A[1][CY[i] + 1][CX[i] + 1],       \
+                                     A[2][CY[i] + 1][CX[i] + 1]);
+            ITOLBM_DIV(1) ITOLBM_DIV(2) ITOLBM_DIV(3) ITOLBM_DIV(4) ITOLBM_DIV(5) ITOLBM_DIV(6)
+            ITOLBM_DIV(7) ITOLBM_DIV(8) ITOLBM_DIV(9) ITOLBM_DIV(10) ITOLBM_DIV(11) ITOLBM_DIV(12)
+            ITOLBM_DIV(13) ITOLBM_DIV(14) ITOLBM_DIV(15) ITOLBM_DIV(16) ITOLBM_DIV(17) ITOLBM_DIV(18)
+#undef ITOLBM_DIV
+            const double kappa = -(3.0 * d);
+            Fx = cc.hs * kappa * g00; Fy = cc.hs * kappa * g01; Fz = cc.hs * kappa * g02;
+        } else {
+            // F_a = d_b T_ab (Operadores_tensao.cpp:306-324); T = xx yy zz xy xz yz
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+#define ITOLBM_DIVT(i)                                                                              \
+    {                                                                                               \
+        const double T[6] = {A[0][CY[i] + 1][CX[i] + 1], A[1][CY[i] + 1][CX[i] + 1],                \
+                             A[2][CY[i] + 1][CX[i] + 1], A[3][CY[i] + 1][CX[i] + 1],                \
+                             A[4][CY[i] + 1][CX[i] + 1], A[5][CY[i] + 1][CX[i] + 1]};               \
+        constexpr double w = (i < 7 ? W1 : W2);                                                     \
+        sx += w * cdot<i>(T[0], T[3], T[4]);                                                        \
+        sy += w * cdot<i>(T[3], T[1], T[5]);                                                        \
+        sz += w * cdot<i>(T[4], T[5], T[2]);                                                        \
+    }
+            ITOLBM_DIVT(1) ITOLBM_DIVT(2) ITOLBM_DIVT(3) ITOLBM_DIVT(4) ITOLBM_DIVT(5) ITOLBM_DIVT(6)
+            ITOLBM_DIVT(7) ITOLBM_DIVT(8) ITOLBM_DIVT(9) ITOLBM_DIVT(10) ITOLBM_DIVT(11) ITOLBM_DIVT(12)
+            ITOLBM_DIVT(13) ITOLBM_DIVT(14) ITOLBM_DIVT(15) ITOLBM_DIVT(16) ITOLBM_DIVT(17) ITOLBM_DIVT(18)
+#undef ITOLBM_DIVT
+            Fx = 3.0 * sx; Fy = 3.0 * sy; Fz = 3.0 * sz;
+        }
+
+        double gx, gy, gz;
+        mediator_gradient(gather9(pb, pc, pd, L.lane), gx, gy, gz);
+
+        // populations of row y again (read two rows ago for the colour field; now an L2 hit)
+        load_row(a.inR, a.inB, a.S, nx, ny, y, L, fR, fB);
+        if (L.interior) {
+            const long long site = (long long)y * nx + L.col;
+            collide<true>(cc, fR, fB, dens(fR), dens(fB), gx, gy, gz, Fx, Fy, Fz);
+            store_row(a.outR, a.outB, a.S, site, fR, fB);
+            if (a.aux) {
+                a.aux[0 * a.S + site] = gx; a.aux[1 * a.S + site] = gy; a.aux[2 * a.S + site] = gz;
+                a.aux[3 * a.S + site] = Fx; a.aux[4 * a.S + site] = Fy; a.aux[5 * a.S + site] = Fz;
+            }
+        }
+        pa = pb; pb = pc; pc = pd; pd = pe;
+        g00 = gp0; g01 = gp1; g02 = gp2;
+        am = a0; a0 = ap;
+    }
+}
+
+}  // namespace
+
+int step_tile_geometry(int op, int nx, int ny, int ly, int* nstrip, int* nseg)
+{
+    const int H = (op == OP_F1 || op == OP_F2) ? 2 : 1;
+    const int WI = 64 - 2 * H;
+    *nstrip = (nx + WI - 1) / WI;
+    *nseg = (ny + ly - 1) / ly;
+    return H;
+}
+
+hipError_t launch_step_f64(const CaseConst& cc, const StepArgs& a, hipStream_t stream)
+{
+    const dim3 grid(a.nstrip * a.nseg), block(64);
+    switch (cc.op) {
+    case OP_P1:
+    case OP_P3: hipLaunchKernelGGL(step_ring1<false>, grid, block, 0, stream, cc, a); break;
+    case OP_F3: hipLaunchKernelGGL(step_ring1<true>, grid, block, 0, stream, cc, a); break;
+    case OP_F1: hipLaunchKernelGGL(step_ring2<OP_F1>, grid, block, 0, stream, cc, a); break;
+    case OP_F2: hipLaunchKernelGGL(step_ring2<OP_F2>, grid, block, 0, stream, cc, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace itolbm
