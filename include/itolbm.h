@@ -30,8 +30,13 @@ enum { LBM_OP_P1 = 1, LBM_OP_P3 = 3, LBM_OP_F1 = 11, LBM_OP_F2 = 12, LBM_OP_F3 =
 
 /* arithmetic / storage forms */
 enum {
-    LBM_PREC_F64 = 0 /* FP64 compute and storage: the drop-in form */
+    LBM_PREC_F64 = 0, /* FP64 compute and storage: the drop-in form, bit-equal to the reference's arithmetic */
+    LBM_PREC_F32 = 1, /* FP32 compute, FP32 storage of s_i / w_i and e_i (304 B per lattice update) */
+    LBM_PREC_F16 = 2  /* FP32 compute, FP16 storage of s_i / w_i and e_i (152 B per lattice update) */
 };
+/* In the FP32/FP16 forms each population is stored as A_i = s_i / w_i, s_i = fR_i + fB_i - w_i (deviation
+ * from the rest state rho0 = 1, scaled per direction by 1 / w_i) and e_i = (fR_i - fB_i) / (fR_i + fB_i)
+ * (colour fraction, exactly +-1 in a pure phase).  Transfers and diagnostics convert to and from FP64. */
 
 /* which population state to transfer */
 enum {
@@ -90,7 +95,7 @@ typedef struct {
     int n_ruim, n_ajuste, ok;
 } lbm_incl_diag;
 
-/* Allocates device state for n_cases cases (Imm_OPS.cpp:158-181 allocations).  Only LBM_PREC_F64. */
+/* Allocates device state for n_cases cases (Imm_OPS.cpp:158-181 allocations) in form `precision`. */
 int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx** out);
 
 /* Frees everything (the reference never frees). */

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libitolbm.so")
 Q = 19
 OPS = {"P1": 1, "P3": 3, "F1": 11, "F2": 12, "F3": 13}
 PREC_F64 = 0
+PREC_F32 = 1
+PREC_F16 = 2
 STATE_POST_STREAM = 0
 STATE_POST_COLLISION = 1
 STEP_RECORD_AUX = 1

@@ -34,7 +34,9 @@ struct CaseState {
     int cur = 0;                            // buffer holding the latest post-collision state
     double* aux = nullptr;                  // [6][S] gm, F
     bool aux_valid = false;
-    int ly = 8, nstrip = 1, nseg = 1;
+    int kind = 0;                           // kernel family (KIND_*)
+    int elem = 8;                           // bytes per stored value: 8 (FP64), 4 (FP32), 2 (FP16)
+    double* f64 = nullptr;                  // [2][19][S] FP64 copy of a low-precision state (diagnostics)
     // diagnostics scratch (allocated on first use)
     double* gal = nullptr;
     double* dsum = nullptr;
@@ -47,6 +49,20 @@ struct CaseState {
     int nblk = 1;
 };
 
+// A set of cases advanced by one launch per step (cases of one kernel family).
+struct Group {
+    int kind = 0;
+    std::vector<int> members;
+    CaseDev* dtab = nullptr;       // device copy of the case table
+    int* dtile = nullptr;          // tile -> index into members
+    std::vector<int> ly;           // rows per segment of each member in this group
+    std::vector<int> cur0;         // members' state buffer index when the table was written
+    std::vector<double*> aux0;     // members' aux pointer when the table was written
+    int k = 0;                     // launches since the table was written
+    int ntiles = 0;
+    bool written = false;
+};
+
 }  // namespace
 
 struct lbm_ctx {
@@ -55,6 +71,8 @@ struct lbm_ctx {
     hipStream_t stream = nullptr;      // stream in use
     hipStream_t own_stream = nullptr;  // the context's own stream
     long long bytes = 0;
+    Group family[N_KINDS];             // all cases of each kernel family (lbm_step)
+    std::vector<Group> single;         // one group per case (lbm_step_case)
 };
 
 namespace {
@@ -113,17 +131,35 @@ CaseConst make_const(const lbm_case_desc& d)
     return c;
 }
 
-int default_rows(int nstrip, int ny)
+// Elements added to each population array's length.  Array i starts at i * S elements; with S a
+// power of two every one of the 38 load and 38 store streams of a wavefront would map to the same
+// HBM channel, so S is padded (ITOLBM_PAD overrides).
+long long stride_pad()
+{
+    if (const char* e = std::getenv("ITOLBM_PAD")) return std::atoll(e);
+    return 0;
+}
+
+int strips_of(const lbm_case_desc& d)
+{
+    const int wi = 64 - 2 * step_halo(d.op);
+    return (d.nx + wi - 1) / wi;
+}
+
+// Rows per wavefront segment for a group: enough segments for >= 4096 wavefronts in the launch,
+// within 4..64 rows (fewer rows = more re-read halo rows).  ITOLBM_LY overrides.
+int group_rows(const std::vector<CaseState>& cs, const std::vector<int>& members)
 {
     if (const char* e = std::getenv("ITOLBM_LY")) {
         const int v = std::atoi(e);
         if (v > 0) return v;
     }
-    // aim at >= 2048 wavefronts (8 per CU) with segments of 4..64 rows
-    long long ly = (long long)nstrip * ny / 2048;
+    long long strip_rows = 0;
+    for (int id : members) strip_rows += (long long)strips_of(cs[id].d) * cs[id].d.ny;
+    long long ly = strip_rows / 4096;
     if (ly < 4) ly = 4;
     if (ly > 64) ly = 64;
-    return (int)std::min<long long>(ly, std::max(ny, 1));
+    return (int)ly;
 }
 
 const int CXh[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
@@ -218,6 +254,51 @@ long long gcdll(long long a, long long b)
     return a;
 }
 
+// (fR, fB) [colour][19][S] FP64 at own site -> pairs (A_i = s_i / w_i, e_i) at ((y * 19 + i) nx + x) in the
+// case's storage type
+int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
+{
+    const int nx = k.d.nx;
+    std::vector<float> a(2 * (size_t)Q * k.n);
+    for (long long p = 0; p < k.n; ++p) {
+        const long long y = p / nx, x = p - y * nx;
+        for (int i = 0; i < Q; ++i) {
+            const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+            const double r = h[(size_t)i * k.S + p], b = h[(size_t)(Q + i) * k.S + p];
+            const double f = r + b;
+            const size_t j = 2 * (size_t)((y * Q + i) * nx + x);
+            a[j] = (float)((f - w) / w);
+            a[j + 1] = (float)((f != 0.0) ? (r - b) / f : 0.0);
+        }
+    }
+    if (k.elem == 4) {
+        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<_Float16> hh(a.size());
+        for (size_t j = 0; j < a.size(); ++j) hh[j] = (_Float16)a[j];
+        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], hh.data(), sizeof(_Float16) * hh.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return LBM_OK;
+}
+
+// FP64 view [colour][19][S] of the current post-collision state (device pointer)
+int state_f64(lbm_ctx* c, CaseState& k, const double** out)
+{
+    if (k.elem == 8) {
+        *out = (const double*)k.pop[k.cur];
+        return LBM_OK;
+    }
+    if (!k.f64) {
+        if (int r = dev_alloc(c, (void**)&k.f64, sizeof(double) * 2 * Q * (size_t)k.S)) return r;
+    }
+    HIPCHK(c, launch_lp_to_f64(k.elem, k.pop[k.cur], k.d.nx, k.S, k.n, k.f64, k.f64 + Q * k.S, c->stream));
+    *out = k.f64;
+    return LBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -235,7 +316,8 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
     if (!out) return LBM_E_ARG;
     *out = nullptr;
     if (!cases || n_cases < 1) return LBM_E_ARG;
-    if (precision != LBM_PREC_F64) return LBM_E_ARG;
+    if (precision != LBM_PREC_F64 && precision != LBM_PREC_F32 && precision != LBM_PREC_F16) return LBM_E_ARG;
+    const int elem = (precision == LBM_PREC_F64) ? 8 : (precision == LBM_PREC_F32 ? 4 : 2);
     lbm_ctx* c = new (std::nothrow) lbm_ctx();
     if (!c) return LBM_E_NOMEM;
     int dev_count = 0;
@@ -261,18 +343,23 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.d = d;
         k.cc = make_const(d);
         k.n = (long long)d.nx * d.ny;
-        k.S = (k.n + 63) / 64 * 64;
-        int ns, nseg;
-        step_tile_geometry(d.op, d.nx, d.ny, 1, &ns, &nseg);
-        k.ly = default_rows(ns, d.ny);
-        step_tile_geometry(d.op, d.nx, d.ny, k.ly, &k.nstrip, &k.nseg);
+        k.S = (k.n + 63) / 64 * 64 + stride_pad();
+        k.kind = step_kind(d.op);
+        k.elem = elem;
         for (int b = 0; b < 2; ++b) {
-            if (dev_alloc(c, (void**)&k.pop[b], sizeof(double) * 2 * Q * (size_t)k.S) ||
-                hipMemsetAsync(k.pop[b], 0, sizeof(double) * 2 * Q * (size_t)k.S, c->stream) != hipSuccess) {
+            if (dev_alloc(c, (void**)&k.pop[b], (size_t)elem * 2 * Q * (size_t)k.S) ||
+                hipMemsetAsync(k.pop[b], 0, (size_t)elem * 2 * Q * (size_t)k.S, c->stream) != hipSuccess) {
                 lbm_destroy(c);
                 return LBM_E_NOMEM;
             }
         }
+    }
+    for (int kd = 0; kd < N_KINDS; ++kd) c->family[kd].kind = kd;
+    c->single.resize(n_cases);
+    for (int i = 0; i < n_cases; ++i) {
+        c->family[c->cases[i].kind].members.push_back(i);
+        c->single[i].kind = c->cases[i].kind;
+        c->single[i].members.push_back(i);
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         lbm_destroy(c);
@@ -287,10 +374,16 @@ int lbm_destroy(lbm_ctx* c)
     if (!c) return LBM_E_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (CaseState& k : c->cases) {
-        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.partial, k.hist, k.tab, k.prof})
+        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.partial, k.hist, k.tab, k.prof, k.f64})
             if (p) (void)hipFree(p);
         if (k.histn) (void)hipFree(k.histn);
     }
+    auto free_group = [](Group& g) {
+        if (g.dtab) (void)hipFree(g.dtab);
+        if (g.dtile) (void)hipFree(g.dtile);
+    };
+    for (Group& g : c->family) free_group(g);
+    for (Group& g : c->single) free_group(g);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
     return LBM_OK;
@@ -302,8 +395,8 @@ int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
     if (!fR || !fB) return fail(c, LBM_E_ARG, "null population array");
     CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
-    std::vector<double> h(2 * Q * (size_t)k.S, 0.0);
     // device holds "post-collision at own site": f*_i(x) = f_i(x + c_i) for a post-stream state f
+    std::vector<double> h(2 * Q * (size_t)k.S, 0.0);
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             const long long site = (long long)x + (long long)nx * y;
@@ -313,8 +406,12 @@ int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
                 h[(size_t)(Q + i) * k.S + site] = fB[(size_t)src * Q + i];
             }
         }
-    HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (k.elem == 8) {
+        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        if (int r = upload_lp(c, k, h)) return r;
+    }
     k.aux_valid = false;
     return LBM_OK;
 }
@@ -410,7 +507,9 @@ int lbm_download_state(lbm_ctx* c, int id, int which, double* fR, double* fB)
     CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
     std::vector<double> h(2 * Q * (size_t)k.S);
-    HIPCHK(c, hipMemcpyAsync(h.data(), k.pop[k.cur], sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    const double* src64 = nullptr;
+    if (int r = state_f64(c, k, &src64)) return r;
+    HIPCHK(c, hipMemcpyAsync(h.data(), src64, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
@@ -424,39 +523,110 @@ int lbm_download_state(lbm_ctx* c, int id, int which, double* fR, double* fB)
     return LBM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// (Re)writes a group's case table when a member's state buffer or aux pointer moved since.
+int group_sync(lbm_ctx* c, Group& g)
+{
+    bool stale = !g.written;
+    for (size_t j = 0; !stale && j < g.members.size(); ++j) {
+        const CaseState& k = c->cases[g.members[j]];
+        stale = (k.cur != (g.cur0[j] ^ (g.k & 1))) || (k.aux != g.aux0[j]);
+    }
+    if (!stale) return LBM_OK;
+    const int nm = (int)g.members.size();
+    if (g.ly.empty()) {
+        const int ly = group_rows(c->cases, g.members);
+        g.ly.assign(nm, ly);
+    }
+    std::vector<CaseDev> tab(nm);
+    std::vector<int> tile_case;
+    g.cur0.assign(nm, 0);
+    g.aux0.assign(nm, nullptr);
+    int tile0 = 0;
+    for (int j = 0; j < nm; ++j) {
+        const CaseState& k = c->cases[g.members[j]];
+        CaseDev& d = tab[j];
+        d.cc = k.cc;
+        d.pop[0] = k.pop[k.cur];
+        d.pop[1] = k.pop[k.cur ^ 1];
+        d.aux = k.aux;
+        d.S = k.S;
+        d.ly = std::max(1, std::min(g.ly[j], k.d.ny));
+        d.nstrip = strips_of(k.d);
+        d.nseg = (k.d.ny + d.ly - 1) / d.ly;
+        d.tile0 = tile0;
+        const int nt = d.nstrip * d.nseg;
+        tile_case.insert(tile_case.end(), nt, j);
+        tile0 += nt;
+        g.cur0[j] = k.cur;
+        g.aux0[j] = k.aux;
+    }
+    if (!g.dtab) {
+        if (int r = dev_alloc(c, (void**)&g.dtab, sizeof(CaseDev) * nm)) return r;
+    }
+    if (g.dtile && tile0 > g.ntiles) {
+        (void)hipFree(g.dtile);
+        g.dtile = nullptr;
+    }
+    if (!g.dtile) {
+        if (int r = dev_alloc(c, (void**)&g.dtile, sizeof(int) * std::max(tile0, 1))) return r;
+    }
+    HIPCHK(c, hipMemcpyAsync(g.dtab, tab.data(), sizeof(CaseDev) * nm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g.dtile, tile_case.data(), sizeof(int) * tile0, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    g.ntiles = tile0;
+    g.k = 0;
+    g.written = true;
+    return LBM_OK;
+}
+
+int group_step(lbm_ctx* c, Group& g, int n_steps, int flags)
+{
+    if (g.members.empty() || n_steps <= 0) return LBM_OK;
+    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    if (rec)
+        for (int id : g.members)
+            if (int r = ensure_aux(c, c->cases[id])) return r;
+    if (int r = group_sync(c, g)) return r;
+    for (int s = 0; s < n_steps; ++s) {
+        StepLaunch a{};
+        a.tab = g.dtab;
+        a.tile_case = g.dtile;
+        a.parity = g.k & 1;
+        a.record = (rec && s == n_steps - 1) ? 1 : 0;
+        const int elem = c->cases[g.members[0]].elem;
+        HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, c->stream)
+                            : launch_step_lp(elem, g.kind, a, g.ntiles, c->stream));
+        g.k++;
+        for (int id : g.members) c->cases[id].cur ^= 1;
+    }
+    for (int id : g.members) c->cases[id].aux_valid = rec;
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int lbm_step_case(lbm_ctx* c, int id, int n_steps, int flags)
 {
     if (int r = check_case(c, id)) return r;
     if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
-    if (n_steps == 0) return LBM_OK;
-    CaseState& k = c->cases[id];
-    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
-    if (rec) {
-        if (int r = ensure_aux(c, k)) return r;
-    }
-    for (int s = 0; s < n_steps; ++s) {
-        StepArgs a{};
-        a.inR = k.pop[k.cur];
-        a.inB = k.pop[k.cur] + Q * k.S;
-        a.outR = k.pop[1 - k.cur];
-        a.outB = k.pop[1 - k.cur] + Q * k.S;
-        a.aux = (rec && s == n_steps - 1) ? k.aux : nullptr;
-        a.S = k.S;
-        a.ly = k.ly;
-        a.nstrip = k.nstrip;
-        a.nseg = k.nseg;
-        HIPCHK(c, launch_step_f64(k.cc, a, c->stream));
-        k.cur ^= 1;
-    }
-    k.aux_valid = rec;
-    return LBM_OK;
+    return group_step(c, c->single[id], n_steps, flags);
 }
 
 int lbm_step(lbm_ctx* c, int n_steps, int flags)
 {
     if (!c) return LBM_E_ARG;
-    for (int i = 0; i < (int)c->cases.size(); ++i)
-        if (int r = lbm_step_case(c, i, n_steps, flags)) return r;
+    if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
+    for (int s = 0; s < n_steps; ++s) {
+        const int f = (s == n_steps - 1) ? flags : 0;
+        for (Group& g : c->family)
+            if (int r = group_step(c, g, 1, f)) return r;
+    }
     return LBM_OK;
 }
 
@@ -501,8 +671,9 @@ int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double 
     if (int r = ensure_diag(c, k)) return r;
     std::memset(o, 0, sizeof(*o));
     const int nx = k.d.nx, ny = k.d.ny;
-    const double* fR = k.pop[k.cur];
-    const double* fB = k.pop[k.cur] + Q * k.S;
+    const double* fR = nullptr;
+    if (int r = state_f64(c, k, &fR)) return r;
+    const double* fB = fR + Q * k.S;
     const double* F = forced ? k.aux + 3 * k.S : nullptr;
 
     // ---- pass 1
@@ -697,8 +868,10 @@ int lbm_diag_incl(lbm_ctx* c, int id, int p, int q, lbm_incl_diag* o, double* pr
     const int nx = k.d.nx, ny = k.d.ny;
     const long long M = gcdll((long long)p * nx, (long long)q * ny) / gcdll(p, q);
     const double m = std::sqrt((double)(p * p + q * q));
+    const double* f64 = nullptr;
+    if (int r = state_f64(c, k, &f64)) return r;
     InclArgs a{};
-    a.fR = k.pop[k.cur]; a.fB = k.pop[k.cur] + Q * k.S; a.S = k.S; a.nx = nx; a.ny = ny;
+    a.fR = f64; a.fB = f64 + Q * k.S; a.S = k.S; a.nx = nx; a.ny = ny;
     a.F = (k.d.op >= OP_F1) ? k.aux + 3 * k.S : nullptr;
     a.gm = k.aux;
     a.p = p; a.q = q; a.M = M; a.m = m;

@@ -25,6 +25,7 @@
 
 #include "lattice.h"
 #include "step_kernels.h"
+#include "tile.h"
 
 namespace itolbm {
 
@@ -32,40 +33,25 @@ namespace {
 
 constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0), correctly rounded
 
-__device__ __forceinline__ int wrapi(int v, int n)
-{
-    v %= n;
-    return v < 0 ? v + n : v;
-}
+// Pointers into device global memory.  Table entries hold generic pointers; casting them to the
+// global address space lets the compiler emit global (not flat) loads and stores with a uniform
+// 64-bit base and a 32-bit per-lane offset.
+using gcdouble = __attribute__((address_space(1))) const double;
+using gdouble = __attribute__((address_space(1))) double;
 
-// Blocks are dealt to the 8 XCDs round-robin; give each XCD a contiguous run of tiles so that
-// tiles sharing halo rows/columns also share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = b & 7, idx = b >> 3;
-    return xcd * q + (xcd < r ? xcd : r) + idx;
-}
-
-struct Lane {
-    int col, xm, xp;      // own column and its periodic x-neighbours
-    int lane;
-    bool interior;        // owns (stores) its column
-};
-
-// Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).
-__device__ __forceinline__ void load_row(const double* __restrict__ pR, const double* __restrict__ pB,
+// Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).  Each population array is a
+// uniform base (p + i S) plus a 32-bit site offset.
+__device__ __forceinline__ void load_row(gcdouble* __restrict__ pR, gcdouble* __restrict__ pB,
                                          long long S, int nx, int ny, int y, const Lane& L,
                                          double fR[Q], double fB[Q])
 {
-    const int rm = wrapi(y - 1, ny) * nx, r0 = wrapi(y, ny) * nx, rp = wrapi(y + 1, ny) * nx;
+    const unsigned rm = wrapi(y - 1, ny) * nx, r0 = wrapi(y, ny) * nx, rp = wrapi(y + 1, ny) * nx;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const int xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
-        const int rs = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
-        const long long idx = (long long)i * S + rs + xs;
-        fR[i] = pR[idx];
-        fB[i] = pB[idx];
+        const unsigned xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
+        const unsigned rs = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
+        fR[i] = (pR + i * S)[rs + xs];
+        fB[i] = (pB + i * S)[rs + xs];
     }
 }
 
@@ -151,24 +137,27 @@ __device__ __forceinline__ void phase_gradient(const Nine& p, double& gx, double
     gx = 3.0 * x; gy = 3.0 * y; gz = 3.0 * z;
 }
 
-// Per-site collision: colisao_comum (Operadores_tensao.cpp:340-373) on pulled populations.
-// FORCED: class F (Guo forcing with F), otherwise class P (perturbation, no force).
-template <bool FORCED>
-__device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], double fB[Q], double rR, double rB,
-                                        double gx, double gy, double gz, double Fx, double Fy, double Fz)
-{
-    const double rho = rR + rB;
-    if (rho <= 1.0e-30) return;
-    const double cR = rR / rho;
-    const double cB = 1.0 - cR;
-    const double mod2 = gx * gx + gy * gy + gz * gz;
-    const double mod = sqrt(mod2);
-
+// Per-site collision, colisao_comum (Operadores_tensao.cpp:340-373), in two stages so that the
+// part that does not depend on the colour gradient (sums, BGK) can run while the next row's loads
+// are in flight.
+struct Site {
     double f[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = fR[i] + fB[i];
+    double rho, cR, cB;    // colour sums of the pulled populations
+    double rho2;           // calcula's density of f = fR + fB
+    double mx, my, mz;     // calcula's momentum of f
+    bool skip;             // rho <= 1e-30: the reference leaves the site untouched
+};
 
-    // calcula (LBM_functions.cpp:1136-1169)
+// f = fR + fB, colour fractions and calcula's sums (LBM_functions.cpp:1136-1169)
+__device__ __forceinline__ void site_sums(Site& s, const double fR[Q], const double fB[Q], double rR, double rB)
+{
+    s.rho = rR + rB;
+    s.skip = (s.rho <= 1.0e-30);
+    s.cR = rR / s.rho;
+    s.cB = 1.0 - s.cR;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s.f[i] = fR[i] + fB[i];
+    const double* f = s.f;
     double rho2 = f[0];
 #pragma unroll
     for (int i = 1; i < Q; ++i) rho2 = rho2 + f[i];
@@ -181,16 +170,23 @@ __device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], doubl
     double mz = f[5] - f[6];
     mz = mz + f[11]; mz = mz - f[12]; mz = mz - f[13]; mz = mz + f[14];
     mz = mz - f[15]; mz = mz + f[16]; mz = mz + f[17]; mz = mz - f[18];
+    s.rho2 = rho2; s.mx = mx; s.my = my; s.mz = mz;
+}
+
+// coll_BGK (Collision_operators.cpp:11-49) on s.f; FORCED adds Guo's source with force F
+template <bool FORCED>
+__device__ __forceinline__ void bgk(const CaseConst& cc, Site& s, double Fx, double Fy, double Fz)
+{
+    double* f = s.f;
+    const double rho2 = s.rho2;
     double vx = 0., vy = 0., vz = 0.;
     if (rho2 != 0.0) {
         const double inv = 1.0 / rho2;
-        vx = mx * inv; vy = my * inv; vz = mz * inv;
+        vx = s.mx * inv; vy = s.my * inv; vz = s.mz * inv;
     }
-
-    // coll_BGK (Collision_operators.cpp:28-47)
     double Sx = 0.0, Sy = 0.0, Sz = 0.0, vF = 0.0;
     if constexpr (FORCED) {
-        const double ax = Fx / rho, ay = Fy / rho, az = Fz / rho;
+        const double ax = Fx / s.rho, ay = Fy / s.rho, az = Fz / s.rho;
         vx = vx + 0.5 * ax; vy = vy + 0.5 * ay; vz = vz + 0.5 * az;
         Sx = ax * rho2; Sy = ay * rho2; Sz = az * rho2;   // F' = a rho'
         vF = 0.0 + Sx * vx;
@@ -200,14 +196,16 @@ __device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], doubl
     const double vquad = vx * vx + vy * vy + vz * vz;
     const double K = 0.5 * vquad * 3.0;
     const double wr0 = W0 * rho2, wr1 = W1 * rho2, wr2 = W2 * rho2;
-
-    double feq[Q];
-    feq[0] = wr0 * ((1.0 + 0.0) - K);
+    const double oot = cc.one_over_tau;
+    // dist_eq (LBM_functions.cpp:924-944) and the relaxation, pair by pair (c_o = -c_i)
+    f[0] = f[0] + (wr0 * ((1.0 + 0.0) - K) - f[0]) * oot;
     auto pair = [&](int i, int o, double wr, double cv) {
         const double lin = cv * 3.0;
         const double qd = 0.5 * cv * cv * 3.0 * 3.0;
-        feq[i] = wr * (1.0 + lin + qd - K);
-        feq[o] = wr * (1.0 - lin + qd - K);
+        const double ei = wr * (1.0 + lin + qd - K);
+        const double eo = wr * (1.0 - lin + qd - K);
+        f[i] = f[i] + (ei - f[i]) * oot;
+        f[o] = f[o] + (eo - f[o]) * oot;
     };
     pair(1, 2, wr1, vx);
     pair(3, 4, wr1, vy);
@@ -218,17 +216,14 @@ __device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], doubl
     pair(13, 14, wr2, vx - vz);
     pair(16, 15, wr2, vy + vz);
     pair(18, 17, wr2, vy - vz);
-
-    const double oot = cc.one_over_tau;
     if constexpr (FORCED) {
-        // source (LBM_functions.cpp:1641-1669), S_i = sfac_i ((cF + 3 cF cv) - vF)
-        double S[Q];
-        S[0] = cc.sfac0 * (0.0 - vF);
+        // source (LBM_functions.cpp:1641-1669): S_i = sfac_i ((cF + 3 cF cv) - vF), added after the relaxation
+        f[0] = f[0] + cc.sfac0 * (0.0 - vF);
 #define ITOLBM_SRC(i, sf)                                                   \
     {                                                                       \
         const double cF = cdot<i>(Sx, Sy, Sz);                              \
         const double cv = cdot<i>(vx, vy, vz);                              \
-        S[i] = sf * (cF + 3.0 * cF * cv - vF);                              \
+        f[i] = f[i] + sf * (cF + 3.0 * cF * cv - vF);                       \
     }
         ITOLBM_SRC(1, cc.sfac1) ITOLBM_SRC(2, cc.sfac1) ITOLBM_SRC(3, cc.sfac1) ITOLBM_SRC(4, cc.sfac1)
         ITOLBM_SRC(5, cc.sfac1) ITOLBM_SRC(6, cc.sfac1) ITOLBM_SRC(7, cc.sfac2) ITOLBM_SRC(8, cc.sfac2)
@@ -236,46 +231,49 @@ __device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], doubl
         ITOLBM_SRC(13, cc.sfac2) ITOLBM_SRC(14, cc.sfac2) ITOLBM_SRC(15, cc.sfac2) ITOLBM_SRC(16, cc.sfac2)
         ITOLBM_SRC(17, cc.sfac2) ITOLBM_SRC(18, cc.sfac2)
 #undef ITOLBM_SRC
-#pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = f[i] + (feq[i] - f[i]) * oot + S[i];
-    } else {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = f[i] + (feq[i] - f[i]) * oot;
-
-        // perturbacao (Operadores_tensao.cpp:116-133): f_i += A|g| (w_i (c.g)^2/|g|^2 - B_i)
-        if (!(mod2 <= 1.0e-60) && cc.A != 0.0) {
-            const double Am = cc.A * mod;
-            f[0] += Am * (0.0 - cc.B0);
-            auto pert = [&](int i, int o, double w, double B, double cg) {
-                const double t = Am * (w * cg * cg / mod2 - B);
-                f[i] += t;
-                f[o] += t;
-            };
-            pert(1, 2, W1, cc.B1, gx);
-            pert(3, 4, W1, cc.B1, gy);
-            pert(5, 6, W1, cc.B1, gz);
-            pert(7, 8, W2, cc.B2, gx + gy);
-            pert(9, 10, W2, cc.B2, gx - gy);
-            pert(11, 12, W2, cc.B2, gx + gz);
-            pert(13, 14, W2, cc.B2, gx - gz);
-            pert(16, 15, W2, cc.B2, gy + gz);
-            pert(18, 17, W2, cc.B2, gy - gz);
-        }
     }
+}
 
-    // recolloring (LBM_functions.cpp:1712-1746)
+// perturbacao (Operadores_tensao.cpp:116-133): f_i += A|g| (w_i (c.g)^2/|g|^2 - B_i), g = gm
+__device__ __forceinline__ void perturb(const CaseConst& cc, double f[Q], double gx, double gy, double gz,
+                                        double mod2, double mod)
+{
+    if (mod2 <= 1.0e-60 || cc.A == 0.0) return;
+    const double Am = cc.A * mod;
+    f[0] += Am * (0.0 - cc.B0);
+    auto pert = [&](int i, int o, double w, double B, double cg) {
+        const double t = Am * (w * cg * cg / mod2 - B);
+        f[i] += t;
+        f[o] += t;
+    };
+    pert(1, 2, W1, cc.B1, gx);
+    pert(3, 4, W1, cc.B1, gy);
+    pert(5, 6, W1, cc.B1, gz);
+    pert(7, 8, W2, cc.B2, gx + gy);
+    pert(9, 10, W2, cc.B2, gx - gy);
+    pert(11, 12, W2, cc.B2, gx + gz);
+    pert(13, 14, W2, cc.B2, gx - gz);
+    pert(16, 15, W2, cc.B2, gy + gz);
+    pert(18, 17, W2, cc.B2, gy - gz);
+}
+
+// recolloring (LBM_functions.cpp:1712-1746) of s.f with the mediator gradient gm; each pair of
+// colour populations goes to out(i, fR_i, fB_i) as soon as it is formed
+template <class Out>
+__device__ __forceinline__ void recolour(const CaseConst& cc, const Site& s, double gx, double gy, double gz,
+                                         double mod, Out out)
+{
+    const double* f = s.f;
+    const double cR = s.cR, cB = s.cB;
     const double fator = cc.beta * cR * cB;
-    fR[0] = cR * f[0];
-    fB[0] = cB * f[0];
-    const double fw1 = fator * W1 * rho, fw2 = fator * W2 * rho;
+    out(0, cR * f[0], cB * f[0]);
+    const double fw1 = fator * W1 * s.rho, fw2 = fator * W2 * s.rho;
     const double mod_d = mod * SQRT2;
     auto recol = [&](int i, int o, double fw, double prod, double den) {
         const double cosp = (mod != 0.0) ? prod / den : 0.0;
         const double t = fw * cosp;
-        fR[i] = cR * f[i] - t;
-        fB[i] = cB * f[i] + t;
-        fR[o] = cR * f[o] - (-t);
-        fB[o] = cB * f[o] + (-t);
+        out(i, cR * f[i] - t, cB * f[i] + t);
+        out(o, cR * f[o] - (-t), cB * f[o] + (-t));
     };
     recol(1, 2, fw1, gx, mod);
     recol(3, 4, fw1, gy, mod);
@@ -288,58 +286,91 @@ __device__ __forceinline__ void collide(const CaseConst& cc, double fR[Q], doubl
     recol(18, 17, fw2, gy - gz, mod_d);
 }
 
-__device__ __forceinline__ void store_row(double* __restrict__ oR, double* __restrict__ oB, long long S,
-                                          long long site, const double fR[Q], const double fB[Q])
+__device__ __forceinline__ void store_row(gdouble* __restrict__ oR, gdouble* __restrict__ oB, long long S,
+                                          unsigned site, const double fR[Q], const double fB[Q])
 {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        __builtin_nontemporal_store(fR[i], oR + (long long)i * S + site);
-        __builtin_nontemporal_store(fB[i], oB + (long long)i * S + site);
+        __builtin_nontemporal_store(fR[i], (oR + i * S) + site);
+        __builtin_nontemporal_store(fB[i], (oB + i * S) + site);
     }
+}
+
+__device__ __forceinline__ void store_aux(gdouble* __restrict__ aux, long long S, unsigned site, double gx, double gy,
+                                          double gz, double Fx, double Fy, double Fz)
+{
+    (aux + 0 * S)[site] = gx; (aux + 1 * S)[site] = gy; (aux + 2 * S)[site] = gz;
+    (aux + 3 * S)[site] = Fx; (aux + 4 * S)[site] = Fy; (aux + 5 * S)[site] = Fz;
 }
 
 __device__ __forceinline__ double phase(double rR, double rB) { return (rR - rB) / (rR + rB); }
 
-__device__ __forceinline__ Lane make_lane(const CaseConst& cc, int strip, int H)
+// One wavefront = one tile of one case: the tile -> case table gives the case, its entry the rest.
+struct TileCtx {
+    CaseConst cc;
+    gcdouble* inR;
+    gcdouble* inB;
+    gdouble* outR;
+    gdouble* outB;
+    gdouble* aux;
+    long long S;
+    int strip, y0, y1;
+};
+
+__device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtx& t)
 {
-    Lane L;
-    L.lane = threadIdx.x;
-    const int WI = 64 - 2 * H;
-    const int xr = strip * WI + L.lane - H;
-    L.col = wrapi(xr, cc.nx);
-    L.xm = (L.col == 0) ? cc.nx - 1 : L.col - 1;
-    L.xp = (L.col == cc.nx - 1) ? 0 : L.col + 1;
-    L.interior = (L.lane >= H) && (L.lane < 64 - H) && (xr < cc.nx);
-    return L;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int c = L.tile_case[tile];
+    const CaseDev& d = L.tab[c];
+    t.cc = d.cc;
+    gcdouble* in = (gcdouble*)d.pop[L.parity];
+    gdouble* out = (gdouble*)d.pop[L.parity ^ 1];
+    t.S = d.S;
+    t.inR = in;
+    t.inB = in + Q * d.S;
+    t.outR = out;
+    t.outB = out + Q * d.S;
+    t.aux = L.record ? (gdouble*)d.aux : nullptr;
+    const int local = tile - d.tile0;
+    t.strip = local % d.nstrip;
+    const int seg = local / d.nstrip;
+    t.y0 = seg * d.ly;
+    t.y1 = min(t.y0 + d.ly, d.cc.ny);
+    return t.y0 < d.cc.ny;
 }
 
 // ------------------------------------------------------------------ P1 / P3 and F3 (1-ring)
 template <bool F3>
-__global__ void __launch_bounds__(64) step_ring1(CaseConst cc, StepArgs a)
+__global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
 {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int strip = tile % a.nstrip, seg = tile / a.nstrip;
-    const Lane L = make_lane(cc, strip, 1);
+    TileCtx t;
+    if (!tile_ctx(launch, t)) return;
+    const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
-    const int y0 = seg * a.ly;
-    const int y1 = min(y0 + a.ly, ny);
-    if (y0 >= ny) return;
+    const Lane L = make_lane(nx, t.strip, 1);
 
     double cR_[Q], cB_[Q], nR[Q], nB[Q];
-    load_row(a.inR, a.inB, a.S, nx, ny, y0 - 1, L, nR, nB);
+    load_row(t.inR, t.inB, t.S, nx, ny, t.y0 - 1, L, nR, nB);
     double phi_m = phase(dens(nR), dens(nB));
-    load_row(a.inR, a.inB, a.S, nx, ny, y0, L, cR_, cB_);
+    load_row(t.inR, t.inB, t.S, nx, ny, t.y0, L, cR_, cB_);
     double rRc = dens(cR_), rBc = dens(cB_);
     double phi_c = phase(rRc, rBc);
 
-    for (int y = y0; y < y1; ++y) {
-        load_row(a.inR, a.inB, a.S, nx, ny, y + 1, L, nR, nB);
+    for (int y = t.y0; y < t.y1; ++y) {
+        // next row's loads go out first; the gradient-independent part of this row's collision
+        // (sums and, for P1/P3, the whole BGK relaxation) runs while they are in flight
+        load_row(t.inR, t.inB, t.S, nx, ny, y + 1, L, nR, nB);
+        Site s;
+        site_sums(s, cR_, cB_, rRc, rBc);
+        if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
+
         const double rRn = dens(nR), rBn = dens(nB);
         const double phi_n = phase(rRn, rBn);
-
         const Nine p9 = gather9(phi_m, phi_c, phi_n, L.lane);
         double gx, gy, gz;
         mediator_gradient(p9, gx, gy, gz);
+        const double mod2 = gx * gx + gy * gy + gz * gz;
+        const double mod = sqrt(mod2);
 
         double Fx = 0.0, Fy = 0.0, Fz = 0.0;
         if constexpr (F3) {
@@ -356,16 +387,28 @@ __global__ void __launch_bounds__(64) step_ring1(CaseConst cc, StepArgs a)
             lap *= 6.0;
             const double mu = 4.0 * cc.bf * ph * (ph * ph - 1.0) - cc.kf * lap;
             Fx = mu * g0; Fy = mu * g1; Fz = mu * g2;
+            bgk<true>(cc, s, Fx, Fy, Fz);
+        } else {
+            perturb(cc, s.f, gx, gy, gz, mod2, mod);
         }
 
         if (L.interior) {
-            const long long site = (long long)y * nx + L.col;
-            collide<F3>(cc, cR_, cB_, rRc, rBc, gx, gy, gz, Fx, Fy, Fz);
-            store_row(a.outR, a.outB, a.S, site, cR_, cB_);
-            if (a.aux) {
-                a.aux[0 * a.S + site] = gx; a.aux[1 * a.S + site] = gy; a.aux[2 * a.S + site] = gz;
-                a.aux[3 * a.S + site] = Fx; a.aux[4 * a.S + site] = Fy; a.aux[5 * a.S + site] = Fz;
+            const unsigned site = (unsigned)(y * nx + L.col);
+            gdouble* __restrict__ oR = t.outR;
+            gdouble* __restrict__ oB = t.outB;
+            const long long S = t.S;
+            auto out = [=](int i, double r, double b) {
+                __builtin_nontemporal_store(r, (oR + i * S) + site);
+                __builtin_nontemporal_store(b, (oB + i * S) + site);
+            };
+            if (!s.skip) {
+                recolour(cc, s, gx, gy, gz, mod, out);
+            } else {   // left untouched (rare)
+                double uR[Q], uB[Q];
+                load_row(t.inR, t.inB, t.S, nx, ny, y, L, uR, uB);
+                store_row(t.outR, t.outB, t.S, site, uR, uB);
             }
+            if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
 #pragma unroll
         for (int i = 0; i < Q; ++i) { cR_[i] = nR[i]; cB_[i] = nB[i]; }
@@ -400,30 +443,31 @@ __device__ __forceinline__ Aux6 aux_of(const CaseConst& cc, double g0, double g1
 }
 
 template <int OP>
-__global__ void __launch_bounds__(64) step_ring2(CaseConst cc, StepArgs a)
+__global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
 {
     constexpr int NA = (OP == OP_F1) ? 3 : 6;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int strip = tile % a.nstrip, seg = tile / a.nstrip;
-    const Lane L = make_lane(cc, strip, 2);
+    TileCtx t;
+    if (!tile_ctx(launch, t)) return;
+    const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
-    const int y0 = seg * a.ly;
-    const int y1 = min(y0 + a.ly, ny);
-    if (y0 >= ny) return;
+    const Lane L = make_lane(nx, t.strip, 2);
 
     double fR[Q], fB[Q];
     auto phi_row = [&](int y) {
-        load_row(a.inR, a.inB, a.S, nx, ny, y, L, fR, fB);
+        load_row(t.inR, t.inB, t.S, nx, ny, y, L, fR, fB);
         return phase(dens(fR), dens(fB));
     };
-    double pa = phi_row(y0 - 2), pb = phi_row(y0 - 1), pc = phi_row(y0), pd = phi_row(y0 + 1);
+    double pa = phi_row(t.y0 - 2), pb = phi_row(t.y0 - 1), pc = phi_row(t.y0), pd = phi_row(t.y0 + 1);
     double gm0, gm1, gm2, g00, g01, g02;
     phase_gradient(gather9(pa, pb, pc, L.lane), gm0, gm1, gm2);     // g at row y0-1
     phase_gradient(gather9(pb, pc, pd, L.lane), g00, g01, g02);     // g at row y0
     Aux6 am = aux_of<OP>(cc, gm0, gm1, gm2), a0 = aux_of<OP>(cc, g00, g01, g02);
 
-    for (int y = y0; y < y1; ++y) {
+    for (int y = t.y0; y < t.y1; ++y) {
         const double pe = phi_row(y + 2);
+        // populations of row y again (read two rows ago for the colour field, now in L2); issued
+        // before the force evaluation so that they arrive while it runs
+        load_row(t.inR, t.inB, t.S, nx, ny, y, L, fR, fB);
         double gp0, gp1, gp2;
         phase_gradient(gather9(pc, pd, pe, L.lane), gp0, gp1, gp2);   // g at row y+1
         const Aux6 ap = aux_of<OP>(cc, gp0, gp1, gp2);
@@ -473,17 +517,25 @@ __global__ void __launch_bounds__(64) step_ring2(CaseConst cc, StepArgs a)
 
         double gx, gy, gz;
         mediator_gradient(gather9(pb, pc, pd, L.lane), gx, gy, gz);
+        const double mod = sqrt(gx * gx + gy * gy + gz * gz);
 
-        // populations of row y again (read two rows ago for the colour field; now an L2 hit)
-        load_row(a.inR, a.inB, a.S, nx, ny, y, L, fR, fB);
         if (L.interior) {
-            const long long site = (long long)y * nx + L.col;
-            collide<true>(cc, fR, fB, dens(fR), dens(fB), gx, gy, gz, Fx, Fy, Fz);
-            store_row(a.outR, a.outB, a.S, site, fR, fB);
-            if (a.aux) {
-                a.aux[0 * a.S + site] = gx; a.aux[1 * a.S + site] = gy; a.aux[2 * a.S + site] = gz;
-                a.aux[3 * a.S + site] = Fx; a.aux[4 * a.S + site] = Fy; a.aux[5 * a.S + site] = Fz;
+            const unsigned site = (unsigned)(y * nx + L.col);
+            Site s;
+            site_sums(s, fR, fB, dens(fR), dens(fB));
+            if (!s.skip) {
+                bgk<true>(cc, s, Fx, Fy, Fz);
+                gdouble* __restrict__ oR = t.outR;
+                gdouble* __restrict__ oB = t.outB;
+                const long long S = t.S;
+                recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) {
+                    __builtin_nontemporal_store(r, (oR + i * S) + site);
+                    __builtin_nontemporal_store(b, (oB + i * S) + site);
+                });
+            } else {
+                store_row(t.outR, t.outB, t.S, site, fR, fB);
             }
+            if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
         pa = pb; pb = pc; pc = pd; pd = pe;
         g00 = gp0; g01 = gp1; g02 = gp2;
@@ -493,24 +545,28 @@ __global__ void __launch_bounds__(64) step_ring2(CaseConst cc, StepArgs a)
 
 }  // namespace
 
-int step_tile_geometry(int op, int nx, int ny, int ly, int* nstrip, int* nseg)
+int step_halo(int op) { return (op == OP_F1 || op == OP_F2) ? 2 : 1; }
+
+int step_kind(int op)
 {
-    const int H = (op == OP_F1 || op == OP_F2) ? 2 : 1;
-    const int WI = 64 - 2 * H;
-    *nstrip = (nx + WI - 1) / WI;
-    *nseg = (ny + ly - 1) / ly;
-    return H;
+    switch (op) {
+    case OP_P1:
+    case OP_P3: return KIND_P;
+    case OP_F3: return KIND_F3;
+    case OP_F1: return KIND_F1;
+    default: return KIND_F2;
+    }
 }
 
-hipError_t launch_step_f64(const CaseConst& cc, const StepArgs& a, hipStream_t stream)
+hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
 {
-    const dim3 grid(a.nstrip * a.nseg), block(64);
-    switch (cc.op) {
-    case OP_P1:
-    case OP_P3: hipLaunchKernelGGL(step_ring1<false>, grid, block, 0, stream, cc, a); break;
-    case OP_F3: hipLaunchKernelGGL(step_ring1<true>, grid, block, 0, stream, cc, a); break;
-    case OP_F1: hipLaunchKernelGGL(step_ring2<OP_F1>, grid, block, 0, stream, cc, a); break;
-    case OP_F2: hipLaunchKernelGGL(step_ring2<OP_F2>, grid, block, 0, stream, cc, a); break;
+    if (ntiles <= 0) return hipSuccess;
+    const dim3 grid(ntiles), block(64);
+    switch (kind) {
+    case KIND_P: hipLaunchKernelGGL(step_ring1<false>, grid, block, 0, stream, a); break;
+    case KIND_F3: hipLaunchKernelGGL(step_ring1<true>, grid, block, 0, stream, a); break;
+    case KIND_F1: hipLaunchKernelGGL(step_ring2<OP_F1>, grid, block, 0, stream, a); break;
+    case KIND_F2: hipLaunchKernelGGL(step_ring2<OP_F2>, grid, block, 0, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -7,20 +7,38 @@
 
 namespace itolbm {
 
-struct StepArgs {
-    const double* inR;   // [19][S] post-collision populations of the previous step (own site)
-    const double* inB;
-    double* outR;        // [19][S] post-collision populations of this step
-    double* outB;
-    double* aux;         // nullptr, or [6][S]: mediator momentum gm xyz and force F xyz
-    long long S;         // population stride, >= nx * ny
+// kernel families: one launch per family advances every case of that family by one step
+enum { KIND_P = 0, KIND_F3 = 1, KIND_F1 = 2, KIND_F2 = 3, N_KINDS = 4 };
+
+// Device-resident entry of one case in a launch's case table.
+struct CaseDev {
+    CaseConst cc;
+    void* pop[2];        // post-collision state buffers: pop[parity] is read, pop[parity ^ 1] written
+    double* aux;         // [6][S]: mediator momentum gm xyz, force F xyz (diagnostic steps), or nullptr
+    long long S;         // population stride (elements)
     int ly;              // rows per wavefront segment
-    int nstrip, nseg;    // tiles: strips of 64 - 2H columns x segments of ly rows
+    int nstrip, nseg;    // tiles of the case: strips of 64 - 2H columns x segments of ly rows
+    int tile0;           // first tile of the case in the launch
 };
 
-// tiles of one case; returns the halo width H of the operator
-int step_tile_geometry(int op, int nx, int ny, int ly, int* nstrip, int* nseg);
+struct StepLaunch {
+    const CaseDev* tab;      // case table
+    const int* tile_case;    // tile -> case index
+    int parity;              // which of pop[0] / pop[1] holds the input state
+    int record;              // write aux for cases that have it
+};
 
-hipError_t launch_step_f64(const CaseConst& cc, const StepArgs& a, hipStream_t stream);
+int step_halo(int op);
+int step_kind(int op);
+
+hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
+
+// FP32-compute forms; storage FP32 (elem = 4) or FP16 (elem = 2): per population the pair
+// (s_i / w_i, e_i), s_i = fR_i + fB_i - w_i, e_i = (fR_i - fB_i) / (fR_i + fB_i), at ((y * 19 + i) nx + x)
+hipError_t launch_step_lp(int elem, int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
+
+// low-precision state (pairs (A_i, e_i) at ((y * 19 + i) nx + x)) -> FP64 colour populations fR, fB [19][S]
+hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long long n, double* fR, double* fB,
+                            hipStream_t stream);
 
 }  // namespace itolbm

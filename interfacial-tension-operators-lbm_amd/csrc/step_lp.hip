@@ -1,0 +1,581 @@
+// FP32-compute step kernels with FP32 or FP16 population storage: the same fused pass as the FP64
+// kernels (pull-stream, colour field, mediator gradient, tension operator, BGK + Guo, recolouring),
+// reformulated in deviation variables so that single and half precision keep their digits.
+//
+// Storage: one pair (A_i, e_i) per site and velocity i, in rows: pair(i, x, y) at ((y * 19 + i) nx + x).
+// A row of one velocity is contiguous (coalesced pulls and stores) and the 19 velocities of a row sit
+// together (few pages per wavefront, so few translation misses):
+//   A_i = s_i / w_i,   s_i = fR_i + fB_i - w_i      deviation of the total population from w_i (rho0 = 1)
+//   e_i = (fR_i - fB_i) / (fR_i + fB_i)             colour fraction of the population, exactly +-1 in a
+//                                                   pure phase
+// From these a site has f_i = w_i (1 + A_i), rho = 1 + sum_i s_i, rho_R - rho_B = sum_i e_i f_i and
+// phi = rho^N = (rho_R - rho_B) / rho; the colour populations are fR_i = f_i (1 + e_i) / 2 and
+// fB_i = f_i (1 - e_i) / 2.  Collision and recolouring (colisao_comum, Operadores_tensao.cpp:340-373;
+// coll_BGK, Collision_operators.cpp:11-49; recolloring, LBM_functions.cpp:1712-1746) act on s_i:
+//   s_i += (s_i^eq - s_i) / tau + S_i (+ A|g|(w_i (c_i.n)^2 - B_i) for P1/P3),
+//   s_i^eq = w_i [ (rho - 1) + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2) ],
+//   e_i = phi - 2 beta cR cB w_i rho cos(c_i, gm) / f_i            (f_i post-collision).
+// On the nz = 1 slab the isotropic stencils reduce to in-plane differences with weights
+// w_axis = W1 + 2 W2 = 1/9 and w_diag = W2 = 1/36, and their z components vanish exactly.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+
+#include "lattice.h"
+#include "step_kernels.h"
+#include "tile.h"
+
+namespace itolbm {
+
+namespace {
+
+constexpr float WF[Q] = {(float)W0, (float)W1, (float)W1, (float)W1, (float)W1, (float)W1, (float)W1,
+                         (float)W2, (float)W2, (float)W2, (float)W2, (float)W2, (float)W2, (float)W2,
+                         (float)W2, (float)W2, (float)W2, (float)W2, (float)W2};
+constexpr float WAX = (float)(W1 + 2.0 * W2);   // in-plane axis weight of the folded stencil
+constexpr float WDG = (float)W2;                // in-plane diagonal weight
+constexpr float ISQ2 = 0.70710678118654752f;    // 1 / |c| for the diagonals
+
+template <typename T>
+using gcst = __attribute__((address_space(1))) const T;
+template <typename T>
+using gst = __attribute__((address_space(1))) T;
+using gdouble = __attribute__((address_space(1))) double;
+
+// Stored pair of one population: FP32 (A, e) as 8 bytes, FP16 (A, e) packed in 4 bytes.
+template <typename ST> struct PairOf;
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <> struct PairOf<float> {
+    using raw = f32x2;
+    __device__ static __forceinline__ float a(raw r) { return r.x; }
+    __device__ static __forceinline__ float e(raw r) { return r.y; }
+    __device__ static __forceinline__ raw make(float a, float e) { return raw{a, e}; }
+};
+template <> struct PairOf<_Float16> {
+    using raw = unsigned int;
+    __device__ static __forceinline__ float a(raw r) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(r & 0xffffu)); }
+    __device__ static __forceinline__ float e(raw r) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(r >> 16)); }
+    __device__ static __forceinline__ raw make(float a, float e)
+    {
+        const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)a);
+        const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)e);
+        return lo | (hi << 16);
+    }
+};
+
+// pulled populations of one site in deviation form
+struct Row {
+    float s[Q];   // s_i = f_i - w_i
+    float e[Q];   // colour fraction
+};
+
+// Element at a 32-bit byte offset from a uniform base: lets the compiler address with the base in
+// SGPRs and one 32-bit VGPR offset (no 64-bit per-lane address arithmetic).  A row of one case is
+// < 4 GiB, so the offset fits.
+template <typename T>
+__device__ __forceinline__ T at_byte(gcst<T>* base, unsigned off)
+{
+    return *(gcst<T>*)((__attribute__((address_space(1))) const char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void put_byte(gst<T>* base, unsigned off, T v)
+{
+    __builtin_nontemporal_store(v, (gst<T>*)((__attribute__((address_space(1))) char*)base + off));
+}
+
+// Issue the pulls of site (col, y): raw pair i from row y - c_iy, column col - c_ix.
+template <typename ST>
+__device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restrict__ in, int nx, int ny, int y,
+                                         const Lane& L, typename PairOf<ST>::raw r[Q])
+{
+    using R = typename PairOf<ST>::raw;
+    const long long pitch = (long long)Q * nx;
+    gcst<R>* rowm = in + wrapi(y - 1, ny) * pitch;
+    gcst<R>* row0 = in + wrapi(y, ny) * pitch;
+    gcst<R>* rowp = in + wrapi(y + 1, ny) * pitch;
+    // per-velocity bases are uniform (SGPRs); the lane part is one of three byte offsets
+    const unsigned om = (unsigned)L.xm * (unsigned)sizeof(R), o0 = (unsigned)L.col * (unsigned)sizeof(R),
+                   op = (unsigned)L.xp * (unsigned)sizeof(R);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : o0);
+        gcst<R>* base = (CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0)) + i * nx;
+        r[i] = at_byte(base, xs);
+    }
+}
+
+template <typename ST>
+__device__ __forceinline__ void decode(const typename PairOf<ST>::raw r[Q], Row& row)
+{
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        row.s[i] = PairOf<ST>::a(r[i]) * WF[i];
+        row.e[i] = PairOf<ST>::e(r[i]);
+    }
+}
+
+// density deviation, rho and phi of a site.  phi sums e_i f_i and f_i in the same order, so a pure
+// phase (all e_i = +-1) gives phi = +-1 exactly and no colour leaks into it; the density used by the
+// collision comes from the deviations, which keep more digits.
+__device__ __forceinline__ void row_phase(const Row& r, float& ds, float& rho, float& phi)
+{
+    float a = 0.f, D = 0.f, F = 0.f;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const float f = r.s[i] + WF[i];
+        a += r.s[i];
+        D += r.e[i] * f;
+        F += f;
+    }
+    ds = a;
+    rho = 1.f + a;
+    phi = D / F;
+}
+
+template <typename ST>
+__device__ __forceinline__ float raw_phase(const typename PairOf<ST>::raw r[Q])
+{
+    Row row;
+    decode<ST>(r, row);
+    float ds, rho, phi;
+    row_phase(row, ds, rho, phi);
+    return phi;
+}
+
+__device__ __forceinline__ float shf(float v, int src) { return __shfl(v, src, 64); }
+
+struct Nine {
+    float v[3][3];
+    __device__ __forceinline__ float at(int dx, int dy) const { return v[dy + 1][dx + 1]; }
+};
+
+__device__ __forceinline__ Nine gather9(float rm, float r0, float rp, int lane)
+{
+    Nine n;
+    const float rows[3] = {rm, r0, rp};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        n.v[k][0] = shf(rows[k], lane - 1);
+        n.v[k][1] = rows[k];
+        n.v[k][2] = shf(rows[k], lane + 1);
+    }
+    return n;
+}
+
+// isotropic in-plane derivative 3 sum_i w_i c_i f(x + c_i), folded onto the slab
+__device__ __forceinline__ float dx_iso(const Nine& f)
+{
+    return 3.f * (WAX * (f.at(1, 0) - f.at(-1, 0)) +
+                  WDG * ((f.at(1, 1) - f.at(-1, -1)) + (f.at(1, -1) - f.at(-1, 1))));
+}
+__device__ __forceinline__ float dy_iso(const Nine& f)
+{
+    return 3.f * (WAX * (f.at(0, 1) - f.at(0, -1)) +
+                  WDG * ((f.at(1, 1) - f.at(-1, -1)) - (f.at(1, -1) - f.at(-1, 1))));
+}
+// 6 sum_i w_i (f(x + c_i) - f(x))
+__device__ __forceinline__ float lap_iso(const Nine& f)
+{
+    const float c = f.at(0, 0);
+    return 6.f * (WAX * ((f.at(1, 0) + f.at(-1, 0)) + (f.at(0, 1) + f.at(0, -1)) - 4.f * c) +
+                  WDG * ((f.at(1, 1) + f.at(-1, -1)) + (f.at(1, -1) + f.at(-1, 1)) - 4.f * c));
+}
+
+struct ConstF {
+    float oot, A, B0, B1, B2, beta, hs, kf, bf, sf0, sf1, sf2;
+};
+
+__device__ __forceinline__ ConstF to_float(const CaseConst& c)
+{
+    ConstF f;
+    f.oot = (float)c.one_over_tau; f.A = (float)c.A; f.B0 = (float)c.B0; f.B1 = (float)c.B1; f.B2 = (float)c.B2;
+    f.beta = (float)c.beta; f.hs = (float)c.hs; f.kf = (float)c.kf; f.bf = (float)c.bf;
+    f.sf0 = (float)c.sfac0; f.sf1 = (float)c.sfac1; f.sf2 = (float)c.sfac2;
+    return f;
+}
+
+// Collision of one site given its pulled row, phi, the mediator gradient gm = -grad(phi) and the
+// force (class F); writes the post-collision s_i and e_i through out(i, s_i, e_i).
+template <bool FORCED, class Out>
+__device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float ds, float rho, float phi, float gx,
+                                           float gy, float Fx, float Fy, Out out)
+{
+    // momentum (sum_i c_i w_i = 0, so it is the momentum of the deviations)
+    float jx = 0.f, jy = 0.f;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        if (CX[i] > 0) jx += r.s[i];
+        if (CX[i] < 0) jx -= r.s[i];
+        if (CY[i] > 0) jy += r.s[i];
+        if (CY[i] < 0) jy -= r.s[i];
+    }
+    const float irho = __builtin_amdgcn_rcpf(rho);
+    float ux = jx * irho, uy = jy * irho;
+    if constexpr (FORCED) {
+        ux += 0.5f * Fx * irho;
+        uy += 0.5f * Fy * irho;
+    }
+    const float usq = ux * ux + uy * uy;
+    const float vF = ux * Fx + uy * Fy;
+    float s[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const float cu = (float)CX[i] * ux + (float)CY[i] * uy;
+        const float seq = WF[i] * (ds + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
+        s[i] = r.s[i] + (seq - r.s[i]) * k.oot;
+        if constexpr (FORCED) {
+            const float cF = (float)CX[i] * Fx + (float)CY[i] * Fy;
+            const float sf = (i == 0) ? k.sf0 : (i < 7 ? k.sf1 : k.sf2);
+            s[i] += sf * (cF + 3.f * cF * cu - vF);
+        }
+    }
+    const float mod2 = gx * gx + gy * gy;
+    const bool has_g = mod2 > 0.f;
+    const float imod = has_g ? rsqrtf(mod2) : 0.f;
+    const float nx_ = gx * imod, ny_ = gy * imod;
+    if constexpr (!FORCED) {
+        // perturbation: A |g| (w_i (c_i.n)^2 - B_i)
+        if (has_g && k.A != 0.f) {
+            const float Am = k.A * mod2 * imod;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const float cn = (float)CX[i] * nx_ + (float)CY[i] * ny_;
+                const float B = (i == 0) ? k.B0 : (i < 7 ? k.B1 : k.B2);
+                s[i] += Am * (WF[i] * cn * cn - B);
+            }
+        }
+    }
+    // recolouring: e_i = phi - 2 beta cR cB w_i rho cos_i / f_i
+    const float cR = 0.5f * (1.f + phi), cB = 0.5f * (1.f - phi);
+    const float kk = 2.f * k.beta * cR * cB * rho;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const float f = s[i] + WF[i];
+        float e = phi;
+        if (i > 0 && has_g) {
+            const float cn = (float)CX[i] * nx_ + (float)CY[i] * ny_;
+            const float cosp = (i < 7) ? cn : cn * ISQ2;
+            e = (f != 0.f) ? phi - kk * WF[i] * cosp * __builtin_amdgcn_rcpf(f) : phi;
+        }
+        out(i, s[i], e);
+    }
+}
+
+struct TileCtxLP {
+    CaseConst cc;
+    const void* in;
+    void* out;
+    gdouble* aux;
+    long long S;
+    int strip, y0, y1;
+};
+
+__device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtxLP& t)
+{
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int c = L.tile_case[tile];
+    const CaseDev& d = L.tab[c];
+    t.cc = d.cc;
+    t.in = d.pop[L.parity];
+    t.out = d.pop[L.parity ^ 1];
+    t.S = d.S;
+    t.aux = L.record ? (gdouble*)d.aux : nullptr;
+    const int local = tile - d.tile0;
+    t.strip = local % d.nstrip;
+    const int seg = local / d.nstrip;
+    t.y0 = seg * d.ly;
+    t.y1 = min(t.y0 + d.ly, d.cc.ny);
+    return t.y0 < d.cc.ny;
+}
+
+__device__ __forceinline__ void store_aux(gdouble* __restrict__ aux, long long S, unsigned site, float gx, float gy,
+                                          float Fx, float Fy)
+{
+    (aux + 0 * S)[site] = gx; (aux + 1 * S)[site] = gy; (aux + 2 * S)[site] = 0.0;
+    (aux + 3 * S)[site] = Fx; (aux + 4 * S)[site] = Fy; (aux + 5 * S)[site] = 0.0;
+}
+
+// in-plane field the class-F force differentiates: n = g/|g| (F1) or T = (sigma/2)(|g| I - g g/|g|)
+// with components xx, yy, xy (F2); g = grad(phi)
+struct AuxF { float g0, g1, a0, a1, a2; };
+
+template <int KIND>
+__device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
+{
+    AuxF a;
+    a.g0 = dx_iso(p);
+    a.g1 = dy_iso(p);
+    const float m = sqrtf(a.g0 * a.g0 + a.g1 * a.g1);
+    if constexpr (KIND == KIND_F1) {
+        const float inv = (m > 1.0e-12f) ? 1.f / m : 0.f;
+        a.a0 = a.g0 * inv; a.a1 = a.g1 * inv; a.a2 = 0.f;
+    } else {
+        const float inv = (m > 1.0e-30f) ? 1.f / m : 0.f;
+        a.a0 = k.hs * (m - a.g0 * a.g0 * inv);
+        a.a1 = k.hs * (m - a.g1 * a.g1 * inv);
+        a.a2 = -k.hs * a.g0 * a.g1 * inv;
+    }
+    return a;
+}
+
+// Strip-marching kernel with a ring of NS row buffers per wavefront.  When row y is collided, rows
+// y .. y+H have arrived (their colour field is known) and rows y+H+1 .. y+NS-1 are still in flight,
+// so every wavefront keeps NS-H-1 rows of loads outstanding while it computes.  The row loop is
+// unrolled NS times so that every buffer index is a compile-time constant (registers, no scratch).
+template <typename ST, int KIND, int NS>
+__global__ void __launch_bounds__(64) lp_march(StepLaunch launch)
+{
+    constexpr int H = (KIND == KIND_F1 || KIND == KIND_F2) ? 2 : 1;
+    static_assert(NS >= 2 * H + 1, "the colour-field ring must hold rows y-H .. y+H");
+    using R = typename PairOf<ST>::raw;
+    TileCtxLP t;
+    if (!tile_ctx(launch, t)) return;
+    const int nx = t.cc.nx, ny = t.cc.ny;
+    const ConstF k = to_float(t.cc);
+    const Lane L = make_lane(nx, t.strip, H);
+    gcst<R>* in = (gcst<R>*)t.in;
+    gst<R>* out = (gst<R>*)t.out;
+    const int y0 = t.y0, y1 = t.y1;
+    const int rlast = y1 - 1 + H;        // last row whose colour field is needed
+
+    R slot[NS][Q];                       // raw rows; row y >= y0 lives in slot (y - y0) % NS
+    float ph[NS];                        // colour field; row r >= y0 - H at index (r - y0) mod NS
+    AuxF ax[NS];                         // class F1/F2: aux at row r, same indexing
+
+    // halo rows above the segment: colour field only
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+        load_raw<ST>(in, nx, ny, y0 - H + j, L, slot[0]);
+        ph[NS - H + j] = raw_phase<ST>(slot[0]);
+    }
+    // fill the ring
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (y0 + s <= rlast) load_raw<ST>(in, nx, ny, y0 + s, L, slot[s]);
+#pragma unroll
+    for (int s = 0; s < H; ++s) ph[s] = raw_phase<ST>(slot[s]);
+    if constexpr (H == 2) {
+        ax[NS - 1] = aux_of<KIND>(k, gather9(ph[NS - 2], ph[NS - 1], ph[0], L.lane));   // row y0 - 1
+        ax[0] = aux_of<KIND>(k, gather9(ph[NS - 1], ph[0], ph[1], L.lane));              // row y0
+    }
+
+    for (int yb = y0; yb < y1; yb += NS) {
+#pragma unroll
+        for (int kq = 0; kq < NS; ++kq) {
+            const int y = yb + kq;
+            if (y < y1) {
+                const int sa = (kq + H) % NS;            // row y + H arrives
+                ph[sa] = raw_phase<ST>(slot[sa]);
+                const int im = (kq + NS - 1) % NS, ip = (kq + 1) % NS;
+                const Nine p9 = gather9(ph[im], ph[kq], ph[ip], L.lane);
+                const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
+                float Fx = 0.f, Fy = 0.f;
+                if constexpr (KIND == KIND_F3) {
+                    const float phc = ph[kq];
+                    const float mu = 4.f * k.bf * phc * (phc * phc - 1.f) - k.kf * lap_iso(p9);
+                    Fx = -mu * gx;
+                    Fy = -mu * gy;
+                } else if constexpr (H == 2) {
+                    ax[ip] = aux_of<KIND>(k, gather9(ph[kq], ph[ip], ph[sa], L.lane));     // row y + 1
+                    const AuxF& am = ax[im];
+                    const AuxF& a0 = ax[kq];
+                    const AuxF& ap = ax[ip];
+                    if constexpr (KIND == KIND_F1) {
+                        const float divn = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) +
+                                           dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
+                        Fx = -k.hs * divn * a0.g0;
+                        Fy = -k.hs * divn * a0.g1;
+                    } else {
+                        const Nine txy = gather9(am.a2, a0.a2, ap.a2, L.lane);
+                        Fx = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) + dy_iso(txy);
+                        Fy = dx_iso(txy) + dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
+                    }
+                }
+                if (L.interior) {
+                    // the decode at arrival must not be reused here (that would keep 38 decoded values
+                    // of every row in flight live): an empty asm makes the raw registers opaque
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) asm volatile("" : "+v"(slot[kq][i]));
+                    Row cur;
+                    decode<ST>(slot[kq], cur);
+                    float ds, rho, phi;
+                    row_phase(cur, ds, rho, phi);
+                    const unsigned base = (unsigned)(L.col);
+                    gst<R>* orow = out + (long long)y * ((long long)Q * nx);
+                    auto st = [&](int i, float s, float e) {
+                        put_byte(orow, ((unsigned)(i * nx) + base) * (unsigned)sizeof(R), PairOf<ST>::make(s * (1.f / WF[i]), e));
+                    };
+                    if (rho > 1.0e-30f) {
+                        collide_lp<KIND != KIND_P>(k, cur, ds, rho, phi, gx, gy, Fx, Fy, st);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < Q; ++i) st(i, cur.s[i], cur.e[i]);
+                    }
+                    if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
+                }
+                if (y + NS <= rlast) load_raw<ST>(in, nx, ny, y + NS, L, slot[kq]);
+            }
+            // keep the unrolled rows apart: without this the scheduler interleaves them and the
+            // live registers multiply
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// One-row-ahead strip kernel: the loads of row y+1 (whose colour field row y needs) are in flight
+// while ... they are waited for, then row y is collided.  Two raw rows per lane (cur, next).
+template <typename ST, int KIND>
+__global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
+{
+    constexpr int H = (KIND == KIND_F1 || KIND == KIND_F2) ? 2 : 1;
+    using R = typename PairOf<ST>::raw;
+    TileCtxLP t;
+    if (!tile_ctx(launch, t)) return;
+    const int nx = t.cc.nx, ny = t.cc.ny;
+    const ConstF k = to_float(t.cc);
+    const Lane L = make_lane(nx, t.strip, H);
+    gcst<R>* in = (gcst<R>*)t.in;
+    gst<R>* out = (gst<R>*)t.out;
+    const int y0 = t.y0, y1 = t.y1;
+
+    R cur[Q], nxt[Q];
+    float p[2 * H + 1];                  // colour field of rows y-H .. y+H (p[H] = row y)
+    AuxF am{}, a0{};
+#pragma unroll
+    for (int j = 0; j < 2 * H; ++j) {
+        load_raw<ST>(in, nx, ny, y0 - H + j, L, nxt);
+        p[j + 1] = raw_phase<ST>(nxt);   // rows y0-H .. y0+H-1 at p[1 .. 2H]
+    }
+    if constexpr (H == 2) {
+        am = aux_of<KIND>(k, gather9(p[1], p[2], p[3], L.lane));   // row y0 - 1
+        a0 = aux_of<KIND>(k, gather9(p[2], p[3], p[4], L.lane));   // row y0
+    }
+    load_raw<ST>(in, nx, ny, y0, L, cur);
+
+    for (int y = y0; y < y1; ++y) {
+        load_raw<ST>(in, nx, ny, y + H, L, nxt);
+        if constexpr (H == 2) {
+            // cur was loaded for row y two iterations ago only if H == 1; reload it (L2) for F1/F2
+        }
+#pragma unroll
+        for (int j = 0; j < 2 * H; ++j) p[j] = p[j + 1];
+        p[2 * H] = raw_phase<ST>(nxt);   // row y + H
+        const Nine p9 = gather9(p[H - 1], p[H], p[H + 1], L.lane);
+        const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
+        float Fx = 0.f, Fy = 0.f;
+        if constexpr (KIND == KIND_F3) {
+            const float phc = p[H];
+            const float mu = 4.f * k.bf * phc * (phc * phc - 1.f) - k.kf * lap_iso(p9);
+            Fx = -mu * gx;
+            Fy = -mu * gy;
+        } else if constexpr (H == 2) {
+            const AuxF ap = aux_of<KIND>(k, gather9(p[2], p[3], p[4], L.lane));    // row y + 1
+            if constexpr (KIND == KIND_F1) {
+                const float divn = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) + dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
+                Fx = -k.hs * divn * a0.g0;
+                Fy = -k.hs * divn * a0.g1;
+            } else {
+                const Nine txy = gather9(am.a2, a0.a2, ap.a2, L.lane);
+                Fx = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) + dy_iso(txy);
+                Fy = dx_iso(txy) + dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
+            }
+            am = a0;
+            a0 = ap;
+        }
+        if (L.interior) {
+            if constexpr (H == 2) load_raw<ST>(in, nx, ny, y, L, cur);
+            Row row;
+            decode<ST>(cur, row);
+            float ds, rho, phi;
+            row_phase(row, ds, rho, phi);
+            gst<R>* orow = out + (long long)y * ((long long)Q * nx);
+            const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
+            auto st = [&](int i, float s, float e) {
+                put_byte(orow + i * nx, ocol, PairOf<ST>::make(s * (1.f / WF[i]), e));
+            };
+            if (rho > 1.0e-30f) {
+                collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+            } else {
+#pragma unroll
+                for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
+            }
+            if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
+        }
+        if constexpr (H == 1) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
+        }
+    }
+}
+
+template <typename ST>
+hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
+{
+    const dim3 grid(ntiles), block(64);
+    static const int ring = [] {
+        const char* e = std::getenv("ITOLBM_RING");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (ring) {
+        switch (kind) {
+        case KIND_P: hipLaunchKernelGGL((lp_march<ST, KIND_P, 4>), grid, block, 0, stream, a); break;
+        case KIND_F3: hipLaunchKernelGGL((lp_march<ST, KIND_F3, 4>), grid, block, 0, stream, a); break;
+        case KIND_F1: hipLaunchKernelGGL((lp_march<ST, KIND_F1, 5>), grid, block, 0, stream, a); break;
+        case KIND_F2: hipLaunchKernelGGL((lp_march<ST, KIND_F2, 5>), grid, block, 0, stream, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    switch (kind) {
+    case KIND_P: hipLaunchKernelGGL((lp_step<ST, KIND_P>), grid, block, 0, stream, a); break;
+    case KIND_F3: hipLaunchKernelGGL((lp_step<ST, KIND_F3>), grid, block, 0, stream, a); break;
+    case KIND_F1: hipLaunchKernelGGL((lp_step<ST, KIND_F1>), grid, block, 0, stream, a); break;
+    case KIND_F2: hipLaunchKernelGGL((lp_step<ST, KIND_F2>), grid, block, 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ conversions for transfers / diagnostics
+// (A, e) pairs in rows -> fR, fB in FP64, [19][S] each (own site, post-collision)
+template <typename ST>
+__global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw* __restrict__ pop, int nx, long long S,
+                                                 long long n, double* __restrict__ fR, double* __restrict__ fB)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long y = p / nx, x = p - y * nx;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const auto r = pop[(y * Q + i) * nx + x];
+            const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+            const double f = w * (1.0 + (double)PairOf<ST>::a(r));
+            const double e = (double)PairOf<ST>::e(r);
+            fR[i * S + p] = 0.5 * f * (1.0 + e);
+            fB[i * S + p] = 0.5 * f * (1.0 - e);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_step_lp(int elem, int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (elem == 4) return launch_typed<float>(kind, a, ntiles, stream);
+    if (elem == 2) return launch_typed<_Float16>(kind, a, ntiles, stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long long n, double* fR, double* fB,
+                            hipStream_t stream)
+{
+    const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+    if (elem == 4)
+        hipLaunchKernelGGL((lp_to_f64<float>), dim3(nb), dim3(256), 0, stream, (const f32x2*)pop, nx, S, n, fR, fB);
+    else
+        hipLaunchKernelGGL((lp_to_f64<_Float16>), dim3(nb), dim3(256), 0, stream, (const unsigned*)pop, nx, S, n, fR,
+                           fB);
+    return hipGetLastError();
+}
+
+}  // namespace itolbm
