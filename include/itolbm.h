@@ -144,6 +144,15 @@ int lbm_sync(lbm_ctx* ctx);
  * reference counterpart: it lets a caller order the steps with its own work and time them. */
 int lbm_set_stream(lbm_ctx* ctx, void* hip_stream);
 
+/* Reduced-precision forms (LBM_PREC_F32 / LBM_PREC_F16) store each population as its deviation from
+ * w_i (1 + r(e)), r(e) = (rho_red + rho_blue) / 2 - 1 + e (rho_red - rho_blue) / 2, where e is the
+ * population's colour fraction (+1 pure red, -1 pure blue).  Setting rho_red / rho_blue close to the
+ * bulk phase densities (for a drop: 3 p_in and 3 p_out from lbm_diag_drop) keeps the stored numbers
+ * small, so FP16 keeps its relative digits where the pressure is measured.  The stored state is
+ * re-expressed in place (stream-ordered); no effect on LBM_PREC_F64.  Initial reference: 1, 1.  No
+ * reference counterpart (the reference stores FP64 populations). */
+int lbm_set_reference(lbm_ctx* ctx, int case_id, double rho_red, double rho_blue);
+
 /* The last recorded step's mediator momentum gm = -grad(phi) (momentum of inif_m) and force field
  * (ini_force), each [site][3]; either pointer may be NULL. */
 int lbm_download_aux(lbm_ctx* ctx, int case_id, double* gm, double* F);

@@ -25,8 +25,8 @@ STEP_RECORD_AUX = 1
 
 # every symbol include/itolbm.h declares
 SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "lbm_upload_state",
-           "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case", "lbm_sync", "lbm_set_stream", "lbm_download_aux", "lbm_diag_drop",
-           "lbm_diag_incl"]
+           "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case", "lbm_sync", "lbm_set_stream", "lbm_set_reference",
+           "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl"]
 
 
 class CaseDesc(C.Structure):
@@ -87,6 +87,7 @@ def lib():
         L.lbm_step_case.argtypes = [P, C.c_int, C.c_int, C.c_int]
         L.lbm_sync.argtypes = [P]
         L.lbm_set_stream.argtypes = [P, C.c_void_p]
+        L.lbm_set_reference.argtypes = [P, C.c_int, C.c_double, C.c_double]
         L.lbm_download_aux.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
         L.lbm_diag_drop.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(DropDiag)]
         L.lbm_diag_incl.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(InclDiag), C.c_void_p]
@@ -191,6 +192,10 @@ class Context:
 
     def sync(self):
         self._chk(lib().lbm_sync(self.h))
+
+    def set_reference(self, rho_red, rho_blue, case=0):
+        """Reference phase densities of the FP32/FP16 deviation storage (no effect on FP64)."""
+        self._chk(lib().lbm_set_reference(self.h, case, float(rho_red), float(rho_blue)))
 
     def aux(self, case=0):
         n = self.n_sites(case)

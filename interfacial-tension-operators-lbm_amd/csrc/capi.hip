@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <string>
@@ -254,11 +255,14 @@ long long gcdll(long long a, long long b)
     return a;
 }
 
-// (fR, fB) [colour][19][S] FP64 at own site -> pairs (A_i = s_i / w_i, e_i) at ((y * 19 + i) nx + x) in the
-// case's storage type
+// (fR, fB) [colour][19][S] FP64 at own site -> pairs (h_i, e_i) at ((y * 19 + i) nx + x) in the case's
+// storage type: e_i = (fR - fB) / (fR + fB), h_i = (f_i - w_i) / w_i - r(e_i) with the case's reference
+// density deviation r(e) = ref_mid + ref_half e.  The FP16 form keeps h as a half and e as the signed
+// minority fraction copysign((1 - |e|) / 2, e) as a half (the device encoding of the same pair).
 int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
 {
     const int nx = k.d.nx;
+    const float rmid = (float)k.cc.ref_mid, rhalf = (float)k.cc.ref_half;
     std::vector<float> a(2 * (size_t)Q * k.n);
     for (long long p = 0; p < k.n; ++p) {
         const long long y = p / nx, x = p - y * nx;
@@ -267,17 +271,26 @@ int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
             const double r = h[(size_t)i * k.S + p], b = h[(size_t)(Q + i) * k.S + p];
             const double f = r + b;
             const size_t j = 2 * (size_t)((y * Q + i) * nx + x);
-            a[j] = (float)((f - w) / w);
-            a[j + 1] = (float)((f != 0.0) ? (r - b) / f : 0.0);
+            const float e = (float)((f != 0.0) ? (r - b) / f : 0.0);
+            a[j] = (float)((f - w) / w) - (rmid + rhalf * e);
+            a[j + 1] = e;
         }
     }
     if (k.elem == 4) {
         HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     } else {
-        std::vector<_Float16> hh(a.size());
-        for (size_t j = 0; j < a.size(); ++j) hh[j] = (_Float16)a[j];
-        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], hh.data(), sizeof(_Float16) * hh.size(), hipMemcpyHostToDevice,
+        std::vector<uint32_t> hh(a.size() / 2);
+        for (size_t j = 0; j < hh.size(); ++j) {
+            const float e = a[2 * j + 1];
+            const float m = std::fmax(0.5f * (1.0f - std::fabs(e)), 0.0f);
+            const _Float16 lo = (_Float16)a[2 * j], hi = (_Float16)std::copysign(m, e);
+            uint16_t l16, h16;
+            std::memcpy(&l16, &lo, 2);
+            std::memcpy(&h16, &hi, 2);
+            hh[j] = (uint32_t)l16 | ((uint32_t)h16 << 16);
+        }
+        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], hh.data(), sizeof(uint32_t) * hh.size(), hipMemcpyHostToDevice,
                                  c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
@@ -294,7 +307,8 @@ int state_f64(lbm_ctx* c, CaseState& k, const double** out)
     if (!k.f64) {
         if (int r = dev_alloc(c, (void**)&k.f64, sizeof(double) * 2 * Q * (size_t)k.S)) return r;
     }
-    HIPCHK(c, launch_lp_to_f64(k.elem, k.pop[k.cur], k.d.nx, k.S, k.n, k.f64, k.f64 + Q * k.S, c->stream));
+    HIPCHK(c, launch_lp_to_f64(k.elem, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
+                               k.f64 + Q * k.S, c->stream));
     *out = k.f64;
     return LBM_OK;
 }
@@ -642,6 +656,23 @@ int lbm_set_stream(lbm_ctx* c, void* s)
     if (!c) return LBM_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->stream = s ? (hipStream_t)s : c->own_stream;
+    return LBM_OK;
+}
+
+int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
+{
+    if (int r = check_case(c, id)) return r;
+    if (!(std::isfinite(rho_red) && std::isfinite(rho_blue) && rho_red > 0.0 && rho_blue > 0.0))
+        return fail(c, LBM_E_ARG, "reference densities must be finite and positive");
+    CaseState& k = c->cases[id];
+    if (k.elem == 8) return LBM_OK;   // FP64 stores the populations themselves
+    const double mid = 0.5 * (rho_red + rho_blue) - 1.0, half = 0.5 * (rho_red - rho_blue);
+    HIPCHK(c, launch_lp_recentre(k.elem, k.pop[k.cur], (long long)Q * k.n, k.cc.ref_mid, k.cc.ref_half, mid, half,
+                                 c->stream));
+    k.cc.ref_mid = mid;
+    k.cc.ref_half = half;
+    c->family[k.kind].written = false;
+    c->single[id].written = false;
     return LBM_OK;
 }
 

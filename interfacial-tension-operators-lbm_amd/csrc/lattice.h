@@ -37,6 +37,9 @@ struct CaseConst {
     double hs;             // 0.5 * sigma          (F1 / F2)
     double kf, bf;         // 0.75 sigma xi, 3 sigma / (8 xi)   (F3)
     double sfac0, sfac1, sfac2;   // (1 - 1/(2 tau)) * w * 3   (Guo source, LBM_functions.cpp:1641-1669)
+    // reduced-precision forms only: stored deviations are taken from the reference density
+    // 1 + ref_mid + ref_half e (e = +1 pure red, -1 pure blue); zero for FP64
+    double ref_mid, ref_half;
 };
 
 }  // namespace itolbm

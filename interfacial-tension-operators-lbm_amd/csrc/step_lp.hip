@@ -42,7 +42,16 @@ template <typename T>
 using gst = __attribute__((address_space(1))) T;
 using gdouble = __attribute__((address_space(1))) double;
 
-// Stored pair of one population: FP32 (A, e) as 8 bytes, FP16 (A, e) packed in 4 bytes.
+// Stored pair of one population, (h_i, e_i) with h_i = A_i - r(e_i), where r(e) = ref_mid + ref_half e
+// is the case's reference density deviation (1 + r(+1) for pure red, 1 + r(-1) for pure blue, set
+// from measured phase densities; zero by default), so that the stored number is small in both bulk
+// phases and keeps its relative digits.
+//   FP32 form, 8 bytes: h and e as floats.
+//   FP16 form, 4 bytes: h as an IEEE half in the low 16 bits; in the high 16 bits the minority colour
+//   fraction m = (1 - |e|) / 2 as a half whose sign bit is that of e, e = copysign(1 - 2m, e).  A
+//   pure phase (m = +-0) decodes to e = +-1 exactly and the thin colour tails of an interface keep
+//   their relative precision, so rounding does not bias the colour masses.  |e| > 1 (a negative
+//   colour population) is stored as |e| = 1.
 template <typename ST> struct PairOf;
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 template <> struct PairOf<float> {
@@ -54,11 +63,16 @@ template <> struct PairOf<float> {
 template <> struct PairOf<_Float16> {
     using raw = unsigned int;
     __device__ static __forceinline__ float a(raw r) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(r & 0xffffu)); }
-    __device__ static __forceinline__ float e(raw r) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(r >> 16)); }
+    __device__ static __forceinline__ float e(raw r)
+    {
+        const float m = (float)__builtin_bit_cast(_Float16, (unsigned short)(r >> 16));
+        return __builtin_copysignf(1.0f - 2.0f * __builtin_fabsf(m), m);
+    }
     __device__ static __forceinline__ raw make(float a, float e)
     {
         const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)a);
-        const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)e);
+        const float m = __builtin_fmaxf(0.5f * (1.0f - __builtin_fabsf(e)), 0.0f);
+        const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)__builtin_copysignf(m, e));
         return lo | (hi << 16);
     }
 };
@@ -104,14 +118,24 @@ __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restr
     }
 }
 
+struct Ref {
+    float mid, half;   // reference density deviation r(e) = mid + half e
+};
+
 template <typename ST>
-__device__ __forceinline__ void decode(const typename PairOf<ST>::raw r[Q], Row& row)
+__device__ __forceinline__ void decode(const typename PairOf<ST>::raw r[Q], Row& row, Ref rf)
 {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        row.s[i] = PairOf<ST>::a(r[i]) * WF[i];
         row.e[i] = PairOf<ST>::e(r[i]);
+        row.s[i] = (PairOf<ST>::a(r[i]) + (rf.mid + rf.half * row.e[i])) * WF[i];
     }
+}
+
+template <typename ST>
+__device__ __forceinline__ typename PairOf<ST>::raw encode(int i, float s, float e, Ref rf)
+{
+    return PairOf<ST>::make(s * (1.f / WF[i]) - (rf.mid + rf.half * e), e);
 }
 
 // density deviation, rho and phi of a site.  phi sums e_i f_i and f_i in the same order, so a pure
@@ -133,10 +157,10 @@ __device__ __forceinline__ void row_phase(const Row& r, float& ds, float& rho, f
 }
 
 template <typename ST>
-__device__ __forceinline__ float raw_phase(const typename PairOf<ST>::raw r[Q])
+__device__ __forceinline__ float raw_phase(const typename PairOf<ST>::raw r[Q], Ref rf)
 {
     Row row;
-    decode<ST>(r, row);
+    decode<ST>(r, row, rf);
     float ds, rho, phi;
     row_phase(row, ds, rho, phi);
     return phi;
@@ -332,6 +356,7 @@ __global__ void __launch_bounds__(64) lp_march(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
+    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
     const Lane L = make_lane(nx, t.strip, H);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
@@ -346,14 +371,14 @@ __global__ void __launch_bounds__(64) lp_march(StepLaunch launch)
 #pragma unroll
     for (int j = 0; j < H; ++j) {
         load_raw<ST>(in, nx, ny, y0 - H + j, L, slot[0]);
-        ph[NS - H + j] = raw_phase<ST>(slot[0]);
+        ph[NS - H + j] = raw_phase<ST>(slot[0], rf);
     }
     // fill the ring
 #pragma unroll
     for (int s = 0; s < NS; ++s)
         if (y0 + s <= rlast) load_raw<ST>(in, nx, ny, y0 + s, L, slot[s]);
 #pragma unroll
-    for (int s = 0; s < H; ++s) ph[s] = raw_phase<ST>(slot[s]);
+    for (int s = 0; s < H; ++s) ph[s] = raw_phase<ST>(slot[s], rf);
     if constexpr (H == 2) {
         ax[NS - 1] = aux_of<KIND>(k, gather9(ph[NS - 2], ph[NS - 1], ph[0], L.lane));   // row y0 - 1
         ax[0] = aux_of<KIND>(k, gather9(ph[NS - 1], ph[0], ph[1], L.lane));              // row y0
@@ -365,7 +390,7 @@ __global__ void __launch_bounds__(64) lp_march(StepLaunch launch)
             const int y = yb + kq;
             if (y < y1) {
                 const int sa = (kq + H) % NS;            // row y + H arrives
-                ph[sa] = raw_phase<ST>(slot[sa]);
+                ph[sa] = raw_phase<ST>(slot[sa], rf);
                 const int im = (kq + NS - 1) % NS, ip = (kq + 1) % NS;
                 const Nine p9 = gather9(ph[im], ph[kq], ph[ip], L.lane);
                 const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
@@ -397,13 +422,13 @@ __global__ void __launch_bounds__(64) lp_march(StepLaunch launch)
 #pragma unroll
                     for (int i = 0; i < Q; ++i) asm volatile("" : "+v"(slot[kq][i]));
                     Row cur;
-                    decode<ST>(slot[kq], cur);
+                    decode<ST>(slot[kq], cur, rf);
                     float ds, rho, phi;
                     row_phase(cur, ds, rho, phi);
                     const unsigned base = (unsigned)(L.col);
                     gst<R>* orow = out + (long long)y * ((long long)Q * nx);
                     auto st = [&](int i, float s, float e) {
-                        put_byte(orow, ((unsigned)(i * nx) + base) * (unsigned)sizeof(R), PairOf<ST>::make(s * (1.f / WF[i]), e));
+                        put_byte(orow, ((unsigned)(i * nx) + base) * (unsigned)sizeof(R), encode<ST>(i, s, e, rf));
                     };
                     if (rho > 1.0e-30f) {
                         collide_lp<KIND != KIND_P>(k, cur, ds, rho, phi, gx, gy, Fx, Fy, st);
@@ -433,6 +458,7 @@ __global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
+    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
     const Lane L = make_lane(nx, t.strip, H);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
@@ -444,7 +470,7 @@ __global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
 #pragma unroll
     for (int j = 0; j < 2 * H; ++j) {
         load_raw<ST>(in, nx, ny, y0 - H + j, L, nxt);
-        p[j + 1] = raw_phase<ST>(nxt);   // rows y0-H .. y0+H-1 at p[1 .. 2H]
+        p[j + 1] = raw_phase<ST>(nxt, rf);   // rows y0-H .. y0+H-1 at p[1 .. 2H]
     }
     if constexpr (H == 2) {
         am = aux_of<KIND>(k, gather9(p[1], p[2], p[3], L.lane));   // row y0 - 1
@@ -459,7 +485,7 @@ __global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
         }
 #pragma unroll
         for (int j = 0; j < 2 * H; ++j) p[j] = p[j + 1];
-        p[2 * H] = raw_phase<ST>(nxt);   // row y + H
+        p[2 * H] = raw_phase<ST>(nxt, rf);   // row y + H
         const Nine p9 = gather9(p[H - 1], p[H], p[H + 1], L.lane);
         const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
         float Fx = 0.f, Fy = 0.f;
@@ -485,13 +511,13 @@ __global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
         if (L.interior) {
             if constexpr (H == 2) load_raw<ST>(in, nx, ny, y, L, cur);
             Row row;
-            decode<ST>(cur, row);
+            decode<ST>(cur, row, rf);
             float ds, rho, phi;
             row_phase(row, ds, rho, phi);
             gst<R>* orow = out + (long long)y * ((long long)Q * nx);
             const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
             auto st = [&](int i, float s, float e) {
-                put_byte(orow + i * nx, ocol, PairOf<ST>::make(s * (1.f / WF[i]), e));
+                put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf));
             };
             if (rho > 1.0e-30f) {
                 collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
@@ -540,7 +566,7 @@ hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t s
 // (A, e) pairs in rows -> fR, fB in FP64, [19][S] each (own site, post-collision)
 template <typename ST>
 __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw* __restrict__ pop, int nx, long long S,
-                                                 long long n, double* __restrict__ fR, double* __restrict__ fB)
+                                                 long long n, Ref rf, double* __restrict__ fR, double* __restrict__ fB)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
         const long long y = p / nx, x = p - y * nx;
@@ -548,11 +574,26 @@ __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw*
         for (int i = 0; i < Q; ++i) {
             const auto r = pop[(y * Q + i) * nx + x];
             const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
-            const double f = w * (1.0 + (double)PairOf<ST>::a(r));
-            const double e = (double)PairOf<ST>::e(r);
-            fR[i * S + p] = 0.5 * f * (1.0 + e);
-            fB[i * S + p] = 0.5 * f * (1.0 - e);
+            const float e = PairOf<ST>::e(r);
+            const float A = PairOf<ST>::a(r) + (rf.mid + rf.half * e);
+            const double f = w * (1.0 + (double)A);
+            fR[i * S + p] = 0.5 * f * (1.0 + (double)e);
+            fB[i * S + p] = 0.5 * f * (1.0 - (double)e);
         }
+    }
+}
+
+// Re-express a stored state against a new reference density, in place: h' = h + r_old(e) - r_new(e).
+template <typename ST>
+__global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __restrict__ pop, long long npairs,
+                                                   Ref from, Ref to)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs;
+         p += (long long)gridDim.x * blockDim.x) {
+        const auto r = pop[p];
+        const float e = PairOf<ST>::e(r);
+        const float A = PairOf<ST>::a(r) + (from.mid + from.half * e);
+        pop[p] = PairOf<ST>::make(A - (to.mid + to.half * e), e);
     }
 }
 
@@ -566,15 +607,28 @@ hipError_t launch_step_lp(int elem, int kind, const StepLaunch& a, int ntiles, h
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long long n, double* fR, double* fB,
-                            hipStream_t stream)
+hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long long n, double rmid, double rhalf,
+                            double* fR, double* fB, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
+    const Ref rf{(float)rmid, (float)rhalf};
     if (elem == 4)
-        hipLaunchKernelGGL((lp_to_f64<float>), dim3(nb), dim3(256), 0, stream, (const f32x2*)pop, nx, S, n, fR, fB);
+        hipLaunchKernelGGL((lp_to_f64<float>), dim3(nb), dim3(256), 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
     else
-        hipLaunchKernelGGL((lp_to_f64<_Float16>), dim3(nb), dim3(256), 0, stream, (const unsigned*)pop, nx, S, n, fR,
-                           fB);
+        hipLaunchKernelGGL((lp_to_f64<_Float16>), dim3(nb), dim3(256), 0, stream, (const unsigned*)pop, nx, S, n, rf,
+                           fR, fB);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, double mid0, double half0, double mid1,
+                              double half1, hipStream_t stream)
+{
+    const int nb = (int)std::min<long long>((npairs + 255) / 256, 8192);
+    const Ref from{(float)mid0, (float)half0}, to{(float)mid1, (float)half1};
+    if (elem == 4)
+        hipLaunchKernelGGL((lp_recentre<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, npairs, from, to);
+    else
+        hipLaunchKernelGGL((lp_recentre<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, npairs, from, to);
     return hipGetLastError();
 }
 

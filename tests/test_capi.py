@@ -75,3 +75,8 @@ def test_create_rejects_bad_arguments_without_touching_a_device():
     assert L.lbm_create((m.CaseDesc * 1)(bad), 0, 0, C.byref(h)) == -1
     assert L.lbm_create((m.CaseDesc * 1)(bad), 1, 5, C.byref(h)) == -1
     assert L.lbm_last_error(None) == b"null context"
+    # every entry point reports a null context as an argument error
+    assert L.lbm_step(None, 1, 0) == -1
+    assert L.lbm_step_case(None, 0, 1, 0) == -1
+    assert L.lbm_set_reference(None, 0, 1.0, 1.0) == -1
+    assert L.lbm_sync(None) == -1
