@@ -4,13 +4,25 @@
 Workload (the 512^2 static-drop configuration): a 512 x 512 x 1 periodic box holding a sharp
 cylindrical drop of radius 100 at the centre, tau in {0.6, 1.0}, sigma = 0.1 (A = 9 sigma / 4 tau for
 the perturbation operators), beta = 0.8, for all seven operators (P1, P3 chi = -0.4 / 0.5 / 2, F1,
-F2, F3 xi = 1.569988): 14 cases resident on the GPU.  One bench step advances every case by one
-time step.  Inputs are synthetic (generated on the host), resident in HBM before timing.
+F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (one launch per operator
+family per step).  A bench step advances every case by one time step.  Inputs are synthetic
+(generated by the library's initial condition), resident in HBM before timing.
 
-Printed: one JSON line with MLUPS (= sum over cases of sites x steps / wall / 1e6, the reference's
-own definition, Imm_OPS.cpp:377), the roofline of the dominant step kernel (algorithmic bytes per
-launch over its HIP-event-timed average duration) and a CPU baseline: the reference's own program
-(built from its sources by oracle/Makefile) timed on this host's cores over a bounded sample.
+The headline form (--prec, default f32) is timed over --steps steps after --warmup; the other two
+storage forms are timed the same way and reported beside it.  Also reported:
+
+  roofline       the dominant kernel (the P1/P3 family launch, 8 of the 14 cases) timed alone with HIP
+                 events on its stream: algorithmic bytes per launch / average launch duration;
+                 traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+                 (profiles/traffic_<form>.json), when present
+  large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in the headline form
+  sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
+                 numpy default_rng(2026), box 4R): batched reduced-precision form vs the plain FP64
+                 one-launch-per-case path, time per step and delta_p agreement after --sweep-steps
+  cpu_baseline   the reference program (built from the reference's sources by oracle/Makefile) on
+                 this host's cores, over a bounded sample of the same workload
+
+MLUPS = sum over cases of sites x steps / wall / 1e6 (the reference's definition, Imm_OPS.cpp:377).
 """
 import argparse
 import json
@@ -30,17 +42,138 @@ SIGMA = 0.1
 NX = NY = 512
 R = 100.0
 PEAK_HBM_GBS = 8000.0
-KERNEL_NAMES = {"P": "step_ring1<false> (P1/P3)", "F3": "step_ring1<true> (F3)", "F1": "step_ring2<F1>",
-                "F2": "step_ring2<F2>"}
+B_LU = {"f64": 608, "f32": 304, "f16": 152}
+PREC_ID = {"f64": 0, "f32": 1, "f16": 2}
+KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_step<float,P> (P1/P3 family)",
+          "f16": "lp_step<_Float16,P> (P1/P3 family)"}
 
 
-def amplitude(kind, tau):
-    return 9.0 * SIGMA / (4.0 * tau) if kind == "P" else SIGMA
+def amplitude(kind, tau, sigma=SIGMA):
+    return 9.0 * sigma / (4.0 * tau) if kind == "P" else sigma
 
 
-def bytes_per_lu(prec):
-    # 2 colours x 19 populations x (1 read + 1 write) x element size
-    return {"f64": 608}[prec]
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Timer:
+    """HIP events on the library's stream (a torch stream handed to the context)."""
+
+    def __init__(self, torch, stream):
+        self.torch, self.stream = torch, stream
+
+    def run(self, fn):
+        t = self.torch
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        t.cuda.synchronize()
+        w0 = time.perf_counter()
+        e0.record(self.stream)
+        fn()
+        e1.record(self.stream)
+        self.stream.synchronize()
+        t.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3, time.perf_counter() - w0
+
+
+def bench_form(m, timer, stream, form, steps, warmup):
+    cases, kinds = [], []
+    for tau in TAUS:
+        for op, kind in OPS:
+            cases.append(m.case_desc(NX, NY, op, tau, amplitude(kind, tau), 0.8))
+            kinds.append(kind)
+    ctx = m.Context(cases, precision=PREC_ID[form])
+    ctx.set_stream(stream.cuda_stream)
+    for i in range(len(cases)):
+        ctx.init_drop(R, width=-1.0, case=i)
+    ctx.step(warmup)
+    gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
+    n_sites = NX * NY * len(cases)
+    # the dominant kernel alone: the P1/P3 family launch (8 cases), same launch shape
+    for i, k in enumerate(kinds):
+        if k != "P":
+            ctx.set_active(False, case=i)
+    nP = sum(1 for k in kinds if k == "P")
+    ctx.step(5)
+    kern_s, _ = timer.run(lambda: ctx.step(steps))
+    ctx.close()
+    avg = kern_s / steps
+    bytes_launch = B_LU[form] * NX * NY * nP
+    return {"MLUPS": n_sites * steps / wall_s / 1e6, "ms_per_step": wall_s / steps * 1e3,
+            "gpu_ms_per_step": gpu_s / steps * 1e3,
+            "dominant_kernel": KERNEL[form], "kernel_cases_per_launch": nP,
+            "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
+            "achieved_GBs": bytes_launch / avg / 1e9, "frac": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
+            "bytes_per_lattice_update": B_LU[form]}
+
+
+def bench_large(m, timer, stream, form, steps):
+    n = 8192
+    ctx = m.Context(m.case_desc(n, n, "P3 0.5", 1.0, amplitude("P", 1.0), 0.8), precision=PREC_ID[form])
+    ctx.set_stream(stream.cuda_stream)
+    ctx.init_drop(1000.0, width=-1.0, v0=0.1)
+    ctx.step(3)
+    gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
+    ctx.close()
+    avg = gpu_s / steps
+    return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
+            "steps": steps, "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
+            "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS}
+
+
+def sweep_cases(m):
+    import numpy as np
+    rng = np.random.default_rng(2026)
+    radii = rng.integers(10, 51, 168)
+    cases, k = [], 0
+    for tau in (0.51, 0.6, 0.8, 1.0, 1.5, 2.0):
+        for sigma in (0.01, 0.05, 0.1, 0.5):
+            for op, kind in OPS:
+                r = int(radii[k])
+                cases.append((m.case_desc(4 * r, 4 * r, op, tau, amplitude(kind, tau, sigma), 0.8), float(r)))
+                k += 1
+    return cases
+
+
+def bench_sweep(m, timer, stream, form, steps):
+    """168 cases: plain FP64 one launch per case and step vs the batched form, `steps` steps each."""
+    import numpy as np
+    cases = sweep_cases(m)
+    descs = [c for c, _ in cases]
+    sites = sum(d.nx * d.ny for d in descs)
+    res = {}
+    for label, prec in (("f64_per_case", 0), (f"{form}_batched", PREC_ID[form])):
+        ctx = m.Context(descs, precision=prec)
+        ctx.set_stream(stream.cuda_stream)
+        for i, (d, r) in enumerate(cases):
+            ctx.init_drop(r, width=-1.0, case=i)
+        if prec == 0:
+            # one launch per case and step (cases are independent, so case by case is the same work)
+            def run():
+                for i in range(len(descs)):
+                    ctx.step(steps - 1, case=i)
+                    ctx.step(1, record=True, case=i)
+        else:
+            def run():
+                ctx.step(steps - 1)
+                ctx.step(1, record=True)
+        gpu_s, wall_s = timer.run(run)
+        diags = [ctx.diag_drop(0.0, steps - 1, case=i) for i in range(len(descs))]
+        dp = np.array([g.delta_p for g in diags])
+        ok = np.array([g.ok for g in diags])
+        ctx.close()
+        res[label] = {"wall_s": wall_s, "ms_per_step": wall_s / steps * 1e3, "dp": dp, "ok": ok}
+    a, b = res["f64_per_case"], res[f"{form}_batched"]
+    good = (a["ok"] == 1) & (b["ok"] == 1) & np.isfinite(a["dp"]) & np.isfinite(b["dp"]) & (np.abs(a["dp"]) > 1e-9)
+    err = np.abs(b["dp"][good] - a["dp"][good]) / np.abs(a["dp"][good])
+    per_case_20k = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
+    return {"cases": len(descs), "sites": sites, "steps": steps, "form": form,
+            "f64_per_case_ms_per_step": a["ms_per_step"], "batched_ms_per_step": b["ms_per_step"],
+            "speedup": a["ms_per_step"] / b["ms_per_step"],
+            "cases_per_hour_20k_steps": {"f64_per_case": per_case_20k(a["ms_per_step"]),
+                                         f"{form}_batched": per_case_20k(b["ms_per_step"])},
+            "dp_compared_cases": int(good.sum()),
+            "dp_rel_err_median": float(np.median(err)) if err.size else None,
+            "dp_rel_err_max": float(err.max()) if err.size else None}
 
 
 def cpu_baseline(max_threads):
@@ -61,8 +194,7 @@ def cpu_baseline(max_threads):
             return {"value": NX * NY * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "reference",
                     "sample": f"reference program (OpenMP, {threads} threads): 512x512 sharp drop R=100, P3 chi=0.5, "
                               f"tau=1, {steps} steps incl. 2 diagnostics, wall {dt:.1f} s"}
-    # fallback: the oracle (scalar C restatement), one core
-    import numpy as np
+    # the reference build is absent: the oracle (scalar C restatement), one core
     import _oracle as O
     cs = O.make_case(NX, NY, "P3 0.5", 1.0, amplitude("P", 1.0))
     s = O.Sim(cs)
@@ -79,9 +211,12 @@ def cpu_baseline(max_threads):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--prec", default="f64", choices=["f64"])
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--prec", default="f32", choices=["f64", "f32", "f16"], help="headline storage form")
+    ap.add_argument("--forms", default="f64,f32,f16", help="forms measured beside the headline")
+    ap.add_argument("--sweep-steps", type=int, default=1000, help="0 skips the 168-case sweep")
+    ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 case")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     a = ap.parse_args()
@@ -93,84 +228,55 @@ def main():
     import torch
     import _pkg
     m = _pkg.load()
-
-    cases, kinds = [], []
-    for tau in TAUS:
-        for op, kind in OPS:
-            cases.append(m.case_desc(NX, NY, op, tau, amplitude(kind, tau), 0.8))
-            kinds.append(kind)
-    ctx = m.Context(cases)
-    for i in range(len(cases)):
-        ctx.init_drop(R, width=-1.0, case=i)
     stream = torch.cuda.Stream()
-    ctx.set_stream(stream.cuda_stream)
+    timer = Timer(torch, stream)
 
-    n_sites = NX * NY
-    for i in range(len(cases)):
-        ctx.step(a.warmup, case=i)
-    ctx.sync()
-    torch.cuda.synchronize()
+    forms = {}
+    for form in [a.prec] + [f for f in a.forms.split(",") if f and f != a.prec]:
+        say(f"bench: form {form}, 14 cases x 512^2, {a.steps} steps")
+        forms[form] = bench_form(m, timer, stream, form, a.steps, a.warmup)
+    head = forms[a.prec]
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(len(cases)):
-        ev[i][0].record(stream)
-        ctx.step(a.steps, case=i)
-        ev[i][1].record(stream)
-    stream.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-
-    per_launch_ms = [ev[i][0].elapsed_time(ev[i][1]) / a.steps for i in range(len(cases))]
-    kstats = {}
-    for k, t in zip(kinds, per_launch_ms):
-        e = kstats.setdefault(k, {"launches_per_step": 0, "ms_per_launch": [], "total_ms": 0.0})
-        e["launches_per_step"] += 1
-        e["ms_per_launch"].append(t)
-        e["total_ms"] += t * a.steps
-    dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
-    bpl = bytes_per_lu(a.prec) * n_sites
-    kernels = {}
-    for k, e in kstats.items():
-        avg = sum(e["ms_per_launch"]) / len(e["ms_per_launch"])
-        kernels[KERNEL_NAMES[k]] = {"launches_per_step": e["launches_per_step"], "avg_launch_us": avg * 1e3,
-                                    "achieved_GBs": bpl / (avg * 1e-3) / 1e9,
-                                    "share_of_time": e["total_ms"] / sum(x["total_ms"] for x in kstats.values())}
-    dom_avg_ms = sum(kstats[dom]["ms_per_launch"]) / len(kstats[dom]["ms_per_launch"])
-    achieved = bpl / (dom_avg_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{a.prec}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get(dom)
+            traffic = json.load(f).get("traffic_bytes_per_launch")
 
-    mlups = len(cases) * n_sites * a.steps / wall / 1e6
     out = {
         "metric": "MLUPS + % HBM roofline, 1 GPU; sweep cases/hour vs plain FP64 at same Δp error",
-        "value": mlups,
+        "value": head["MLUPS"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": a.prec,
+        "dtype": "f64" if a.prec == "f64" else "f32",
         "data": "synthetic",
         "config": {"workload": "512x512x1 static sharp drop R=100; tau {0.6, 1.0} x 7 operators "
-                               "(P1, P3 chi -0.4/0.5/2, F1, F2, F3 xi 1.569988), sigma 0.1, beta 0.8; "
-                               f"{a.prec.upper()} one-case-per-launch path",
-                   "cases": len(cases), "sites_per_case": n_sites, "bytes_per_lattice_update": bytes_per_lu(a.prec)},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic},
-        "kernels": kernels,
+                               "(P1, P3 chi -0.4/0.5/2, F1, F2, F3 xi 1.569988), sigma 0.1, beta 0.8; 14 cases "
+                               f"batched; {a.prec} storage" + ("" if a.prec == "f64" else ", fp32 compute"),
+                   "cases": 14, "sites_per_case": NX * NY, "storage": a.prec,
+                   "bytes_per_lattice_update": B_LU[a.prec]},
+        "roofline": {"bound": "hbm", "kernel": head["dominant_kernel"], "achieved": head["achieved_GBs"],
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": head["frac"], "traffic": traffic,
+                     "algorithmic_bytes_per_launch": head["bytes_per_launch"],
+                     "avg_launch_us": head["kernel_avg_launch_us"]},
+        "forms": forms,
     }
+    if a.large_steps > 0:
+        say(f"bench: 8192^2 {a.prec}, {a.large_steps} steps")
+        out["large"] = bench_large(m, timer, stream, a.prec if a.prec != "f64" else "f32", a.large_steps)
+    if a.sweep_steps > 0:
+        sform = a.prec if a.prec != "f64" else "f32"
+        say(f"bench: 168-case sweep, {a.sweep_steps} steps, f64 per case vs {sform} batched")
+        out["sweep"] = bench_sweep(m, timer, stream, sform, a.sweep_steps)
     if not a.no_cpu_baseline:
-        cb = cpu_baseline(a.cpu_threads)
-        out["cpu_baseline"] = cb
-    ctx.close()
+        say("bench: cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(a.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

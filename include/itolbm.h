@@ -32,7 +32,7 @@ enum { LBM_OP_P1 = 1, LBM_OP_P3 = 3, LBM_OP_F1 = 11, LBM_OP_F2 = 12, LBM_OP_F3 =
 enum {
     LBM_PREC_F64 = 0, /* FP64 compute and storage: the drop-in form, bit-equal to the reference's arithmetic */
     LBM_PREC_F32 = 1, /* FP32 compute, FP32 storage of s_i / w_i and e_i (304 B per lattice update) */
-    LBM_PREC_F16 = 2  /* FP32 compute, FP16 storage of s_i / w_i and e_i (152 B per lattice update) */
+    LBM_PREC_F16 = 2  /* FP32 compute, FP16 storage of s_i / w_i and of the minority colour fraction (152 B per lattice update) */
 };
 /* In the FP32/FP16 forms each population is stored as A_i = s_i / w_i, s_i = fR_i + fB_i - w_i (deviation
  * from the rest state rho0 = 1, scaled per direction by 1 / w_i) and e_i = (fR_i - fB_i) / (fR_i + fB_i)
@@ -143,6 +143,11 @@ int lbm_sync(lbm_ctx* ctx);
  * NULL returns to the context's own stream.  Waits for work already queued first.  No
  * reference counterpart: it lets a caller order the steps with its own work and time them. */
 int lbm_set_stream(lbm_ctx* ctx, void* hip_stream);
+
+/* Includes (active != 0, the initial state) or leaves out (0) a case from lbm_step, e.g. once a batch
+ * member has finished or become unstable; lbm_step_case still advances it.  No reference
+ * counterpart (the reference runs one case per process). */
+int lbm_set_active(lbm_ctx* ctx, int case_id, int active);
 
 /* Reduced-precision forms (LBM_PREC_F32 / LBM_PREC_F16) store each population as its deviation from
  * w_i (1 + r(e)), r(e) = (rho_red + rho_blue) / 2 - 1 + e (rho_red - rho_blue) / 2, where e is the

@@ -37,6 +37,7 @@ struct CaseState {
     bool aux_valid = false;
     int kind = 0;                           // kernel family (KIND_*)
     int elem = 8;                           // bytes per stored value: 8 (FP64), 4 (FP32), 2 (FP16)
+    bool active = true;                     // advanced by lbm_step
     double* f64 = nullptr;                  // [2][19][S] FP64 copy of a low-precision state (diagnostics)
     // diagnostics scratch (allocated on first use)
     double* gal = nullptr;
@@ -45,6 +46,7 @@ struct CaseState {
     double* hist = nullptr;
     unsigned long long* histn = nullptr;
     double* tab = nullptr;                  // cosx[nx] sinx[nx] cosy[ny] siny[ny] ct[256] st[256] r[256]
+    double* ordered = nullptr;              // [S + 8]: per-site phi terms, then the site-ordered sums
     double* prof = nullptr;
     int n_casca = 0;
     int nblk = 1;
@@ -61,6 +63,7 @@ struct Group {
     std::vector<double*> aux0;     // members' aux pointer when the table was written
     int k = 0;                     // launches since the table was written
     int ntiles = 0;
+    int cap = 0;                   // entries dtab can hold
     bool written = false;
 };
 
@@ -190,6 +193,11 @@ int ensure_aux(lbm_ctx* c, CaseState& k)
     return LBM_OK;
 }
 
+// Up to this many sites the drop's area and centroid sums run in the reference's site order on one
+// wavefront (bitwise equal sums, hence the same centre, radius and region masks; ~0.1 ms per 10^5
+// sites); larger boxes use the block-tree sums (equal to ~1e-16 relative).
+constexpr long long ORDERED_MAX_SITES = 1LL << 24;
+
 int ensure_diag(lbm_ctx* c, CaseState& k)
 {
     if (k.gal) return LBM_OK;
@@ -200,6 +208,7 @@ int ensure_diag(lbm_ctx* c, CaseState& k)
     int r;
     if ((r = dev_alloc(c, (void**)&k.gal, sizeof(double) * 4 * (size_t)k.S))) return r;
     if ((r = dev_alloc(c, (void**)&k.dsum, sizeof(double) * 2 * (size_t)k.S))) return r;
+    if ((r = dev_alloc(c, (void**)&k.ordered, sizeof(double) * ((size_t)k.S + 8)))) return r;
     if ((r = dev_alloc(c, (void**)&k.partial, sizeof(double) * (size_t)k.nblk * np))) return r;
     if ((r = dev_alloc(c, (void**)&k.hist, sizeof(double) * (size_t)k.n_casca))) return r;
     if ((r = dev_alloc(c, (void**)&k.histn, sizeof(unsigned long long) * (size_t)k.n_casca))) return r;
@@ -388,7 +397,7 @@ int lbm_destroy(lbm_ctx* c)
     if (!c) return LBM_E_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (CaseState& k : c->cases) {
-        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.partial, k.hist, k.tab, k.prof, k.f64})
+        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.ordered, k.partial, k.hist, k.tab, k.prof, k.f64})
             if (p) (void)hipFree(p);
         if (k.histn) (void)hipFree(k.histn);
     }
@@ -456,9 +465,18 @@ int lbm_init_drop(lbm_ctx* c, int id, double R, double width, int x0, int y0, do
 {
     if (int r = check_case(c, id)) return r;
     if (!std::isfinite(R) || !std::isfinite(v0) || !std::isfinite(width)) return fail(c, LBM_E_ARG, "non-finite argument");
-    const CaseState& k = c->cases[id];
+    CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
     const double k_perfil = (width > 0.0) ? 2.0 * std::log(9.0) / width : -1.0;
+    if (k.elem != 8) {
+        // reduced-precision forms: computed on the device in the stored form (no host arrays, so
+        // boxes of 10^8 sites initialise in milliseconds); FP64 keeps the host path below, whose
+        // libm exp gives the reference's bits
+        HIPCHK(c, launch_lp_init_drop(k.elem, k.pop[k.cur], nx, ny, R, k_perfil, x0, y0, v0, k.cc.ref_mid,
+                                      k.cc.ref_half, c->stream));
+        k.aux_valid = false;
+        return LBM_OK;
+    }
     const double rho_ini_R = 1.0, rho_ini_B = 1.0, delta_rho = 0.0;
     std::vector<double> fR((size_t)k.n * Q), fB((size_t)k.n * Q);
     for (int x = 0; x < nx; x++)
@@ -578,8 +596,13 @@ int group_sync(lbm_ctx* c, Group& g)
         g.cur0[j] = k.cur;
         g.aux0[j] = k.aux;
     }
+    if (g.dtab && nm > g.cap) {
+        (void)hipFree(g.dtab);
+        g.dtab = nullptr;
+    }
     if (!g.dtab) {
         if (int r = dev_alloc(c, (void**)&g.dtab, sizeof(CaseDev) * nm)) return r;
+        g.cap = nm;
     }
     if (g.dtile && tile0 > g.ntiles) {
         (void)hipFree(g.dtile);
@@ -659,6 +682,21 @@ int lbm_set_stream(lbm_ctx* c, void* s)
     return LBM_OK;
 }
 
+int lbm_set_active(lbm_ctx* c, int id, int active)
+{
+    if (int r = check_case(c, id)) return r;
+    CaseState& k = c->cases[id];
+    if (k.active == (active != 0)) return LBM_OK;
+    k.active = active != 0;
+    Group& g = c->family[k.kind];
+    g.members.clear();
+    for (int j = 0; j < (int)c->cases.size(); ++j)
+        if (c->cases[j].active && c->cases[j].kind == k.kind) g.members.push_back(j);
+    g.ly.clear();   // row segments are re-chosen for the new member set
+    g.written = false;
+    return LBM_OK;
+}
+
 int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
 {
     if (int r = check_case(c, id)) return r;
@@ -711,10 +749,27 @@ int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double 
     DropPass1Args a1{};
     a1.fR = fR; a1.fB = fB; a1.S = k.S; a1.nx = nx; a1.ny = ny; a1.F = F;
     a1.cosx = k.tab; a1.sinx = k.tab + nx; a1.cosy = k.tab + 2 * nx; a1.siny = k.tab + 2 * nx + ny;
-    a1.gal = k.gal; a1.dsum = k.dsum; a1.partial = k.partial;
+    a1.gal = k.gal; a1.dsum = k.dsum; a1.aphi = k.ordered; a1.partial = k.partial;
     HIPCHK(c, launch_drop_pass1(a1, k.nblk, c->stream));
+    // the sums that place the drop (area, circular centroid) in the reference's site order
+    const bool ordered = k.n <= ORDERED_MAX_SITES;
+    if (ordered) {
+        OrderedArgs ao{};
+        ao.aphi = k.ordered; ao.n = k.n; ao.nx = nx;
+        ao.cosx = a1.cosx; ao.sinx = a1.sinx; ao.cosy = a1.cosy; ao.siny = a1.siny;
+        ao.out = k.ordered + k.S;
+        HIPCHK(c, launch_drop_ordered(ao, c->stream));
+    }
     std::vector<double> r1;
     if (int r = reduce_partials<P1_NP>(c, k, r1, P1_NS, P1_NX, P1_NN)) return r;
+    if (ordered) {
+        double h[5];
+        HIPCHK(c, hipMemcpyAsync(h, k.ordered + k.S, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        r1[P1_AREA] = h[0];
+        r1[P1_CXC] = h[1]; r1[P1_CXS] = h[2];
+        r1[P1_CYC] = h[3]; r1[P1_CYS] = h[4];
+    }
     const double n_fluid = r1[P1_NFLUID];
     const double tx = 2.0 * PI / (double)nx, ty = 2.0 * PI / (double)ny;
     double x0 = std::atan2(r1[P1_CXS], r1[P1_CXC]) / tx;

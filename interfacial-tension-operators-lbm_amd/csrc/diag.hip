@@ -115,6 +115,7 @@ __global__ void __launch_bounds__(256) drop_pass1(DropPass1Args a)
         calcula(fR, vxR, vyR, vzR, rho_R);
         calcula(fB, vxB, vyB, vzB, rho_B);
         s[P1_NFLUID] += 1.0;
+        a.aphi[p] = 0.0;
         if (!(rho_R == rho_R) || !(rho_B == rho_B) || !(vxR == vxR) || !(vxB == vxB)) {
             s[P1_NRUIM] += 1.0;
         } else {
@@ -140,6 +141,7 @@ __global__ void __launch_bounds__(256) drop_pass1(DropPass1Args a)
                 const double u = sqrt(u2);
                 if (u > mx[0]) mx[0] = u;
                 const double phi = rho_R / rho;
+                a.aphi[p] = phi;
                 s[P1_AREA] = s[P1_AREA] + phi;
                 s[P1_CXC] = s[P1_CXC] + phi * a.cosx[x];
                 s[P1_CXS] = s[P1_CXS] + phi * a.sinx[x];
@@ -195,6 +197,38 @@ __global__ void __launch_bounds__(256) drop_pass1(DropPass1Args a)
         a.gal[p] = ux; a.gal[a.S + p] = uy; a.gal[2 * a.S + p] = uz; a.gal[3 * a.S + p] = ph;
     }
     block_reduce<P1_NS, P1_NX, P1_NN>(s, mx, mn, a.partial + (long long)blockIdx.x * P1_NP);
+}
+
+// One wavefront; lane k < 5 accumulates sum k over the sites in order p = 0 .. n-1 (x fastest), the
+// order of the reference's loop, so each sum is rounded exactly as there.  A site that adds nothing
+// adds +0.0 or -0.0, which leaves a running sum that starts at +0.0 unchanged.
+__global__ void __launch_bounds__(64) drop_ordered(OrderedArgs a)
+{
+    const int k = threadIdx.x;
+    if (k >= 5) return;
+    const double* tab = (k == 1) ? a.cosx : (k == 2) ? a.sinx : (k == 3) ? a.cosy : a.siny;
+    double acc = 0.0;
+    int x = 0, y = 0;
+    long long p = 0;
+    constexpr int U = 8;   // loads of U sites are issued before their additions
+    for (; p + U <= a.n; p += U) {
+        double t[U], w[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            t[j] = a.aphi[p + j];
+            w[j] = (k == 0) ? 1.0 : (k <= 2 ? tab[x] : tab[y]);
+            if (++x == a.nx) { x = 0; ++y; }
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) acc = (k == 0) ? acc + t[j] : acc + t[j] * w[j];
+    }
+    for (; p < a.n; ++p) {
+        const double t = a.aphi[p];
+        const double w = (k == 0) ? 1.0 : (k <= 2 ? tab[x] : tab[y]);
+        acc = (k == 0) ? acc + t : acc + t * w;
+        if (++x == a.nx) { x = 0; ++y; }
+    }
+    a.out[k] = acc;
 }
 
 // ------------------------------------------------------------------------------ drop, pass 2
@@ -392,6 +426,12 @@ hipError_t launch_drop_pass1(const DropPass1Args& a, int nblk, hipStream_t st)
     hipLaunchKernelGGL(drop_pass1, dim3(nblk), dim3(256), 0, st, a);
     return hipGetLastError();
 }
+hipError_t launch_drop_ordered(const OrderedArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(drop_ordered, dim3(1), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_drop_pass2(const DropPass2Args& a, int nblk, hipStream_t st)
 {
     hipLaunchKernelGGL(drop_pass2, dim3(nblk), dim3(256), 0, st, a);

@@ -47,7 +47,22 @@ struct DropPass1Args {
     const double* siny;
     double* gal;           // out [4][S]: u_x, u_y, u_z, phi = rho_R / rho (calc_galilean's arrays)
     double* dsum;          // out [2][S]: rho_R - rho_B, rho_R + rho_B
+    double* aphi;          // out [S]: calc_drop_radius's phi term of each site (0 where it adds none)
     double* partial;       // out [nblk][P1_NP]
+};
+
+// Area and circular-centroid sums of calc_drop_radius (Other_functions.cpp:89-183) accumulated in
+// site order, as the reference's loop does: they fix the drop centre and radius, and with them every
+// region and shell the later passes select, so they must carry the reference's exact bits.
+struct OrderedArgs {
+    const double* aphi;    // [n] phi term per site
+    long long n;
+    int nx;
+    const double* cosx;
+    const double* sinx;
+    const double* cosy;
+    const double* siny;
+    double* out;           // [5]: area, sum phi cos(tx x), sum phi sin(tx x), ... cos(ty y), sin(ty y)
 };
 
 struct DropPass2Args {
@@ -98,6 +113,7 @@ struct InclArgs {
 
 hipError_t launch_drop_pass1(const DropPass1Args& a, int nblk, hipStream_t st);
 hipError_t launch_drop_pass2(const DropPass2Args& a, int nblk, hipStream_t st);
+hipError_t launch_drop_ordered(const OrderedArgs& a, hipStream_t st);
 hipError_t launch_drop_modes(const ModesArgs& a, hipStream_t st);
 hipError_t launch_incl_pass(const InclArgs& a, int nblk, hipStream_t st);
 

@@ -40,6 +40,9 @@ hipError_t launch_step_lp(int elem, int kind, const StepLaunch& a, int ntiles, h
 // low-precision state (pairs (A_i, e_i) at ((y * 19 + i) nx + x)) -> FP64 colour populations fR, fB [19][S]
 hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long long n, double rmid, double rhalf,
                             double* fR, double* fB, hipStream_t stream);
+// drop initial condition written on the device in a reduced-precision form
+hipError_t launch_lp_init_drop(int elem, void* pop, int nx, int ny, double R, double kp, int x0, int y0, double v0,
+                               double rmid, double rhalf, hipStream_t stream);
 // in-place change of the reference density deviation r(e) = mid + half e of a reduced-precision state
 hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, double mid0, double half0, double mid1,
                               double half1, hipStream_t stream);

@@ -597,6 +597,40 @@ __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __r
     }
 }
 
+// Drop initial condition written directly in the stored form (initial_conditions_drop,
+// Initial_conditions.cpp:37-169, EQ_CLASSICO, cylinder, rho_R = rho_B = 1): f_R = f^eq(phi rho,
+// (v0, 0, 0)), f_B = f^eq((1 - phi) rho, (v0, 0, 0)).  The stored value at site x for velocity i is
+// the population that streams into x, i.e. the initial value at x + c_i (the kernel pulls).
+template <typename ST>
+__global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __restrict__ pop, int nx, int ny,
+                                                    double R, double kp, int x0, int y0, double v0, Ref rf)
+{
+    const long long n = (long long)nx * ny;
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const int y = (int)(p / nx), x = (int)(p - (long long)y * nx);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int xs = wrapi(x + CX[i], nx), ys = wrapi(y + CY[i], ny);
+            const double dx = (double)(xs - x0), dy = (double)(ys - y0);
+            const double r = sqrt(dx * dx + dy * dy);
+            double phi;
+            if (kp > 0.0) {
+                const double arg = fmin(fmax(kp * (r - R), -60.0), 60.0);
+                phi = 1.0 / (1.0 + exp(arg));
+            } else {
+                phi = (r < R) ? 1.0 : 0.0;
+            }
+            const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+            const double cv = (double)CX[i] * v0;
+            const double g = w * (1.0 + 3.0 * cv + 4.5 * cv * cv - 1.5 * v0 * v0);
+            const double fR = g * phi, fB = g * (1.0 - phi), f = fR + fB;
+            const float e = (float)((f != 0.0) ? (fR - fB) / f : 0.0);
+            const float h = (float)((f - w) / w) - (rf.mid + rf.half * e);
+            pop[((long long)y * Q + i) * nx + x] = PairOf<ST>::make(h, e);
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_step_lp(int elem, int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
@@ -629,6 +663,21 @@ hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, double mid0
         hipLaunchKernelGGL((lp_recentre<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, npairs, from, to);
     else
         hipLaunchKernelGGL((lp_recentre<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, npairs, from, to);
+    return hipGetLastError();
+}
+
+hipError_t launch_lp_init_drop(int elem, void* pop, int nx, int ny, double R, double kp, int x0, int y0, double v0,
+                               double rmid, double rhalf, hipStream_t stream)
+{
+    const long long n = (long long)nx * ny;
+    const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
+    const Ref rf{(float)rmid, (float)rhalf};
+    if (elem == 4)
+        hipLaunchKernelGGL((lp_init_drop<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0,
+                           v0, rf);
+    else
+        hipLaunchKernelGGL((lp_init_drop<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, nx, ny, R, kp,
+                           x0, y0, v0, rf);
     return hipGetLastError();
 }
 
