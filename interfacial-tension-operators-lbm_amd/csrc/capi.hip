@@ -144,9 +144,11 @@ long long stride_pad()
     return 0;
 }
 
+// Strips per row: one-row stencils (P1, P3, F3) use aligned 64-column strips with separate halo
+// loads; F1 / F2 use 60 owned columns plus 2 halo lanes on each side.
 int strips_of(const lbm_case_desc& d)
 {
-    const int wi = 64 - 2 * step_halo(d.op);
+    const int wi = (step_halo(d.op) == 1) ? 64 : 64 - 2 * step_halo(d.op);
     return (d.nx + wi - 1) / wi;
 }
 

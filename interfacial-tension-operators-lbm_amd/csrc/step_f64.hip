@@ -98,6 +98,30 @@ __device__ __forceinline__ Nine gather9(double rm, double r0, double rp, int lan
     return n;
 }
 
+// As gather9, with the columns left of lane 0 and right of lane 63 taken from the strip's halo
+// values (hl, hr per row), for aligned 64-column strips.
+__device__ __forceinline__ Nine gather9h(const double r[3], const double hl[3], const double hr[3], int lane)
+{
+    Nine n;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double lf = shfl(r[k], lane - 1), rt = shfl(r[k], lane + 1);
+        n.v[k][0] = (lane == 0) ? hl[k] : lf;
+        n.v[k][1] = r[k];
+        n.v[k][2] = (lane == 63) ? hr[k] : rt;
+    }
+    return n;
+}
+
+// density of the 19 values held by lanes base .. base+18, summed in index order as dens()
+__device__ __forceinline__ double lane_dens(double v, int base)
+{
+    double r = lane_value(v, base);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) r = r + lane_value(v, base + i);
+    return r;
+}
+
 // Mediator momentum gm = sum_i c_i M_i, M_i(x) = w_i (3 phi(x - c_i)):
 // emite_mediadores + propag_site_med + momentum (LBM_functions.cpp:1286-1337, 1056-1087).
 __device__ __forceinline__ void mediator_gradient(const Nine& phi, double& gx, double& gy, double& gz)
@@ -340,6 +364,8 @@ __device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtx& t)
 }
 
 // ------------------------------------------------------------------ P1 / P3 and F3 (1-ring)
+// Aligned 64-column strips; the colour field of the two columns beside the strip comes from one
+// extra 38-lane load per colour and row (Halo1), summed in the same order as every other site's.
 template <bool F3>
 __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
 {
@@ -347,26 +373,50 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
-    const Lane L = make_lane(nx, t.strip, 1);
+    const Lane L = make_lane_aligned(nx, t.strip);
+    const Halo1 h = make_halo1(nx, t.strip);
+    gcdouble* const hR = t.inR + h.i * t.S;
+    gcdouble* const hB = t.inB + h.i * t.S;
+    // every lane loads (lanes >= 38 re-read an in-range value that no one uses): no branch
+    auto halo_load = [&](int y, double& vr, double& vb) {
+        int yy = y + h.dy;
+        yy = (yy < 0) ? yy + ny : yy;
+        yy = (yy >= ny) ? yy - ny : yy;
+        const unsigned src = (unsigned)yy * (unsigned)nx + h.x;
+        vr = hR[src];
+        vb = hB[src];
+    };
+    auto halo_phase = [&](double vr, double vb, double& l, double& r) {
+        l = phase(lane_dens(vr, 0), lane_dens(vb, 0));
+        r = phase(lane_dens(vr, Q), lane_dens(vb, Q));
+    };
 
     double cR_[Q], cB_[Q], nR[Q], nB[Q];
+    double vr, vb;
+    double ph[3], hl[3], hr[3];   // colour field of rows y-1, y, y+1 and of the two halo columns
     load_row(t.inR, t.inB, t.S, nx, ny, t.y0 - 1, L, nR, nB);
-    double phi_m = phase(dens(nR), dens(nB));
+    halo_load(t.y0 - 1, vr, vb);
+    ph[0] = phase(dens(nR), dens(nB));
+    halo_phase(vr, vb, hl[0], hr[0]);
     load_row(t.inR, t.inB, t.S, nx, ny, t.y0, L, cR_, cB_);
+    halo_load(t.y0, vr, vb);
     double rRc = dens(cR_), rBc = dens(cB_);
-    double phi_c = phase(rRc, rBc);
+    ph[1] = phase(rRc, rBc);
+    halo_phase(vr, vb, hl[1], hr[1]);
 
     for (int y = t.y0; y < t.y1; ++y) {
         // next row's loads go out first; the gradient-independent part of this row's collision
         // (sums and, for P1/P3, the whole BGK relaxation) runs while they are in flight
         load_row(t.inR, t.inB, t.S, nx, ny, y + 1, L, nR, nB);
+        halo_load(y + 1, vr, vb);
         Site s;
         site_sums(s, cR_, cB_, rRc, rBc);
         if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
 
         const double rRn = dens(nR), rBn = dens(nB);
-        const double phi_n = phase(rRn, rBn);
-        const Nine p9 = gather9(phi_m, phi_c, phi_n, L.lane);
+        ph[2] = phase(rRn, rBn);
+        halo_phase(vr, vb, hl[2], hr[2]);
+        const Nine p9 = gather9h(ph, hl, hr, L.lane);
         double gx, gy, gz;
         mediator_gradient(p9, gx, gy, gz);
         const double mod2 = gx * gx + gy * gy + gz * gz;
@@ -377,15 +427,15 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
             // mu grad phi (Operadores_tensao.cpp:273-287, 325-330)
             double g0, g1, g2;
             phase_gradient(p9, g0, g1, g2);
-            const double ph = phi_c;
+            const double phc = ph[1];
             double lap = 0.0;
-#define ITOLBM_LAP(i) lap += (i < 7 ? W1 : W2) * (p9.at(CX[i], CY[i]) - ph);
+#define ITOLBM_LAP(i) lap += (i < 7 ? W1 : W2) * (p9.at(CX[i], CY[i]) - phc);
             ITOLBM_LAP(1) ITOLBM_LAP(2) ITOLBM_LAP(3) ITOLBM_LAP(4) ITOLBM_LAP(5) ITOLBM_LAP(6)
             ITOLBM_LAP(7) ITOLBM_LAP(8) ITOLBM_LAP(9) ITOLBM_LAP(10) ITOLBM_LAP(11) ITOLBM_LAP(12)
             ITOLBM_LAP(13) ITOLBM_LAP(14) ITOLBM_LAP(15) ITOLBM_LAP(16) ITOLBM_LAP(17) ITOLBM_LAP(18)
 #undef ITOLBM_LAP
             lap *= 6.0;
-            const double mu = 4.0 * cc.bf * ph * (ph * ph - 1.0) - cc.kf * lap;
+            const double mu = 4.0 * cc.bf * phc * (phc * phc - 1.0) - cc.kf * lap;
             Fx = mu * g0; Fy = mu * g1; Fz = mu * g2;
             bgk<true>(cc, s, Fx, Fy, Fz);
         } else {
@@ -413,7 +463,9 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
 #pragma unroll
         for (int i = 0; i < Q; ++i) { cR_[i] = nR[i]; cB_[i] = nB[i]; }
         rRc = rRn; rBc = rBn;
-        phi_m = phi_c; phi_c = phi_n;
+        ph[0] = ph[1]; ph[1] = ph[2];
+        hl[0] = hl[1]; hl[1] = hl[2];
+        hr[0] = hr[1]; hr[1] = hr[2];
     }
 }
 

@@ -168,6 +168,31 @@ __device__ __forceinline__ float raw_phase(const typename PairOf<ST>::raw r[Q], 
 
 __device__ __forceinline__ float shf(float v, int src) { return __shfl(v, src, 64); }
 
+// Colour field of the two halo sites whose 19 pulled pairs sit in lanes 0..18 and 19..37 of v: each
+// lane decodes its own pair as raw_phase does, then f_i and e_i f_i are summed over the site's lanes
+// in index order, so the result equals the owning lane's raw_phase bit for bit.
+template <typename ST>
+__device__ __forceinline__ void halo_phase(typename PairOf<ST>::raw v, Ref rf, float& left, float& right)
+{
+    const int lane = threadIdx.x;
+    const int i = lane >= Q ? lane - Q : lane;
+    const float w = (i == 0) ? WF[0] : (i < 7 ? WF[1] : WF[7]);
+    const float e = PairOf<ST>::e(v);
+    const float si = (PairOf<ST>::a(v) + (rf.mid + rf.half * e)) * w;
+    const float f = si + w;
+    const float ef = e * f;
+    float D0 = 0.f, F0 = 0.f, D1 = 0.f, F1 = 0.f;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        D0 += lane_value(ef, k);
+        F0 += lane_value(f, k);
+        D1 += lane_value(ef, Q + k);
+        F1 += lane_value(f, Q + k);
+    }
+    left = D0 / F0;
+    right = D1 / F1;
+}
+
 struct Nine {
     float v[3][3];
     __device__ __forceinline__ float at(int dx, int dy) const { return v[dy + 1][dx + 1]; }
@@ -182,6 +207,22 @@ __device__ __forceinline__ Nine gather9(float rm, float r0, float rp, int lane)
         n.v[k][0] = shf(rows[k], lane - 1);
         n.v[k][1] = rows[k];
         n.v[k][2] = shf(rows[k], lane + 1);
+    }
+    return n;
+}
+
+// As gather9, with the columns left of lane 0 and right of lane 63 taken from the strip's halo
+// values (hl, hr per row), for aligned 64-column strips.
+__device__ __forceinline__ Nine gather9h(float rm, float r0, float rp, const float hl[3], const float hr[3], int lane)
+{
+    Nine n;
+    const float rows[3] = {rm, r0, rp};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float lf = shf(rows[k], lane - 1), rt = shf(rows[k], lane + 1);
+        n.v[k][0] = (lane == 0) ? hl[k] : lf;
+        n.v[k][1] = rows[k];
+        n.v[k][2] = (lane == 63) ? hr[k] : rt;
     }
     return n;
 }
@@ -342,113 +383,9 @@ __device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
     return a;
 }
 
-// Strip-marching kernel with a ring of NS row buffers per wavefront.  When row y is collided, rows
-// y .. y+H have arrived (their colour field is known) and rows y+H+1 .. y+NS-1 are still in flight,
-// so every wavefront keeps NS-H-1 rows of loads outstanding while it computes.  The row loop is
-// unrolled NS times so that every buffer index is a compile-time constant (registers, no scratch).
-template <typename ST, int KIND, int NS>
-__global__ void __launch_bounds__(64) lp_march(StepLaunch launch)
-{
-    constexpr int H = (KIND == KIND_F1 || KIND == KIND_F2) ? 2 : 1;
-    static_assert(NS >= 2 * H + 1, "the colour-field ring must hold rows y-H .. y+H");
-    using R = typename PairOf<ST>::raw;
-    TileCtxLP t;
-    if (!tile_ctx(launch, t)) return;
-    const int nx = t.cc.nx, ny = t.cc.ny;
-    const ConstF k = to_float(t.cc);
-    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
-    const Lane L = make_lane(nx, t.strip, H);
-    gcst<R>* in = (gcst<R>*)t.in;
-    gst<R>* out = (gst<R>*)t.out;
-    const int y0 = t.y0, y1 = t.y1;
-    const int rlast = y1 - 1 + H;        // last row whose colour field is needed
-
-    R slot[NS][Q];                       // raw rows; row y >= y0 lives in slot (y - y0) % NS
-    float ph[NS];                        // colour field; row r >= y0 - H at index (r - y0) mod NS
-    AuxF ax[NS];                         // class F1/F2: aux at row r, same indexing
-
-    // halo rows above the segment: colour field only
-#pragma unroll
-    for (int j = 0; j < H; ++j) {
-        load_raw<ST>(in, nx, ny, y0 - H + j, L, slot[0]);
-        ph[NS - H + j] = raw_phase<ST>(slot[0], rf);
-    }
-    // fill the ring
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-        if (y0 + s <= rlast) load_raw<ST>(in, nx, ny, y0 + s, L, slot[s]);
-#pragma unroll
-    for (int s = 0; s < H; ++s) ph[s] = raw_phase<ST>(slot[s], rf);
-    if constexpr (H == 2) {
-        ax[NS - 1] = aux_of<KIND>(k, gather9(ph[NS - 2], ph[NS - 1], ph[0], L.lane));   // row y0 - 1
-        ax[0] = aux_of<KIND>(k, gather9(ph[NS - 1], ph[0], ph[1], L.lane));              // row y0
-    }
-
-    for (int yb = y0; yb < y1; yb += NS) {
-#pragma unroll
-        for (int kq = 0; kq < NS; ++kq) {
-            const int y = yb + kq;
-            if (y < y1) {
-                const int sa = (kq + H) % NS;            // row y + H arrives
-                ph[sa] = raw_phase<ST>(slot[sa], rf);
-                const int im = (kq + NS - 1) % NS, ip = (kq + 1) % NS;
-                const Nine p9 = gather9(ph[im], ph[kq], ph[ip], L.lane);
-                const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
-                float Fx = 0.f, Fy = 0.f;
-                if constexpr (KIND == KIND_F3) {
-                    const float phc = ph[kq];
-                    const float mu = 4.f * k.bf * phc * (phc * phc - 1.f) - k.kf * lap_iso(p9);
-                    Fx = -mu * gx;
-                    Fy = -mu * gy;
-                } else if constexpr (H == 2) {
-                    ax[ip] = aux_of<KIND>(k, gather9(ph[kq], ph[ip], ph[sa], L.lane));     // row y + 1
-                    const AuxF& am = ax[im];
-                    const AuxF& a0 = ax[kq];
-                    const AuxF& ap = ax[ip];
-                    if constexpr (KIND == KIND_F1) {
-                        const float divn = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) +
-                                           dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
-                        Fx = -k.hs * divn * a0.g0;
-                        Fy = -k.hs * divn * a0.g1;
-                    } else {
-                        const Nine txy = gather9(am.a2, a0.a2, ap.a2, L.lane);
-                        Fx = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) + dy_iso(txy);
-                        Fy = dx_iso(txy) + dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
-                    }
-                }
-                if (L.interior) {
-                    // the decode at arrival must not be reused here (that would keep 38 decoded values
-                    // of every row in flight live): an empty asm makes the raw registers opaque
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) asm volatile("" : "+v"(slot[kq][i]));
-                    Row cur;
-                    decode<ST>(slot[kq], cur, rf);
-                    float ds, rho, phi;
-                    row_phase(cur, ds, rho, phi);
-                    const unsigned base = (unsigned)(L.col);
-                    gst<R>* orow = out + (long long)y * ((long long)Q * nx);
-                    auto st = [&](int i, float s, float e) {
-                        put_byte(orow, ((unsigned)(i * nx) + base) * (unsigned)sizeof(R), encode<ST>(i, s, e, rf));
-                    };
-                    if (rho > 1.0e-30f) {
-                        collide_lp<KIND != KIND_P>(k, cur, ds, rho, phi, gx, gy, Fx, Fy, st);
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < Q; ++i) st(i, cur.s[i], cur.e[i]);
-                    }
-                    if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
-                }
-                if (y + NS <= rlast) load_raw<ST>(in, nx, ny, y + NS, L, slot[kq]);
-            }
-            // keep the unrolled rows apart: without this the scheduler interleaves them and the
-            // live registers multiply
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-}
-
-// One-row-ahead strip kernel: the loads of row y+1 (whose colour field row y needs) are in flight
-// while ... they are waited for, then row y is collided.  Two raw rows per lane (cur, next).
+// One-row-ahead strip kernel for the two-row stencils (F1, F2): strips of 60 owned columns with two
+// halo lanes on each side; rows y-2 .. y+2 of the colour field are kept, and row y's populations are
+// read again (from L2) when its force is known.
 template <typename ST, int KIND>
 __global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
 {
@@ -534,27 +471,205 @@ __global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
     }
 }
 
+__device__ __forceinline__ void opaque(f32x2& v)
+{
+    float a = v.x, b = v.y;
+    asm volatile("" : "+v"(a), "+v"(b));
+    v = f32x2{a, b};
+}
+__device__ __forceinline__ void opaque(unsigned& v) { asm volatile("" : "+v"(v)); }
+
+// Halo of an aligned strip for the pair layout: one 38-lane load per row (Halo1).  The address is
+// computed from a per-lane row index, apart from the strip's own loads, whose row bases stay uniform
+// (scalar registers) that way.
+template <typename ST>
+struct HaloLP {
+    using R = typename PairOf<ST>::raw;
+    unsigned long long off;   // element offset of (velocity i, column x) within a row block
+    int dy;                   // -c_iy
+    bool on;
+    __device__ __forceinline__ void init(int nx, int strip)
+    {
+        const Halo1 h = make_halo1(nx, strip);
+        off = (unsigned long long)h.i * (unsigned)nx + h.x;
+        dy = h.dy;
+        on = h.on;
+    }
+    __device__ __forceinline__ R load(gcst<R>* in, int nx, int ny, int y) const
+    {
+        int yy = y + dy;
+        yy = (yy < 0) ? yy + ny : yy;
+        yy = (yy >= ny) ? yy - ny : yy;
+        // every lane loads (lanes >= 38 re-read an in-range value that no one uses): no branch
+        return in[(unsigned long long)yy * ((unsigned long long)Q * (unsigned)nx) + off];
+    }
+};
+
+// Collision of row y of an aligned strip, given the colour field of rows y-1, y, y+1 (own lanes and
+// the two halo columns).
+template <typename ST, int KIND>
+__device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
+                                            gst<typename PairOf<ST>::raw>* out, int y,
+                                            const typename PairOf<ST>::raw (&cur)[Q], const float (&p)[3],
+                                            const float (&hl)[3], const float (&hr)[3])
+{
+    using R = typename PairOf<ST>::raw;
+    const int nx = t.cc.nx;
+    const Nine p9 = gather9h(p[0], p[1], p[2], hl, hr, L.lane);
+    const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
+    float Fx = 0.f, Fy = 0.f;
+    if constexpr (KIND == KIND_F3) {
+        const float phc = p[1];
+        const float mu = 4.f * k.bf * phc * (phc * phc - 1.f) - k.kf * lap_iso(p9);
+        Fx = -mu * gx;
+        Fy = -mu * gy;
+    }
+    if (L.interior) {
+        Row row;
+        decode<ST>(cur, row, rf);
+        float ds, rho, phi;
+        row_phase(row, ds, rho, phi);
+        gst<R>* orow = out + (long long)y * ((long long)Q * nx);
+        const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
+        auto st = [&](int i, float s, float e) { put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf)); };
+        if (rho > 1.0e-30f) {
+            collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
+        }
+        if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
+    }
+}
+
+// One-row-ahead kernel on aligned strips (P1, P3, F3): row y+1's loads (own and halo) are issued,
+// its colour field completes row y's gradient, row y is collided.
+template <typename ST, int KIND>
+__global__ void __launch_bounds__(64) lp_row1(StepLaunch launch)
+{
+    static_assert(KIND == KIND_P || KIND == KIND_F3, "one-row stencils only");
+    using R = typename PairOf<ST>::raw;
+    TileCtxLP t;
+    if (!tile_ctx(launch, t)) return;
+    const int nx = t.cc.nx, ny = t.cc.ny;
+    const ConstF k = to_float(t.cc);
+    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
+    const Lane L = make_lane_aligned(nx, t.strip);
+    gcst<R>* in = (gcst<R>*)t.in;
+    gst<R>* out = (gst<R>*)t.out;
+    HaloLP<ST> h;
+    h.init(nx, t.strip);
+    const int y0 = t.y0, y1 = t.y1;
+
+    R cur[Q], nxt[Q];
+    float p[3], hl[3], hr[3];
+    load_raw<ST>(in, nx, ny, y0 - 1, L, nxt);
+    R hv = h.load(in, nx, ny, y0 - 1);
+    p[1] = raw_phase<ST>(nxt, rf);
+    halo_phase<ST>(hv, rf, hl[1], hr[1]);
+    load_raw<ST>(in, nx, ny, y0, L, cur);
+    hv = h.load(in, nx, ny, y0);
+    p[2] = raw_phase<ST>(cur, rf);
+    halo_phase<ST>(hv, rf, hl[2], hr[2]);
+    for (int y = y0; y < y1; ++y) {
+        load_raw<ST>(in, nx, ny, y + 1, L, nxt);
+        hv = h.load(in, nx, ny, y + 1);
+        p[0] = p[1]; p[1] = p[2];
+        hl[0] = hl[1]; hl[1] = hl[2];
+        hr[0] = hr[1]; hr[1] = hr[2];
+        p[2] = raw_phase<ST>(nxt, rf);
+        halo_phase<ST>(hv, rf, hl[2], hr[2]);
+        collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
+    }
+}
+
+// Two-row-deep pipelined variant: three raw row buffers rotate, so while row y is collided the
+// loads of row y+2 are in flight.  The row loop is unrolled three times (buffer roles are
+// compile-time constants).
+template <typename ST, int KIND>
+__device__ __forceinline__ void pipe_row(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
+                                         const HaloLP<ST>& h, gcst<typename PairOf<ST>::raw>* in,
+                                         gst<typename PairOf<ST>::raw>* out, int y, typename PairOf<ST>::raw (&cur)[Q],
+                                         typename PairOf<ST>::raw (&nxt)[Q], typename PairOf<ST>::raw (&nn)[Q],
+                                         typename PairOf<ST>::raw& hnxt, typename PairOf<ST>::raw& hnn, float (&p)[3],
+                                         float (&hl)[3], float (&hr)[3])
+{
+    const int nx = t.cc.nx, ny = t.cc.ny;
+    if (y + 2 <= t.y1) {   // rows up to y1 feed the gradients
+        load_raw<ST>(in, nx, ny, y + 2, L, nn);
+        hnn = h.load(in, nx, ny, y + 2);
+    }
+    p[0] = p[1]; p[1] = p[2];
+    hl[0] = hl[1]; hl[1] = hl[2];
+    hr[0] = hr[1]; hr[1] = hr[2];
+    p[2] = raw_phase<ST>(nxt, rf);   // row y + 1 (waits for it)
+    halo_phase<ST>(hnxt, rf, hl[2], hr[2]);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) opaque(cur[i]);   // re-decode rather than keep decoded values alive
+    collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr);
+}
+
+template <typename ST, int KIND>
+__global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
+{
+    static_assert(KIND == KIND_P || KIND == KIND_F3, "one-row stencils only");
+    using R = typename PairOf<ST>::raw;
+    TileCtxLP t;
+    if (!tile_ctx(launch, t)) return;
+    const int nx = t.cc.nx, ny = t.cc.ny;
+    const ConstF k = to_float(t.cc);
+    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
+    const Lane L = make_lane_aligned(nx, t.strip);
+    gcst<R>* in = (gcst<R>*)t.in;
+    gst<R>* out = (gst<R>*)t.out;
+    HaloLP<ST> h;
+    h.init(nx, t.strip);
+    const int y0 = t.y0, y1 = t.y1;
+
+    R ra[Q], rb[Q], rc[Q];
+    R ha, hb, hc;
+    float p[3], hl[3], hr[3];
+    load_raw<ST>(in, nx, ny, y0 - 1, L, rc);
+    hc = h.load(in, nx, ny, y0 - 1);
+    load_raw<ST>(in, nx, ny, y0, L, ra);
+    ha = h.load(in, nx, ny, y0);
+    load_raw<ST>(in, nx, ny, y0 + 1, L, rb);
+    hb = h.load(in, nx, ny, y0 + 1);
+    p[1] = raw_phase<ST>(rc, rf);
+    halo_phase<ST>(hc, rf, hl[1], hr[1]);
+    p[2] = raw_phase<ST>(ra, rf);
+    halo_phase<ST>(ha, rf, hl[2], hr[2]);
+    for (int y = y0; y < y1; y += 3) {
+        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y, ra, rb, rc, hb, hc, p, hl, hr);
+        if (y + 1 >= y1) break;
+        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 1, rb, rc, ra, hc, ha, p, hl, hr);
+        if (y + 2 >= y1) break;
+        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 2, rc, ra, rb, ha, hb, p, hl, hr);
+    }
+}
+
 template <typename ST>
 hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
 {
     const dim3 grid(ntiles), block(64);
-    static const int ring = [] {
-        const char* e = std::getenv("ITOLBM_RING");
-        return e ? std::atoi(e) : 0;
+    // two-row pipeline for FP32 storage, one row ahead for FP16 storage (measured best on MI355X);
+    // ITOLBM_PIPE=0/1 forces one or the other
+    static const int pipe_env = [] {
+        const char* e = std::getenv("ITOLBM_PIPE");
+        return e ? std::atoi(e) : -1;
     }();
-    if (ring) {
-        switch (kind) {
-        case KIND_P: hipLaunchKernelGGL((lp_march<ST, KIND_P, 4>), grid, block, 0, stream, a); break;
-        case KIND_F3: hipLaunchKernelGGL((lp_march<ST, KIND_F3, 4>), grid, block, 0, stream, a); break;
-        case KIND_F1: hipLaunchKernelGGL((lp_march<ST, KIND_F1, 5>), grid, block, 0, stream, a); break;
-        case KIND_F2: hipLaunchKernelGGL((lp_march<ST, KIND_F2, 5>), grid, block, 0, stream, a); break;
-        default: return hipErrorInvalidValue;
-        }
-        return hipGetLastError();
-    }
+    const bool pipe = (pipe_env >= 0) ? pipe_env != 0 : sizeof(typename PairOf<ST>::raw) == 8;
     switch (kind) {
-    case KIND_P: hipLaunchKernelGGL((lp_step<ST, KIND_P>), grid, block, 0, stream, a); break;
-    case KIND_F3: hipLaunchKernelGGL((lp_step<ST, KIND_F3>), grid, block, 0, stream, a); break;
+    case KIND_P:
+        if (pipe) hipLaunchKernelGGL((lp_pipe<ST, KIND_P>), grid, block, 0, stream, a);
+        else hipLaunchKernelGGL((lp_row1<ST, KIND_P>), grid, block, 0, stream, a);
+        break;
+    case KIND_F3:
+        if (pipe) hipLaunchKernelGGL((lp_pipe<ST, KIND_F3>), grid, block, 0, stream, a);
+        else hipLaunchKernelGGL((lp_row1<ST, KIND_F3>), grid, block, 0, stream, a);
+        break;
     case KIND_F1: hipLaunchKernelGGL((lp_step<ST, KIND_F1>), grid, block, 0, stream, a); break;
     case KIND_F2: hipLaunchKernelGGL((lp_step<ST, KIND_F2>), grid, block, 0, stream, a); break;
     default: return hipErrorInvalidValue;

@@ -42,4 +42,67 @@ __device__ __forceinline__ Lane make_lane(int nx, int strip, int H)
     return L;
 }
 
+// Aligned strip (one-row stencils): the 64 lanes own the 64 columns strip*64 .. strip*64+63, so
+// every wavefront-wide load of a column row and every store starts on a 64-element boundary.  The
+// columns just outside the strip (its halo) are handled by Halo1.
+__device__ __forceinline__ Lane make_lane_aligned(int nx, int strip)
+{
+    Lane L;
+    L.lane = threadIdx.x;
+    const int xr = strip * 64 + L.lane;
+    L.col = wrapi(xr, nx);
+    L.xm = (L.col == 0) ? nx - 1 : L.col - 1;
+    L.xp = (L.col == nx - 1) ? 0 : L.col + 1;
+    L.interior = xr < nx;
+    return L;
+}
+
+// Bit masks of the velocities with c_x = +1, c_x = -1, c_y = +1, c_y = -1 (D3Q19 ordering).
+constexpr unsigned CXP_MASK = (1u << 1) | (1u << 7) | (1u << 9) | (1u << 11) | (1u << 13);
+constexpr unsigned CXN_MASK = (1u << 2) | (1u << 8) | (1u << 10) | (1u << 12) | (1u << 14);
+constexpr unsigned CYP_MASK = (1u << 3) | (1u << 7) | (1u << 10) | (1u << 16) | (1u << 18);
+constexpr unsigned CYN_MASK = (1u << 4) | (1u << 8) | (1u << 9) | (1u << 15) | (1u << 17);
+
+// The two halo columns of an aligned strip, strip*64 - 1 (site 0) and strip*64 + 64 (site 1): lane
+// j < 38 pulls population i = j mod 19 of halo site j / 19, i.e. the value at
+// (column - c_ix, row - c_iy).  One wavefront load per row fetches all 38.
+struct Halo1 {
+    int i;        // velocity of this lane's value
+    int dy;       // row offset of the pulled value (-c_iy)
+    unsigned x;   // column of the pulled value
+    bool on;      // lane < 38
+};
+
+__device__ __forceinline__ Halo1 make_halo1(int nx, int strip)
+{
+    Halo1 h;
+    const int lane = threadIdx.x;
+    h.on = lane < 2 * Q;
+    const int site = lane >= Q ? 1 : 0;
+    h.i = h.on ? lane - site * Q : 0;   // lanes >= 38: velocity 0 of site 1 (a valid address)
+    const int cx = (int)((CXP_MASK >> h.i) & 1u) - (int)((CXN_MASK >> h.i) & 1u);
+    const int cy = (int)((CYP_MASK >> h.i) & 1u) - (int)((CYN_MASK >> h.i) & 1u);
+    const int hc = strip * 64 + (site ? 64 : -1);
+    h.x = (unsigned)wrapi(hc - cx, nx);
+    h.dy = -cy;
+    return h;
+}
+
+// Value of lane `src` (a compile-time constant after unrolling) in every lane.
+__device__ __forceinline__ float lane_value(float v, int src)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+}
+__device__ __forceinline__ unsigned lane_value(unsigned v, int src)
+{
+    return (unsigned)__builtin_amdgcn_readlane((int)v, src);
+}
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 }  // namespace itolbm
