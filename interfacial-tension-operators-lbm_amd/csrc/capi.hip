@@ -65,6 +65,7 @@ struct Group {
     int ntiles = 0;
     int cap = 0;                   // entries dtab can hold
     bool written = false;
+    bool tuned = false;            // rows per segment chosen by timing (group_tune)
 };
 
 }  // namespace
@@ -74,6 +75,8 @@ struct lbm_ctx {
     std::string err;
     hipStream_t stream = nullptr;      // stream in use
     hipStream_t own_stream = nullptr;  // the context's own stream
+    hipEvent_t ev[2] = {nullptr, nullptr};   // autotuner timing
+    bool autotune = true;
     long long bytes = 0;
     Group family[N_KINDS];             // all cases of each kernel family (lbm_step)
     std::vector<Group> single;         // one group per case (lbm_step_case)
@@ -144,13 +147,8 @@ long long stride_pad()
     return 0;
 }
 
-// Strips per row: one-row stencils (P1, P3, F3) use aligned 64-column strips with separate halo
-// loads; F1 / F2 use 60 owned columns plus 2 halo lanes on each side.
-int strips_of(const lbm_case_desc& d)
-{
-    const int wi = (step_halo(d.op) == 1) ? 64 : 64 - 2 * step_halo(d.op);
-    return (d.nx + wi - 1) / wi;
-}
+// Strips per row: aligned 64-column strips (halo columns come from separate loads).
+int strips_of(const lbm_case_desc& d) { return (d.nx + 63) / 64; }
 
 // Rows per wavefront segment for a group: enough segments for >= 4096 wavefronts in the launch,
 // within 4..64 rows (fewer rows = more re-read halo rows).  ITOLBM_LY overrides.
@@ -410,6 +408,8 @@ int lbm_destroy(lbm_ctx* c)
     for (Group& g : c->family) free_group(g);
     for (Group& g : c->single) free_group(g);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     delete c;
     return LBM_OK;
 }
@@ -622,6 +622,69 @@ int group_sync(lbm_ctx* c, Group& g)
     return LBM_OK;
 }
 
+int launch_group(lbm_ctx* c, Group& g, int parity, int record)
+{
+    StepLaunch a{};
+    a.tab = g.dtab;
+    a.tile_case = g.dtile;
+    a.parity = parity;
+    a.record = record;
+    const int elem = c->cases[g.members[0]].elem;
+    HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, c->stream)
+                        : launch_step_lp(elem, g.kind, a, g.ntiles, c->stream));
+    return LBM_OK;
+}
+
+// Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel).
+// Each candidate's launch is timed at the group's current parity: it reads the current state and
+// writes the other buffer, so repeating it leaves the simulation unchanged, and the results of a
+// step do not depend on the tile shape (every site is computed the same way), so the choice only
+// changes speed.  Short rows re-read more halo rows; long rows give fewer, longer wavefronts.
+int group_tune(lbm_ctx* c, Group& g)
+{
+    g.tuned = true;
+    if (std::getenv("ITOLBM_LY")) return LBM_OK;
+    int ny_max = 1;
+    long long sites = 0;
+    for (int id : g.members) {
+        ny_max = std::max(ny_max, c->cases[id].d.ny);
+        sites += c->cases[id].n;
+    }
+    if (sites < 4096) return LBM_OK;   // too small to matter
+    if (!c->ev[0]) {
+        HIPCHK(c, hipEventCreate(&c->ev[0]));
+        HIPCHK(c, hipEventCreate(&c->ev[1]));
+    }
+    const int cand[] = {4, 8, 16, 32, 64};
+    int best = -1;
+    float best_ms = 0.f;
+    int last_eff = -1;
+    for (int ly : cand) {
+        const int eff = std::min(ly, ny_max);
+        if (eff == last_eff) continue;
+        last_eff = eff;
+        g.ly.assign(g.members.size(), ly);
+        g.written = false;
+        if (int r = group_sync(c, g)) return r;
+        const int parity = g.k & 1;
+        if (int r = launch_group(c, g, parity, 0)) return r;   // warm-up
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        for (int k = 0; k < 3; ++k)
+            if (int r = launch_group(c, g, parity, 0)) return r;
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        if (best < 0 || ms < best_ms) {
+            best = ly;
+            best_ms = ms;
+        }
+    }
+    g.ly.assign(g.members.size(), best);
+    g.written = false;
+    return group_sync(c, g);
+}
+
 int group_step(lbm_ctx* c, Group& g, int n_steps, int flags)
 {
     if (g.members.empty() || n_steps <= 0) return LBM_OK;
@@ -630,15 +693,10 @@ int group_step(lbm_ctx* c, Group& g, int n_steps, int flags)
         for (int id : g.members)
             if (int r = ensure_aux(c, c->cases[id])) return r;
     if (int r = group_sync(c, g)) return r;
+    if (!g.tuned && c->autotune)
+        if (int r = group_tune(c, g)) return r;
     for (int s = 0; s < n_steps; ++s) {
-        StepLaunch a{};
-        a.tab = g.dtab;
-        a.tile_case = g.dtile;
-        a.parity = g.k & 1;
-        a.record = (rec && s == n_steps - 1) ? 1 : 0;
-        const int elem = c->cases[g.members[0]].elem;
-        HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, c->stream)
-                            : launch_step_lp(elem, g.kind, a, g.ntiles, c->stream));
+        if (int r = launch_group(c, g, g.k & 1, (rec && s == n_steps - 1) ? 1 : 0)) return r;
         g.k++;
         for (int id : g.members) c->cases[id].cur ^= 1;
     }
@@ -696,6 +754,7 @@ int lbm_set_active(lbm_ctx* c, int id, int active)
         if (c->cases[j].active && c->cases[j].kind == k.kind) g.members.push_back(j);
     g.ly.clear();   // row segments are re-chosen for the new member set
     g.written = false;
+    g.tuned = false;
     return LBM_OK;
 }
 

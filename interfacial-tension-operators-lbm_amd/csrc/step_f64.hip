@@ -122,6 +122,21 @@ __device__ __forceinline__ double lane_dens(double v, int base)
     return r;
 }
 
+// value g (0 .. 75) of a two-load halo (Halo2)
+__device__ __forceinline__ double halo2_value(double a, double b, int g)
+{
+    return (g < 64) ? lane_value(a, g) : lane_value(b, g - 64);
+}
+
+// density of halo site s of a two-load halo, summed in index order as dens()
+__device__ __forceinline__ double halo2_dens(double a, double b, int s)
+{
+    double r = halo2_value(a, b, Q * s);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) r = r + halo2_value(a, b, Q * s + i);
+    return r;
+}
+
 // Mediator momentum gm = sum_i c_i M_i, M_i(x) = w_i (3 phi(x - c_i)):
 // emite_mediadores + propag_site_med + momentum (LBM_functions.cpp:1286-1337, 1056-1087).
 __device__ __forceinline__ void mediator_gradient(const Nine& phi, double& gx, double& gy, double& gz)
@@ -494,6 +509,10 @@ __device__ __forceinline__ Aux6 aux_of(const CaseConst& cc, double g0, double g1
     return r;
 }
 
+// Aligned 64-column strips for F1 / F2: the colour field is needed two columns beyond the strip on
+// each side; those four halo columns come from two extra loads per colour and row (Halo2) and their
+// gradient and n / T fields are evaluated on the wavefront's uniform values with the same functions
+// as every lane's, so the results are identical to the owning lanes'.
 template <int OP>
 __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
 {
@@ -502,37 +521,89 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
-    const Lane L = make_lane(nx, t.strip, 2);
+    const Lane L = make_lane_aligned(nx, t.strip);
+    const Halo2 h = make_halo2(nx, t.strip);
+    gcdouble* const hR0 = t.inR + h.i[0] * t.S;
+    gcdouble* const hB0 = t.inB + h.i[0] * t.S;
+    gcdouble* const hR1 = t.inR + h.i[1] * t.S;
+    gcdouble* const hB1 = t.inB + h.i[1] * t.S;
 
     double fR[Q], fB[Q];
-    auto phi_row = [&](int y) {
+    // colour field of a row: own lanes, and the four halo columns (c0-2, c0-1, c0+64, c0+65)
+    auto phi_row = [&](int y, double hw[4]) {
         load_row(t.inR, t.inB, t.S, nx, ny, y, L, fR, fB);
+        const unsigned s0 = (unsigned)wrap_row(y + h.dy[0], ny) * (unsigned)nx + h.x[0];
+        const unsigned s1 = (unsigned)wrap_row(y + h.dy[1], ny) * (unsigned)nx + h.x[1];
+        const double ra = hR0[s0], ba = hB0[s0], rb = hR1[s1], bb = hB1[s1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) hw[q] = phase(halo2_dens(ra, rb, q), halo2_dens(ba, bb, q));
         return phase(dens(fR), dens(fB));
     };
-    double pa = phi_row(t.y0 - 2), pb = phi_row(t.y0 - 1), pc = phi_row(t.y0), pd = phi_row(t.y0 + 1);
+    // rows y-2 .. y+2 of the colour field: own lanes P, halo columns HW
+    double P[5], HW[5][4];
+    P[0] = phi_row(t.y0 - 2, HW[0]);
+    P[1] = phi_row(t.y0 - 1, HW[1]);
+    P[2] = phi_row(t.y0, HW[2]);
+    P[3] = phi_row(t.y0 + 1, HW[3]);
+    // gradient of the colour field at the centre row of three (k .. k+2 of the window), own lanes
+    auto grad_lanes = [&](int k, double& g0, double& g1, double& g2) {
+        const double r[3] = {P[k], P[k + 1], P[k + 2]};
+        const double hl[3] = {HW[k][1], HW[k + 1][1], HW[k + 2][1]};
+        const double hr[3] = {HW[k][2], HW[k + 1][2], HW[k + 2][2]};
+        phase_gradient(gather9h(r, hl, hr, L.lane), g0, g1, g2);
+    };
+    // n / T at the halo columns c0-1 (side 0) and c0+64 (side 1), centre row k+1 of the window
+    auto aux_halo = [&](int k, int side) {
+        Nine n;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            if (side == 0) {
+                n.v[j][0] = HW[k + j][0];
+                n.v[j][1] = HW[k + j][1];
+                n.v[j][2] = lane_value(P[k + j], 0);
+            } else {
+                n.v[j][0] = lane_value(P[k + j], 63);
+                n.v[j][1] = HW[k + j][2];
+                n.v[j][2] = HW[k + j][3];
+            }
+        }
+        double g0, g1, g2;
+        phase_gradient(n, g0, g1, g2);
+        return aux_of<OP>(cc, g0, g1, g2);
+    };
     double gm0, gm1, gm2, g00, g01, g02;
-    phase_gradient(gather9(pa, pb, pc, L.lane), gm0, gm1, gm2);     // g at row y0-1
-    phase_gradient(gather9(pb, pc, pd, L.lane), g00, g01, g02);     // g at row y0
+    grad_lanes(0, gm0, gm1, gm2);   // row y0-1
+    grad_lanes(1, g00, g01, g02);   // row y0
     Aux6 am = aux_of<OP>(cc, gm0, gm1, gm2), a0 = aux_of<OP>(cc, g00, g01, g02);
 
     for (int y = t.y0; y < t.y1; ++y) {
-        const double pe = phi_row(y + 2);
+        P[4] = phi_row(y + 2, HW[4]);
         // populations of row y again (read two rows ago for the colour field, now in L2); issued
         // before the force evaluation so that they arrive while it runs
         load_row(t.inR, t.inB, t.S, nx, ny, y, L, fR, fB);
         double gp0, gp1, gp2;
-        phase_gradient(gather9(pc, pd, pe, L.lane), gp0, gp1, gp2);   // g at row y+1
+        grad_lanes(2, gp0, gp1, gp2);   // row y+1
         const Aux6 ap = aux_of<OP>(cc, gp0, gp1, gp2);
+        // halo n / T for rows y-1, y, y+1
+        Aux6 hl[3], hr[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            hl[j] = aux_halo(j, 0);
+            hr[j] = aux_halo(j, 1);
+        }
 
         // neighbour aux values: A[k][dy+1][dx+1]
         double A[NA][3][3];
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
-            const Nine n = gather9(am.a[k], a0.a[k], ap.a[k], L.lane);
+            const double r[3] = {am.a[k], a0.a[k], ap.a[k]};
+            const double l3[3] = {hl[0].a[k], hl[1].a[k], hl[2].a[k]};
+            const double r3[3] = {hr[0].a[k], hr[1].a[k], hr[2].a[k]};
+            const Nine n = gather9h(r, l3, r3, L.lane);
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
+            for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) A[k][r][c] = n.v[r][c];
+                for (int c = 0; c < 3; ++c) A[k][rr][c] = n.v[rr][c];
         }
         double Fx, Fy, Fz;
         if constexpr (OP == OP_F1) {
@@ -568,7 +639,12 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
         }
 
         double gx, gy, gz;
-        mediator_gradient(gather9(pb, pc, pd, L.lane), gx, gy, gz);
+        {
+            const double r[3] = {P[1], P[2], P[3]};
+            const double hl1[3] = {HW[1][1], HW[2][1], HW[3][1]};
+            const double hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
+            mediator_gradient(gather9h(r, hl1, hr0, L.lane), gx, gy, gz);
+        }
         const double mod = sqrt(gx * gx + gy * gy + gz * gz);
 
         if (L.interior) {
@@ -589,7 +665,12 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
-        pa = pb; pb = pc; pc = pd; pd = pe;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            P[k] = P[k + 1];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) HW[k][q] = HW[k + 1][q];
+        }
         g00 = gp0; g01 = gp1; g02 = gp2;
         am = a0; a0 = ap;
     }

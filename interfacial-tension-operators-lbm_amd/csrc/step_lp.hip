@@ -383,101 +383,175 @@ __device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
     return a;
 }
 
-// One-row-ahead strip kernel for the two-row stencils (F1, F2): strips of 60 owned columns with two
-// halo lanes on each side; rows y-2 .. y+2 of the colour field are kept, and row y's populations are
-// read again (from L2) when its force is known.
-template <typename ST, int KIND>
-__global__ void __launch_bounds__(64) lp_step(StepLaunch launch)
+// The four halo columns of an aligned strip for the pair layout (Halo2): two loads per row.
+template <typename ST>
+struct Halo2LP {
+    using R = typename PairOf<ST>::raw;
+    unsigned long long off[2];   // element offset of (velocity i, column x) within a row block
+    int dy[2];
+    __device__ __forceinline__ void init(int nx, int strip)
+    {
+        const Halo2 h = make_halo2(nx, strip);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            off[k] = (unsigned long long)h.i[k] * (unsigned)nx + h.x[k];
+            dy[k] = h.dy[k];
+        }
+    }
+    __device__ __forceinline__ void load(gcst<R>* in, int nx, int ny, int y, R& a, R& b) const
+    {
+        const unsigned long long pitch = (unsigned long long)Q * (unsigned)nx;
+        a = in[(unsigned long long)wrap_row(y + dy[0], ny) * pitch + off[0]];
+        b = in[(unsigned long long)wrap_row(y + dy[1], ny) * pitch + off[1]];
+    }
+};
+
+// Colour field of the four halo sites of a two-load halo: per-lane decode as raw_phase does, then
+// the site's f_i and e_i f_i summed in index order (bit-identical to the owning lane's raw_phase).
+template <typename ST>
+__device__ __forceinline__ void halo2_phase(typename PairOf<ST>::raw va, typename PairOf<ST>::raw vb, Ref rf,
+                                            float (&hw)[4])
 {
-    constexpr int H = (KIND == KIND_F1 || KIND == KIND_F2) ? 2 : 1;
+    const int lane = threadIdx.x;
+    float f[2], ef[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        int g = k * 64 + lane;
+        if (g >= 4 * Q) g = 4 * Q - 1;
+        const int i = g % Q;
+        const float w = (i == 0) ? WF[0] : (i < 7 ? WF[1] : WF[7]);
+        const auto v = k ? vb : va;
+        const float e = PairOf<ST>::e(v);
+        const float si = (PairOf<ST>::a(v) + (rf.mid + rf.half * e)) * w;
+        f[k] = si + w;
+        ef[k] = e * f[k];
+    }
+#pragma unroll
+    for (int site = 0; site < 4; ++site) {
+        float D = 0.f, F = 0.f;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int g = Q * site + i;
+            D += (g < 64) ? lane_value(ef[0], g) : lane_value(ef[1], g - 64);
+            F += (g < 64) ? lane_value(f[0], g) : lane_value(f[1], g - 64);
+        }
+        hw[site] = D / F;
+    }
+}
+
+// Aligned 64-column strip kernel for the two-row stencils (F1, F2): rows y-2 .. y+2 of the colour
+// field are kept for the own lanes (P) and the four halo columns (HW); the gradient and the n / T
+// field at the halo columns c0-1 and c0+64 are evaluated on the wavefront's uniform values with the
+// same functions as every lane's.  Row y's populations are read again (from L2) once its force is
+// known.
+template <typename ST, int KIND>
+__global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
+{
+    static_assert(KIND == KIND_F1 || KIND == KIND_F2, "two-row stencils only");
     using R = typename PairOf<ST>::raw;
     TileCtxLP t;
     if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
     const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
-    const Lane L = make_lane(nx, t.strip, H);
+    const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
+    Halo2LP<ST> h;
+    h.init(nx, t.strip);
     const int y0 = t.y0, y1 = t.y1;
 
-    R cur[Q], nxt[Q];
-    float p[2 * H + 1];                  // colour field of rows y-H .. y+H (p[H] = row y)
-    AuxF am{}, a0{};
+    R cur[Q];
+    float P[5], HW[5][4];
+    auto phi_row = [&](int y, float (&hw)[4]) {
+        load_raw<ST>(in, nx, ny, y, L, cur);
+        R va, vb;
+        h.load(in, nx, ny, y, va, vb);
+        halo2_phase<ST>(va, vb, rf, hw);
+        return raw_phase<ST>(cur, rf);
+    };
+    auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
+        const float hl[3] = {HW[j][1], HW[j + 1][1], HW[j + 2][1]};
+        const float hr[3] = {HW[j][2], HW[j + 1][2], HW[j + 2][2]};
+        return aux_of<KIND>(k, gather9h(P[j], P[j + 1], P[j + 2], hl, hr, L.lane));
+    };
+    auto aux_halo = [&](int j, int side) {   // n / T at c0-1 (side 0) or c0+64 (side 1)
+        Nine n;
 #pragma unroll
-    for (int j = 0; j < 2 * H; ++j) {
-        load_raw<ST>(in, nx, ny, y0 - H + j, L, nxt);
-        p[j + 1] = raw_phase<ST>(nxt, rf);   // rows y0-H .. y0+H-1 at p[1 .. 2H]
-    }
-    if constexpr (H == 2) {
-        am = aux_of<KIND>(k, gather9(p[1], p[2], p[3], L.lane));   // row y0 - 1
-        a0 = aux_of<KIND>(k, gather9(p[2], p[3], p[4], L.lane));   // row y0
-    }
-    load_raw<ST>(in, nx, ny, y0, L, cur);
+        for (int r = 0; r < 3; ++r) {
+            if (side == 0) {
+                n.v[r][0] = HW[j + r][0];
+                n.v[r][1] = HW[j + r][1];
+                n.v[r][2] = lane_value(P[j + r], 0);
+            } else {
+                n.v[r][0] = lane_value(P[j + r], 63);
+                n.v[r][1] = HW[j + r][2];
+                n.v[r][2] = HW[j + r][3];
+            }
+        }
+        return aux_of<KIND>(k, n);
+    };
+    P[0] = phi_row(y0 - 2, HW[0]);
+    P[1] = phi_row(y0 - 1, HW[1]);
+    P[2] = phi_row(y0, HW[2]);
+    P[3] = phi_row(y0 + 1, HW[3]);
+    AuxF am = aux_lanes(0), a0 = aux_lanes(1);
 
     for (int y = y0; y < y1; ++y) {
-        load_raw<ST>(in, nx, ny, y + H, L, nxt);
-        if constexpr (H == 2) {
-            // cur was loaded for row y two iterations ago only if H == 1; reload it (L2) for F1/F2
-        }
+        P[4] = phi_row(y + 2, HW[4]);
+        load_raw<ST>(in, nx, ny, y, L, cur);   // row y again, for its collision
+        const AuxF ap = aux_lanes(2);
+        AuxF hl[3], hr[3];
 #pragma unroll
-        for (int j = 0; j < 2 * H; ++j) p[j] = p[j + 1];
-        p[2 * H] = raw_phase<ST>(nxt, rf);   // row y + H
-        const Nine p9 = gather9(p[H - 1], p[H], p[H + 1], L.lane);
-        const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
-        float Fx = 0.f, Fy = 0.f;
-        if constexpr (KIND == KIND_F3) {
-            const float phc = p[H];
-            const float mu = 4.f * k.bf * phc * (phc * phc - 1.f) - k.kf * lap_iso(p9);
-            Fx = -mu * gx;
-            Fy = -mu * gy;
-        } else if constexpr (H == 2) {
-            const AuxF ap = aux_of<KIND>(k, gather9(p[2], p[3], p[4], L.lane));    // row y + 1
-            if constexpr (KIND == KIND_F1) {
-                const float divn = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) + dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
-                Fx = -k.hs * divn * a0.g0;
-                Fy = -k.hs * divn * a0.g1;
-            } else {
-                const Nine txy = gather9(am.a2, a0.a2, ap.a2, L.lane);
-                Fx = dx_iso(gather9(am.a0, a0.a0, ap.a0, L.lane)) + dy_iso(txy);
-                Fy = dx_iso(txy) + dy_iso(gather9(am.a1, a0.a1, ap.a1, L.lane));
-            }
-            am = a0;
-            a0 = ap;
+        for (int r = 0; r < 3; ++r) {
+            hl[r] = aux_halo(r, 0);
+            hr[r] = aux_halo(r, 1);
         }
+        auto field = [&](float AuxF::*m) {
+            const float l3[3] = {hl[0].*m, hl[1].*m, hl[2].*m};
+            const float r3[3] = {hr[0].*m, hr[1].*m, hr[2].*m};
+            return gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
+        };
+        float Fx, Fy;
+        if constexpr (KIND == KIND_F1) {
+            const float divn = dx_iso(field(&AuxF::a0)) + dy_iso(field(&AuxF::a1));
+            Fx = -k.hs * divn * a0.g0;
+            Fy = -k.hs * divn * a0.g1;
+        } else {
+            const Nine txy = field(&AuxF::a2);
+            Fx = dx_iso(field(&AuxF::a0)) + dy_iso(txy);
+            Fy = dx_iso(txy) + dy_iso(field(&AuxF::a1));
+        }
+        const float hl1[3] = {HW[1][1], HW[2][1], HW[3][1]};
+        const float hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
+        const Nine p9 = gather9h(P[1], P[2], P[3], hl1, hr0, L.lane);
+        const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
         if (L.interior) {
-            if constexpr (H == 2) load_raw<ST>(in, nx, ny, y, L, cur);
             Row row;
             decode<ST>(cur, row, rf);
             float ds, rho, phi;
             row_phase(row, ds, rho, phi);
             gst<R>* orow = out + (long long)y * ((long long)Q * nx);
             const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-            auto st = [&](int i, float s, float e) {
-                put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf));
-            };
+            auto st = [&](int i, float s, float e) { put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf)); };
             if (rho > 1.0e-30f) {
-                collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+                collide_lp<true>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
             } else {
 #pragma unroll
                 for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
             }
             if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
         }
-        if constexpr (H == 1) {
 #pragma unroll
-            for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
+        for (int r = 0; r < 4; ++r) {
+            P[r] = P[r + 1];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) HW[r][q] = HW[r + 1][q];
         }
+        am = a0;
+        a0 = ap;
     }
 }
-
-__device__ __forceinline__ void opaque(f32x2& v)
-{
-    float a = v.x, b = v.y;
-    asm volatile("" : "+v"(a), "+v"(b));
-    v = f32x2{a, b};
-}
-__device__ __forceinline__ void opaque(unsigned& v) { asm volatile("" : "+v"(v)); }
 
 // Halo of an aligned strip for the pair layout: one 38-lane load per row (Halo1).  The address is
 // computed from a per-lane row index, apart from the strip's own loads, whose row bases stay uniform
@@ -585,6 +659,14 @@ __global__ void __launch_bounds__(64) lp_row1(StepLaunch launch)
     }
 }
 
+__device__ __forceinline__ void opaque(f32x2& v)
+{
+    float a = v.x, b = v.y;
+    asm volatile("" : "+v"(a), "+v"(b));
+    v = f32x2{a, b};
+}
+__device__ __forceinline__ void opaque(unsigned& v) { asm volatile("" : "+v"(v)); }
+
 // Two-row-deep pipelined variant: three raw row buffers rotate, so while row y is collided the
 // loads of row y+2 are in flight.  The row loop is unrolled three times (buffer roles are
 // compile-time constants).
@@ -670,8 +752,8 @@ hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t s
         if (pipe) hipLaunchKernelGGL((lp_pipe<ST, KIND_F3>), grid, block, 0, stream, a);
         else hipLaunchKernelGGL((lp_row1<ST, KIND_F3>), grid, block, 0, stream, a);
         break;
-    case KIND_F1: hipLaunchKernelGGL((lp_step<ST, KIND_F1>), grid, block, 0, stream, a); break;
-    case KIND_F2: hipLaunchKernelGGL((lp_step<ST, KIND_F2>), grid, block, 0, stream, a); break;
+    case KIND_F1: hipLaunchKernelGGL((lp_f12<ST, KIND_F1>), grid, block, 0, stream, a); break;
+    case KIND_F2: hipLaunchKernelGGL((lp_f12<ST, KIND_F2>), grid, block, 0, stream, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

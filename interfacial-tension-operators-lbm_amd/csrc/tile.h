@@ -88,6 +88,41 @@ __device__ __forceinline__ Halo1 make_halo1(int nx, int strip)
     return h;
 }
 
+// The four halo columns of an aligned strip for two-row stencils: strip*64 - 2, - 1 (sites 0, 1)
+// and strip*64 + 64, + 65 (sites 2, 3).  Their 76 pulled values take two wavefront loads per row:
+// value g = 19 s + i sits in lane g of load A (g < 64) or lane g - 64 of load B.  Lanes of load B
+// past 11 re-read a valid address that no one uses.
+struct Halo2 {
+    int i[2];        // velocity of this lane's value in loads A, B
+    int dy[2];       // -c_iy
+    unsigned x[2];   // pulled column
+};
+
+__device__ __forceinline__ Halo2 make_halo2(int nx, int strip)
+{
+    Halo2 h;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int slot = 0; slot < 2; ++slot) {
+        int g = slot * 64 + lane;
+        if (g >= 4 * Q) g = 4 * Q - 1;
+        const int site = g / Q, i = g - site * Q;
+        const int cx = (int)((CXP_MASK >> i) & 1u) - (int)((CXN_MASK >> i) & 1u);
+        const int cy = (int)((CYP_MASK >> i) & 1u) - (int)((CYN_MASK >> i) & 1u);
+        const int hc = strip * 64 + (site < 2 ? site - 2 : 64 + site - 2);
+        h.i[slot] = i;
+        h.dy[slot] = -cy;
+        h.x[slot] = (unsigned)wrapi(hc - cx, nx);
+    }
+    return h;
+}
+
+__device__ __forceinline__ int wrap_row(int y, int ny)
+{
+    y = (y < 0) ? y + ny : y;
+    return (y >= ny) ? y - ny : y;
+}
+
 // Value of lane `src` (a compile-time constant after unrolling) in every lane.
 __device__ __forceinline__ float lane_value(float v, int src)
 {
