@@ -76,6 +76,9 @@ struct lbm_ctx {
     hipStream_t stream = nullptr;      // stream in use
     hipStream_t own_stream = nullptr;  // the context's own stream
     hipEvent_t ev[2] = {nullptr, nullptr};   // autotuner timing
+    hipEvent_t fork = nullptr;                 // lbm_step: families run on side streams
+    hipStream_t side[N_KINDS] = {};
+    hipEvent_t join[N_KINDS] = {};
     bool autotune = true;
     long long bytes = 0;
     Group family[N_KINDS];             // all cases of each kernel family (lbm_step)
@@ -150,8 +153,8 @@ long long stride_pad()
 // Strips per row: aligned 64-column strips (halo columns come from separate loads).
 int strips_of(const lbm_case_desc& d) { return (d.nx + 63) / 64; }
 
-// Rows per wavefront segment for a group: enough segments for >= 4096 wavefronts in the launch,
-// within 4..64 rows (fewer rows = more re-read halo rows).  ITOLBM_LY overrides.
+// Rows per wavefront segment for a group before tuning: enough segments for >= 4096 wavefronts in
+// the launch, within 1..64 rows (fewer rows = more re-read halo rows).  ITOLBM_LY overrides.
 int group_rows(const std::vector<CaseState>& cs, const std::vector<int>& members)
 {
     if (const char* e = std::getenv("ITOLBM_LY")) {
@@ -161,7 +164,7 @@ int group_rows(const std::vector<CaseState>& cs, const std::vector<int>& members
     long long strip_rows = 0;
     for (int id : members) strip_rows += (long long)strips_of(cs[id].d) * cs[id].d.ny;
     long long ly = strip_rows / 4096;
-    if (ly < 4) ly = 4;
+    if (ly < 1) ly = 1;
     if (ly > 64) ly = 64;
     return (int)ly;
 }
@@ -410,6 +413,11 @@ int lbm_destroy(lbm_ctx* c)
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    for (int k = 0; k < N_KINDS; ++k) {
+        if (c->side[k]) (void)hipStreamDestroy(c->side[k]);
+        if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+    }
     delete c;
     return LBM_OK;
 }
@@ -622,7 +630,7 @@ int group_sync(lbm_ctx* c, Group& g)
     return LBM_OK;
 }
 
-int launch_group(lbm_ctx* c, Group& g, int parity, int record)
+int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
 {
     StepLaunch a{};
     a.tab = g.dtab;
@@ -630,10 +638,10 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record)
     a.parity = parity;
     a.record = record;
     const int elem = c->cases[g.members[0]].elem;
-    HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, c->stream)
-                        : launch_step_lp(elem, g.kind, a, g.ntiles, c->stream));
+    HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, st) : launch_step_lp(elem, g.kind, a, g.ntiles, st));
     return LBM_OK;
 }
+int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_group(c, g, parity, record, c->stream); }
 
 // Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel).
 // Each candidate's launch is timed at the group's current parity: it reads the current state and
@@ -645,17 +653,12 @@ int group_tune(lbm_ctx* c, Group& g)
     g.tuned = true;
     if (std::getenv("ITOLBM_LY")) return LBM_OK;
     int ny_max = 1;
-    long long sites = 0;
-    for (int id : g.members) {
-        ny_max = std::max(ny_max, c->cases[id].d.ny);
-        sites += c->cases[id].n;
-    }
-    if (sites < 4096) return LBM_OK;   // too small to matter
+    for (int id : g.members) ny_max = std::max(ny_max, c->cases[id].d.ny);
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
         HIPCHK(c, hipEventCreate(&c->ev[1]));
     }
-    const int cand[] = {4, 8, 16, 32, 64};
+    const int cand[] = {1, 2, 4, 8, 16, 32, 64};
     int best = -1;
     float best_ms = 0.f;
     int last_eff = -1;
@@ -685,24 +688,34 @@ int group_tune(lbm_ctx* c, Group& g)
     return group_sync(c, g);
 }
 
-int group_step(lbm_ctx* c, Group& g, int n_steps, int flags)
+// Tables, aux buffers and tile shape of a group, before its launches are queued.
+int group_prepare(lbm_ctx* c, Group& g, bool rec)
 {
-    if (g.members.empty() || n_steps <= 0) return LBM_OK;
-    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     if (rec)
         for (int id : g.members)
             if (int r = ensure_aux(c, c->cases[id])) return r;
     if (int r = group_sync(c, g)) return r;
     if (!g.tuned && c->autotune)
         if (int r = group_tune(c, g)) return r;
+    return LBM_OK;
+}
+
+int group_step(lbm_ctx* c, Group& g, int n_steps, int flags, hipStream_t st, bool prepared)
+{
+    if (g.members.empty() || n_steps <= 0) return LBM_OK;
+    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    if (!prepared)
+        if (int r = group_prepare(c, g, rec)) return r;
     for (int s = 0; s < n_steps; ++s) {
-        if (int r = launch_group(c, g, g.k & 1, (rec && s == n_steps - 1) ? 1 : 0)) return r;
+        if (int r = launch_group(c, g, g.k & 1, (rec && s == n_steps - 1) ? 1 : 0, st)) return r;
         g.k++;
         for (int id : g.members) c->cases[id].cur ^= 1;
     }
     for (int id : g.members) c->cases[id].aux_valid = rec;
     return LBM_OK;
 }
+
+int group_step(lbm_ctx* c, Group& g, int n_steps, int flags) { return group_step(c, g, n_steps, flags, c->stream, false); }
 
 }  // namespace
 
@@ -719,10 +732,32 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
 {
     if (!c) return LBM_E_ARG;
     if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
-    for (int s = 0; s < n_steps; ++s) {
-        const int f = (s == n_steps - 1) ? flags : 0;
-        for (Group& g : c->family)
-            if (int r = group_step(c, g, 1, f)) return r;
+    if (n_steps == 0) return LBM_OK;
+    std::vector<Group*> act;
+    for (Group& g : c->family)
+        if (!g.members.empty()) act.push_back(&g);
+    if (act.size() == 1) return group_step(c, *act[0], n_steps, flags);
+    if (act.empty()) return LBM_OK;
+    // The kernel families advance disjoint sets of cases, so each runs all n steps on a stream of
+    // its own (forked from and joined back to the context's stream): small launches of different
+    // families fill the GPU together.
+    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    for (Group* g : act)
+        if (int r = group_prepare(c, *g, rec)) return r;
+    if (!c->fork) {
+        HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+        for (int k = 0; k < N_KINDS; ++k) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
+            HIPCHK(c, hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->fork, c->stream));
+    for (size_t j = 0; j < act.size(); ++j) {
+        hipStream_t st = c->side[j];
+        HIPCHK(c, hipStreamWaitEvent(st, c->fork, 0));
+        if (int r = group_step(c, *act[j], n_steps, flags, st, true)) return r;
+        HIPCHK(c, hipEventRecord(c->join[j], st));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[j], 0));
     }
     return LBM_OK;
 }

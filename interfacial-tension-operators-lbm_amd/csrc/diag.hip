@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(64) drop_ordered(OrderedArgs a)
     double acc = 0.0;
     int x = 0, y = 0;
     long long p = 0;
-    constexpr int U = 8;   // loads of U sites are issued before their additions
+    constexpr int U = 32;   // loads of U sites are issued before their additions
     for (; p + U <= a.n; p += U) {
         double t[U], w[U];
 #pragma unroll
