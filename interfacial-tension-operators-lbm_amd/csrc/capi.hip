@@ -279,7 +279,7 @@ int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
     for (long long p = 0; p < k.n; ++p) {
         const long long y = p / nx, x = p - y * nx;
         for (int i = 0; i < Q; ++i) {
-            const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+            const double w = (i == 0) ? WL0 : (i < 7 ? WL1 : WL2);   // the weights the device decodes with
             const double r = h[(size_t)i * k.S + p], b = h[(size_t)(Q + i) * k.S + p];
             const double f = r + b;
             const size_t j = 2 * (size_t)((y * Q + i) * nx + x);

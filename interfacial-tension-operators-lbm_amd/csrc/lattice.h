@@ -22,6 +22,15 @@ constexpr double W0 = 1. / 3.;
 constexpr double W1 = 1. / 18.;
 constexpr double W2 = 1. / 36.;
 
+// Lattice weights of the FP32 / FP16 forms.  The FP32 collision hands the density deviation
+// sum_i s_i on multiplied by the exact sum of its weights; with the nearest floats to 1/3, 1/18,
+// 1/36 that sum is 1 + 1.5e-8, which compounds into a drift of rho - 1 of 3e-4 over 20000 steps.
+// These weights sum to exactly 1: WL1 is the float nearest 1/18 for which 1 - 12 WL1 is a float,
+// WL2 = WL1 / 2, WL0 = 1 - 12 WL1 (relative offsets from 1/3, 1/18, 1/36: 1.2e-7, -6e-8, -6e-8).
+constexpr float WL0 = 0x1.555558p-2f;
+constexpr float WL1 = 0x1.c71c7p-5f;
+constexpr float WL2 = 0x1.c71c7p-6f;
+
 // operator ids of Operadores_tensao.cpp:45-49
 enum OpId { OP_P1 = 1, OP_P3 = 3, OP_F1 = 11, OP_F2 = 12, OP_F3 = 13 };
 

@@ -29,9 +29,7 @@ namespace itolbm {
 
 namespace {
 
-constexpr float WF[Q] = {(float)W0, (float)W1, (float)W1, (float)W1, (float)W1, (float)W1, (float)W1,
-                         (float)W2, (float)W2, (float)W2, (float)W2, (float)W2, (float)W2, (float)W2,
-                         (float)W2, (float)W2, (float)W2, (float)W2, (float)W2};
+constexpr float WF[Q] = {WL0, WL1, WL1, WL1, WL1, WL1, WL1, WL2, WL2, WL2, WL2, WL2, WL2, WL2, WL2, WL2, WL2, WL2, WL2};
 constexpr float WAX = (float)(W1 + 2.0 * W2);   // in-plane axis weight of the folded stencil
 constexpr float WDG = (float)W2;                // in-plane diagonal weight
 constexpr float ISQ2 = 0.70710678118654752f;    // 1 / |c| for the diagonals
@@ -135,7 +133,7 @@ __device__ __forceinline__ void decode(const typename PairOf<ST>::raw r[Q], Row&
 template <typename ST>
 __device__ __forceinline__ typename PairOf<ST>::raw encode(int i, float s, float e, Ref rf)
 {
-    return PairOf<ST>::make(s * (1.f / WF[i]) - (rf.mid + rf.half * e), e);
+    return PairOf<ST>::make(s / WF[i] - (rf.mid + rf.half * e), e);
 }
 
 // density deviation, rho and phi of a site.  phi sums e_i f_i and f_i in the same order, so a pure
@@ -770,7 +768,7 @@ __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw*
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const auto r = pop[(y * Q + i) * nx + x];
-            const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+            const double w = (double)WF[i];
             const float e = PairOf<ST>::e(r);
             const float A = PairOf<ST>::a(r) + (rf.mid + rf.half * e);
             const double f = w * (1.0 + (double)A);
@@ -817,7 +815,7 @@ __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __
             } else {
                 phi = (r < R) ? 1.0 : 0.0;
             }
-            const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
+            const double w = (double)WF[i];
             const double cv = (double)CX[i] * v0;
             const double g = w * (1.0 + 3.0 * cv + 4.5 * cv * cv - 1.5 * v0 * v0);
             const double fR = g * phi, fB = g * (1.0 - phi), f = fR + fB;
