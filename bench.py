@@ -17,7 +17,8 @@ storage forms are timed the same way and reported beside it.  Also reported:
                  (profiles/traffic_<form>.json), when present
   large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in the headline form
   sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
-                 numpy default_rng(2026), box 4R): batched reduced-precision form vs the plain FP64
+                 numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
+                 batched reduced-precision form vs the plain FP64
                  one-launch-per-case path, time per step and delta_p agreement after --sweep-steps
   cpu_baseline   the reference program (built from the reference's sources by oracle/Makefile) on
                  this host's cores, over a bounded sample of the same workload
@@ -145,7 +146,7 @@ def bench_sweep(m, timer, stream, form, steps):
         ctx = m.Context(descs, precision=prec)
         ctx.set_stream(stream.cuda_stream)
         for i, (d, r) in enumerate(cases):
-            ctx.init_drop(r, width=-1.0, case=i)
+            ctx.init_drop(r, width=3.45, case=i)
         # untimed warm-up (first launches, tile-shape tuning), the same 5 steps on both paths
         if prec == 0:
             for i in range(len(descs)):
