@@ -268,23 +268,24 @@ long long gcdll(long long a, long long b)
 }
 
 // (fR, fB) [colour][19][S] FP64 at own site -> pairs (h_i, e_i) at ((y * 19 + i) nx + x) in the case's
-// storage type: e_i = (fR - fB) / (fR + fB), h_i = (f_i - w_i) / w_i - r(e_i) with the case's reference
-// density deviation r(e) = ref_mid + ref_half e.  The FP16 form keeps h as a half and e as the signed
+// storage type: e_i = (fR - fB) / (fR + fB), h_i = 2^k (f_i - w_i) - c_k r(e_i) with the case's reference
+// density deviation r(e) = ref_mid + ref_half e (lattice.h).  The FP16 form keeps h as a half and e as the signed
 // minority fraction copysign((1 - |e|) / 2, e) as a half (the device encoding of the same pair).
 int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
 {
     const int nx = k.d.nx;
-    const float rmid = (float)k.cc.ref_mid, rhalf = (float)k.cc.ref_half;
+    const LpRef rf = make_lp_ref(k.cc.ref_mid, k.cc.ref_half);
     std::vector<float> a(2 * (size_t)Q * k.n);
     for (long long p = 0; p < k.n; ++p) {
         const long long y = p / nx, x = p - y * nx;
         for (int i = 0; i < Q; ++i) {
+            const int cl = vclass(i);
             const double w = (i == 0) ? WL0 : (i < 7 ? WL1 : WL2);   // the weights the device decodes with
             const double r = h[(size_t)i * k.S + p], b = h[(size_t)(Q + i) * k.S + p];
             const double f = r + b;
             const size_t j = 2 * (size_t)((y * Q + i) * nx + x);
             const float e = (float)((f != 0.0) ? (r - b) / f : 0.0);
-            a[j] = (float)((f - w) / w) - (rmid + rhalf * e);
+            a[j] = (float)((f - w) * (double)SCL[cl]) - (rf.mid(cl) + rf.half(cl) * e);
             a[j + 1] = e;
         }
     }
@@ -801,7 +802,7 @@ int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
     CaseState& k = c->cases[id];
     if (k.elem == 8) return LBM_OK;   // FP64 stores the populations themselves
     const double mid = 0.5 * (rho_red + rho_blue) - 1.0, half = 0.5 * (rho_red - rho_blue);
-    HIPCHK(c, launch_lp_recentre(k.elem, k.pop[k.cur], (long long)Q * k.n, k.cc.ref_mid, k.cc.ref_half, mid, half,
+    HIPCHK(c, launch_lp_recentre(k.elem, k.pop[k.cur], (long long)Q * k.n, k.d.nx, k.cc.ref_mid, k.cc.ref_half, mid, half,
                                  c->stream));
     k.cc.ref_mid = mid;
     k.cc.ref_half = half;

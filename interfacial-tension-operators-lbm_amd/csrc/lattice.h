@@ -31,6 +31,32 @@ constexpr float WL0 = 0x1.555558p-2f;
 constexpr float WL1 = 0x1.c71c7p-5f;
 constexpr float WL2 = 0x1.c71c7p-6f;
 
+// Stored form of a population in the FP32 / FP16 forms.  The deviation s_i = f_i - w_i is kept
+// multiplied by a power of two near 1 / w_i (4, 16, 32 for the rest, axis and diagonal velocities), so
+// that encoding and decoding scale exactly, less the case's reference density in the same units:
+//   h_i = 2^k s_i - c_k r(e_i),   c_k = 2^k w_k,   r(e) = ref_mid + ref_half e.
+// The weights in those units are exact floats (c_0 = 4 WL0, c_1 = 16 WL1 = c_2 = 32 WL2).
+constexpr float SCL[3] = {4.f, 16.f, 32.f};
+constexpr float ISCL[3] = {0.25f, 0.0625f, 0.03125f};
+constexpr float CWL[3] = {WL0 * 4.f, WL1 * 16.f, WL2 * 32.f};
+static_assert(CWL[1] == CWL[2], "axis and diagonal classes share the weight unit");
+
+// velocity class of population i: 0 rest, 1 axis, 2 diagonal
+__host__ __device__ constexpr int vclass(int i) { return i == 0 ? 0 : (i < 7 ? 1 : 2); }
+
+// c_k r(e) = mid(k) + half(k) e, formed identically on the host and the device (c_1 = c_2, so the
+// axis and diagonal classes share one pair; scalar fields keep it in registers)
+struct LpRef {
+    float mid0, half0, mid1, half1;
+    __host__ __device__ float mid(int k) const { return k == 0 ? mid0 : mid1; }
+    __host__ __device__ float half(int k) const { return k == 0 ? half0 : half1; }
+};
+__host__ __device__ inline LpRef make_lp_ref(double ref_mid, double ref_half)
+{
+    const float m = (float)ref_mid, h = (float)ref_half;
+    return LpRef{CWL[0] * m, CWL[0] * h, CWL[1] * m, CWL[1] * h};
+}
+
 // operator ids of Operadores_tensao.cpp:45-49
 enum OpId { OP_P1 = 1, OP_P3 = 3, OP_F1 = 11, OP_F2 = 12, OP_F3 = 13 };
 

@@ -44,7 +44,7 @@ hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long
 hipError_t launch_lp_init_drop(int elem, void* pop, int nx, int ny, double R, double kp, int x0, int y0, double v0,
                                double rmid, double rhalf, hipStream_t stream);
 // in-place change of the reference density deviation r(e) = mid + half e of a reduced-precision state
-hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, double mid0, double half0, double mid1,
+hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, int nx, double mid0, double half0, double mid1,
                               double half1, hipStream_t stream);
 
 }  // namespace itolbm

@@ -40,10 +40,12 @@ template <typename T>
 using gst = __attribute__((address_space(1))) T;
 using gdouble = __attribute__((address_space(1))) double;
 
-// Stored pair of one population, (h_i, e_i) with h_i = A_i - r(e_i), where r(e) = ref_mid + ref_half e
-// is the case's reference density deviation (1 + r(+1) for pure red, 1 + r(-1) for pure blue, set
-// from measured phase densities; zero by default), so that the stored number is small in both bulk
-// phases and keeps its relative digits.
+// Stored pair of one population, (h_i, e_i) with h_i = 2^k s_i - c_k r(e_i) (lattice.h: s_i = f_i - w_i,
+// 2^k = 4 / 16 / 32 by velocity class, c_k = 2^k w_k), where r(e) = ref_mid + ref_half e is the case's
+// reference density deviation (1 + r(+1) for pure red, 1 + r(-1) for pure blue, set from measured
+// phase densities; zero by default), so that the stored number is small in both bulk phases and
+// keeps its relative digits.  The scaling by powers of two makes encoding and decoding exact apart
+// from the one rounding of each sum.
 //   FP32 form, 8 bytes: h and e as floats.
 //   FP16 form, 4 bytes: h as an IEEE half in the low 16 bits; in the high 16 bits the minority colour
 //   fraction m = (1 - |e|) / 2 as a half whose sign bit is that of e, e = copysign(1 - 2m, e).  A
@@ -116,79 +118,150 @@ __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restr
     }
 }
 
-struct Ref {
-    float mid, half;   // reference density deviation r(e) = mid + half e
-};
+using Ref = LpRef;
+
+// deviation s of a stored pair (a, e) of velocity class k
+__device__ __forceinline__ float decode_s(float a, float e, int k, const Ref& rf)
+{
+    return (a + (rf.mid(k) + rf.half(k) * e)) * ISCL[k];
+}
 
 template <typename ST>
-__device__ __forceinline__ void decode(const typename PairOf<ST>::raw r[Q], Row& row, Ref rf)
+__device__ __forceinline__ void decode(const typename PairOf<ST>::raw r[Q], Row& row, const Ref& rf)
 {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         row.e[i] = PairOf<ST>::e(r[i]);
-        row.s[i] = (PairOf<ST>::a(r[i]) + (rf.mid + rf.half * row.e[i])) * WF[i];
+        row.s[i] = decode_s(PairOf<ST>::a(r[i]), row.e[i], vclass(i), rf);
     }
 }
 
 template <typename ST>
-__device__ __forceinline__ typename PairOf<ST>::raw encode(int i, float s, float e, Ref rf)
+__device__ __forceinline__ typename PairOf<ST>::raw encode(int i, float s, float e, const Ref& rf)
 {
-    return PairOf<ST>::make(s / WF[i] - (rf.mid + rf.half * e), e);
+    const int k = vclass(i);
+    return PairOf<ST>::make(s * SCL[k] - (rf.mid(k) + rf.half(k) * e), e);
 }
 
-// density deviation, rho and phi of a site.  phi sums e_i f_i and f_i in the same order, so a pure
-// phase (all e_i = +-1) gives phi = +-1 exactly and no colour leaks into it; the density used by the
-// collision comes from the deviations, which keep more digits.
-__device__ __forceinline__ void row_phase(const Row& r, float& ds, float& rho, float& phi)
+// density deviation, rho and phi of a site.  phi = sum e_i f_i / sum f_i is formed in FP64 (the
+// products of two floats are exact there), so it carries the colour content of the stored row to
+// 1e-16: the recolouring hands that content on unchanged apart from the rounding of each stored e_i,
+// which is independent between populations.  An FP32 sum would change a site's colour by a rounding
+// that is the same every step in a steady state, i.e. a colour source at every interface site.  A pure
+// phase (all e_i = +-1) gives phi = +-1 exactly.  The density used by the collision comes from the
+// deviations, which keep more digits.
+__device__ __forceinline__ void row_phase(const Row& r, float& ds, float& rho, double& phi)
 {
-    float a = 0.f, D = 0.f, F = 0.f;
+    float a = 0.f;
+    double D = 0.0, F = 0.0;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const float f = r.s[i] + WF[i];
         a += r.s[i];
-        D += r.e[i] * f;
-        F += f;
+        D += (double)r.e[i] * (double)f;
+        F += (double)f;
     }
     ds = a;
     rho = 1.f + a;
     phi = D / F;
 }
 
+// colour field of a site (the gradients use it rounded to FP32)
 template <typename ST>
-__device__ __forceinline__ float raw_phase(const typename PairOf<ST>::raw r[Q], Ref rf)
+__device__ __forceinline__ double raw_phase(const typename PairOf<ST>::raw r[Q], const Ref& rf)
 {
     Row row;
     decode<ST>(r, row, rf);
-    float ds, rho, phi;
+    float ds, rho;
+    double phi;
     row_phase(row, ds, rho, phi);
     return phi;
 }
 
+// density deviation and rho of a decoded row (the summation of row_phase)
+__device__ __forceinline__ void row_density(const Row& r, float& ds, float& rho)
+{
+    float a = 0.f;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) a += r.s[i];
+    ds = a;
+    rho = 1.f + a;
+}
+
 __device__ __forceinline__ float shf(float v, int src) { return __shfl(v, src, 64); }
 
-// Colour field of the two halo sites whose 19 pulled pairs sit in lanes 0..18 and 19..37 of v: each
-// lane decodes its own pair as raw_phase does, then f_i and e_i f_i are summed over the site's lanes
-// in index order, so the result equals the owning lane's raw_phase bit for bit.
-template <typename ST>
-__device__ __forceinline__ void halo_phase(typename PairOf<ST>::raw v, Ref rf, float& left, float& right)
+// 64-bit value of the lane that DPP control CTRL selects (rows outside ROWS give 0)
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_value(double v)
 {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sums of v over lanes 0..31 and over lanes 32..63 in a fixed tree order (pairs, quads, half rows,
+// rows of 16, then row 0 into row 1 and row 2 into row 3); the totals end in lanes 31 and 63.
+__device__ __forceinline__ double half_wave_sum(double v)
+{
+    v += dpp_value<0xB1>(v);        // quad_perm [1, 0, 3, 2]
+    v += dpp_value<0x4E>(v);        // quad_perm [2, 3, 0, 1]
+    v += dpp_value<0x141>(v);       // row_half_mirror
+    v += dpp_value<0x140>(v);       // row_mirror
+    v += dpp_value<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    return v;
+}
+
+// Halo columns of an aligned strip in the pair layout: the 19 pulled values of a halo site sit in
+// lanes 0..18 (first site) or 32..50 (second site) of one wavefront load; lanes 19..31 and 51..63
+// re-read velocity 0 of their site (a valid address) and add nothing.
+struct HaloLaneLP {
+    int i, dy;
+    unsigned x;
+    bool on;
+};
+__device__ __forceinline__ HaloLaneLP halo_lane_lp(int nx, int col_first, int col_second)
+{
+    HaloLaneLP h;
     const int lane = threadIdx.x;
-    const int i = lane >= Q ? lane - Q : lane;
-    const float w = (i == 0) ? WF[0] : (i < 7 ? WF[1] : WF[7]);
+    const int site = lane >> 5, j = lane & 31;
+    h.on = j < Q;
+    h.i = h.on ? j : 0;
+    const int cx = (int)((CXP_MASK >> h.i) & 1u) - (int)((CXN_MASK >> h.i) & 1u);
+    const int cy = (int)((CYP_MASK >> h.i) & 1u) - (int)((CYN_MASK >> h.i) & 1u);
+    h.x = (unsigned)wrapi((site ? col_second : col_first) - cx, nx);
+    h.dy = -cy;
+    return h;
+}
+
+// Colour field of the two halo sites of one halo load: each lane decodes its pair as raw_phase does
+// and the site's f_i and e_i f_i are summed in FP64 over its lanes.  The sum order differs from the
+// owning lane's, but the terms are the same and the FP64 sums agree with it to ~1e-16, far below the
+// FP32 rounding of the result.
+template <typename ST>
+__device__ __forceinline__ void halo_pair_phase(typename PairOf<ST>::raw v, const Ref& rf, float& first,
+                                                float& second)
+{
+    const int j = threadIdx.x & 31;
+    const bool on = j < Q;
+    const int k = j == 0 ? 0 : (j < 7 ? 1 : 2);
+    const float w = k == 0 ? WL0 : (k == 1 ? WL1 : WL2);
+    const float mid = rf.mid(k), half = rf.half(k);
+    const float isc = k == 0 ? ISCL[0] : (k == 1 ? ISCL[1] : ISCL[2]);
     const float e = PairOf<ST>::e(v);
-    const float si = (PairOf<ST>::a(v) + (rf.mid + rf.half * e)) * w;
+    const float si = (PairOf<ST>::a(v) + (mid + half * e)) * isc;
     const float f = si + w;
-    const float ef = e * f;
-    float D0 = 0.f, F0 = 0.f, D1 = 0.f, F1 = 0.f;
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        D0 += lane_value(ef, k);
-        F0 += lane_value(f, k);
-        D1 += lane_value(ef, Q + k);
-        F1 += lane_value(f, Q + k);
-    }
-    left = D0 / F0;
-    right = D1 / F1;
+    const double D = half_wave_sum(on ? (double)e * (double)f : 0.0);
+    const double F = half_wave_sum(on ? (double)f : 0.0);
+    const float ph = (float)(D / F);   // meaningful in lanes 31 and 63
+    first = lane_value(ph, 31);
+    second = lane_value(ph, 63);
+}
+
+template <typename ST>
+__device__ __forceinline__ void halo_phase(typename PairOf<ST>::raw v, const Ref& rf, float& left, float& right)
+{
+    halo_pair_phase<ST>(v, rf, left, right);
 }
 
 struct Nine {
@@ -260,7 +333,7 @@ __device__ __forceinline__ ConstF to_float(const CaseConst& c)
 // Collision of one site given its pulled row, phi, the mediator gradient gm = -grad(phi) and the
 // force (class F); writes the post-collision s_i and e_i through out(i, s_i, e_i).
 template <bool FORCED, class Out>
-__device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float ds, float rho, float phi, float gx,
+__device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float ds, float rho, double phid, float gx,
                                            float gy, float Fx, float Fy, Out out)
 {
     // momentum (sum_i c_i w_i = 0, so it is the momentum of the deviations)
@@ -308,7 +381,9 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
             }
         }
     }
-    // recolouring: e_i = phi - 2 beta cR cB w_i rho cos_i / f_i
+    // recolouring: e_i = phi - 2 beta cR cB w_i rho cos_i / f_i, the difference taken in FP64 and
+    // rounded once
+    const float phi = (float)phid;
     const float cR = 0.5f * (1.f + phi), cB = 0.5f * (1.f - phi);
     const float kk = 2.f * k.beta * cR * cB * rho;
 #pragma unroll
@@ -318,7 +393,7 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
         if (i > 0 && has_g) {
             const float cn = (float)CX[i] * nx_ + (float)CY[i] * ny_;
             const float cosp = (i < 7) ? cn : cn * ISQ2;
-            e = (f != 0.f) ? phi - kk * WF[i] * cosp * __builtin_amdgcn_rcpf(f) : phi;
+            e = (f != 0.f) ? (float)(phid - (double)(kk * WF[i] * cosp * __builtin_amdgcn_rcpf(f))) : phi;
         }
         out(i, s[i], e);
     }
@@ -382,6 +457,8 @@ __device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
 }
 
 // The four halo columns of an aligned strip for the pair layout (Halo2): two loads per row.
+// Two-ring halo of an aligned strip (two-row stencils): load A holds columns c0-2 and c0-1, load B
+// columns c0+64 and c0+65, in the lane layout of halo_lane_lp.
 template <typename ST>
 struct Halo2LP {
     using R = typename PairOf<ST>::raw;
@@ -389,11 +466,12 @@ struct Halo2LP {
     int dy[2];
     __device__ __forceinline__ void init(int nx, int strip)
     {
-        const Halo2 h = make_halo2(nx, strip);
+        const int c0 = strip * 64;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            off[k] = (unsigned long long)h.i[k] * (unsigned)nx + h.x[k];
-            dy[k] = h.dy[k];
+        for (int s = 0; s < 2; ++s) {
+            const HaloLaneLP h = s ? halo_lane_lp(nx, c0 + 64, c0 + 65) : halo_lane_lp(nx, c0 - 2, c0 - 1);
+            off[s] = (unsigned long long)h.i * (unsigned)nx + h.x;
+            dy[s] = h.dy;
         }
     }
     __device__ __forceinline__ void load(gcst<R>* in, int nx, int ny, int y, R& a, R& b) const
@@ -404,37 +482,13 @@ struct Halo2LP {
     }
 };
 
-// Colour field of the four halo sites of a two-load halo: per-lane decode as raw_phase does, then
-// the site's f_i and e_i f_i summed in index order (bit-identical to the owning lane's raw_phase).
+// colour field of the four halo columns c0-2, c0-1, c0+64, c0+65
 template <typename ST>
-__device__ __forceinline__ void halo2_phase(typename PairOf<ST>::raw va, typename PairOf<ST>::raw vb, Ref rf,
+__device__ __forceinline__ void halo2_phase(typename PairOf<ST>::raw va, typename PairOf<ST>::raw vb, const Ref& rf,
                                             float (&hw)[4])
 {
-    const int lane = threadIdx.x;
-    float f[2], ef[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        int g = k * 64 + lane;
-        if (g >= 4 * Q) g = 4 * Q - 1;
-        const int i = g % Q;
-        const float w = (i == 0) ? WF[0] : (i < 7 ? WF[1] : WF[7]);
-        const auto v = k ? vb : va;
-        const float e = PairOf<ST>::e(v);
-        const float si = (PairOf<ST>::a(v) + (rf.mid + rf.half * e)) * w;
-        f[k] = si + w;
-        ef[k] = e * f[k];
-    }
-#pragma unroll
-    for (int site = 0; site < 4; ++site) {
-        float D = 0.f, F = 0.f;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const int g = Q * site + i;
-            D += (g < 64) ? lane_value(ef[0], g) : lane_value(ef[1], g - 64);
-            F += (g < 64) ? lane_value(f[0], g) : lane_value(f[1], g - 64);
-        }
-        hw[site] = D / F;
-    }
+    halo_pair_phase<ST>(va, rf, hw[0], hw[1]);
+    halo_pair_phase<ST>(vb, rf, hw[2], hw[3]);
 }
 
 // Aligned 64-column strip kernel for the two-row stencils (F1, F2): rows y-2 .. y+2 of the colour
@@ -451,7 +505,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
-    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
+    const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
@@ -466,7 +520,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
         R va, vb;
         h.load(in, nx, ny, y, va, vb);
         halo2_phase<ST>(va, vb, rf, hw);
-        return raw_phase<ST>(cur, rf);
+        return (float)raw_phase<ST>(cur, rf);
     };
     auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
         const float hl[3] = {HW[j][1], HW[j + 1][1], HW[j + 2][1]};
@@ -527,7 +581,8 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
         if (L.interior) {
             Row row;
             decode<ST>(cur, row, rf);
-            float ds, rho, phi;
+            float ds, rho;
+            double phi;
             row_phase(row, ds, rho, phi);
             gst<R>* orow = out + (long long)y * ((long long)Q * nx);
             const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
@@ -562,7 +617,7 @@ struct HaloLP {
     bool on;
     __device__ __forceinline__ void init(int nx, int strip)
     {
-        const Halo1 h = make_halo1(nx, strip);
+        const HaloLaneLP h = halo_lane_lp(nx, strip * 64 - 1, strip * 64 + 64);
         off = (unsigned long long)h.i * (unsigned)nx + h.x;
         dy = h.dy;
         on = h.on;
@@ -572,7 +627,7 @@ struct HaloLP {
         int yy = y + dy;
         yy = (yy < 0) ? yy + ny : yy;
         yy = (yy >= ny) ? yy - ny : yy;
-        // every lane loads (lanes >= 38 re-read an in-range value that no one uses): no branch
+        // every lane loads (the unused lanes re-read an in-range value): no branch
         return in[(unsigned long long)yy * ((unsigned long long)Q * (unsigned)nx) + off];
     }
 };
@@ -583,7 +638,7 @@ template <typename ST, int KIND>
 __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
                                             gst<typename PairOf<ST>::raw>* out, int y,
                                             const typename PairOf<ST>::raw (&cur)[Q], const float (&p)[3],
-                                            const float (&hl)[3], const float (&hr)[3])
+                                            const float (&hl)[3], const float (&hr)[3], double phi)
 {
     using R = typename PairOf<ST>::raw;
     const int nx = t.cc.nx;
@@ -599,8 +654,8 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
     if (L.interior) {
         Row row;
         decode<ST>(cur, row, rf);
-        float ds, rho, phi;
-        row_phase(row, ds, rho, phi);
+        float ds, rho;
+        row_density(row, ds, rho);
         gst<R>* orow = out + (long long)y * ((long long)Q * nx);
         const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
         auto st = [&](int i, float s, float e) { put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf)); };
@@ -625,7 +680,7 @@ __global__ void __launch_bounds__(64) lp_row1(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
-    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
+    const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
@@ -637,11 +692,12 @@ __global__ void __launch_bounds__(64) lp_row1(StepLaunch launch)
     float p[3], hl[3], hr[3];
     load_raw<ST>(in, nx, ny, y0 - 1, L, nxt);
     R hv = h.load(in, nx, ny, y0 - 1);
-    p[1] = raw_phase<ST>(nxt, rf);
+    p[1] = (float)raw_phase<ST>(nxt, rf);
     halo_phase<ST>(hv, rf, hl[1], hr[1]);
     load_raw<ST>(in, nx, ny, y0, L, cur);
     hv = h.load(in, nx, ny, y0);
-    p[2] = raw_phase<ST>(cur, rf);
+    double phn = raw_phase<ST>(cur, rf);   // phi of the next row to collide
+    p[2] = (float)phn;
     halo_phase<ST>(hv, rf, hl[2], hr[2]);
     for (int y = y0; y < y1; ++y) {
         load_raw<ST>(in, nx, ny, y + 1, L, nxt);
@@ -649,9 +705,11 @@ __global__ void __launch_bounds__(64) lp_row1(StepLaunch launch)
         p[0] = p[1]; p[1] = p[2];
         hl[0] = hl[1]; hl[1] = hl[2];
         hr[0] = hr[1]; hr[1] = hr[2];
-        p[2] = raw_phase<ST>(nxt, rf);
+        const double phc = phn;
+        phn = raw_phase<ST>(nxt, rf);
+        p[2] = (float)phn;
         halo_phase<ST>(hv, rf, hl[2], hr[2]);
-        collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr);
+        collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr, phc);
 #pragma unroll
         for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
     }
@@ -674,7 +732,7 @@ __device__ __forceinline__ void pipe_row(const TileCtxLP& t, const ConstF& k, co
                                          gst<typename PairOf<ST>::raw>* out, int y, typename PairOf<ST>::raw (&cur)[Q],
                                          typename PairOf<ST>::raw (&nxt)[Q], typename PairOf<ST>::raw (&nn)[Q],
                                          typename PairOf<ST>::raw& hnxt, typename PairOf<ST>::raw& hnn, float (&p)[3],
-                                         float (&hl)[3], float (&hr)[3])
+                                         float (&hl)[3], float (&hr)[3], double& phn)
 {
     const int nx = t.cc.nx, ny = t.cc.ny;
     if (y + 2 <= t.y1) {   // rows up to y1 feed the gradients
@@ -684,11 +742,13 @@ __device__ __forceinline__ void pipe_row(const TileCtxLP& t, const ConstF& k, co
     p[0] = p[1]; p[1] = p[2];
     hl[0] = hl[1]; hl[1] = hl[2];
     hr[0] = hr[1]; hr[1] = hr[2];
-    p[2] = raw_phase<ST>(nxt, rf);   // row y + 1 (waits for it)
+    const double phc = phn;
+    phn = raw_phase<ST>(nxt, rf);   // row y + 1 (waits for it)
+    p[2] = (float)phn;
     halo_phase<ST>(hnxt, rf, hl[2], hr[2]);
 #pragma unroll
     for (int i = 0; i < Q; ++i) opaque(cur[i]);   // re-decode rather than keep decoded values alive
-    collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr);
+    collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr, phc);
 }
 
 template <typename ST, int KIND>
@@ -700,7 +760,7 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
-    const Ref rf{(float)t.cc.ref_mid, (float)t.cc.ref_half};
+    const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
@@ -717,16 +777,17 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     ha = h.load(in, nx, ny, y0);
     load_raw<ST>(in, nx, ny, y0 + 1, L, rb);
     hb = h.load(in, nx, ny, y0 + 1);
-    p[1] = raw_phase<ST>(rc, rf);
+    p[1] = (float)raw_phase<ST>(rc, rf);
     halo_phase<ST>(hc, rf, hl[1], hr[1]);
-    p[2] = raw_phase<ST>(ra, rf);
+    double phn = raw_phase<ST>(ra, rf);   // phi of the next row to collide
+    p[2] = (float)phn;
     halo_phase<ST>(ha, rf, hl[2], hr[2]);
     for (int y = y0; y < y1; y += 3) {
-        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y, ra, rb, rc, hb, hc, p, hl, hr);
+        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y, ra, rb, rc, hb, hc, p, hl, hr, phn);
         if (y + 1 >= y1) break;
-        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 1, rb, rc, ra, hc, ha, p, hl, hr);
+        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 1, rb, rc, ra, hc, ha, p, hl, hr, phn);
         if (y + 2 >= y1) break;
-        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 2, rc, ra, rb, ha, hb, p, hl, hr);
+        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 2, rc, ra, rb, ha, hb, p, hl, hr, phn);
     }
 }
 
@@ -758,7 +819,8 @@ hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t s
 }
 
 // ------------------------------------------------------------------ conversions for transfers / diagnostics
-// (A, e) pairs in rows -> fR, fB in FP64, [19][S] each (own site, post-collision)
+// (h, e) pairs in rows -> fR, fB in FP64, [19][S] each (own site, post-collision); s is decoded as
+// the step kernels decode it, f = w + s in FP64
 template <typename ST>
 __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw* __restrict__ pop, int nx, long long S,
                                                  long long n, Ref rf, double* __restrict__ fR, double* __restrict__ fB)
@@ -768,27 +830,29 @@ __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw*
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const auto r = pop[(y * Q + i) * nx + x];
-            const double w = (double)WF[i];
             const float e = PairOf<ST>::e(r);
-            const float A = PairOf<ST>::a(r) + (rf.mid + rf.half * e);
-            const double f = w * (1.0 + (double)A);
+            const float s = decode_s(PairOf<ST>::a(r), e, vclass(i), rf);
+            const double f = (double)WF[i] + (double)s;
             fR[i * S + p] = 0.5 * f * (1.0 + (double)e);
             fB[i * S + p] = 0.5 * f * (1.0 - (double)e);
         }
     }
 }
 
-// Re-express a stored state against a new reference density, in place: h' = h + r_old(e) - r_new(e).
+// Re-express a stored state against a new reference density, in place:
+// h' = h + c_k r_old(e) - c_k r_new(e).  Pair p is velocity (p / nx) mod 19 of its row block.
 template <typename ST>
 __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __restrict__ pop, long long npairs,
-                                                   Ref from, Ref to)
+                                                   int nx, Ref from, Ref to)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs;
          p += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)((p / nx) % Q);
+        const int k = vclass(i);
         const auto r = pop[p];
         const float e = PairOf<ST>::e(r);
-        const float A = PairOf<ST>::a(r) + (from.mid + from.half * e);
-        pop[p] = PairOf<ST>::make(A - (to.mid + to.half * e), e);
+        const float A = PairOf<ST>::a(r) + (from.mid(k) + from.half(k) * e);
+        pop[p] = PairOf<ST>::make(A - (to.mid(k) + to.half(k) * e), e);
     }
 }
 
@@ -820,7 +884,8 @@ __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __
             const double g = w * (1.0 + 3.0 * cv + 4.5 * cv * cv - 1.5 * v0 * v0);
             const double fR = g * phi, fB = g * (1.0 - phi), f = fR + fB;
             const float e = (float)((f != 0.0) ? (fR - fB) / f : 0.0);
-            const float h = (float)((f - w) / w) - (rf.mid + rf.half * e);
+            const int k = vclass(i);
+            const float h = (float)((f - w) * (double)SCL[k]) - (rf.mid(k) + rf.half(k) * e);
             pop[((long long)y * Q + i) * nx + x] = PairOf<ST>::make(h, e);
         }
     }
@@ -840,7 +905,7 @@ hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long
                             double* fR, double* fB, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
-    const Ref rf{(float)rmid, (float)rhalf};
+    const Ref rf = make_lp_ref(rmid, rhalf);
     if (elem == 4)
         hipLaunchKernelGGL((lp_to_f64<float>), dim3(nb), dim3(256), 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
     else
@@ -849,15 +914,16 @@ hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long
     return hipGetLastError();
 }
 
-hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, double mid0, double half0, double mid1,
+hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, int nx, double mid0, double half0, double mid1,
                               double half1, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((npairs + 255) / 256, 8192);
-    const Ref from{(float)mid0, (float)half0}, to{(float)mid1, (float)half1};
+    const Ref from = make_lp_ref(mid0, half0), to = make_lp_ref(mid1, half1);
     if (elem == 4)
-        hipLaunchKernelGGL((lp_recentre<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, npairs, from, to);
+        hipLaunchKernelGGL((lp_recentre<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, npairs, nx, from, to);
     else
-        hipLaunchKernelGGL((lp_recentre<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, npairs, from, to);
+        hipLaunchKernelGGL((lp_recentre<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, npairs, nx, from,
+                           to);
     return hipGetLastError();
 }
 
@@ -866,7 +932,7 @@ hipError_t launch_lp_init_drop(int elem, void* pop, int nx, int ny, double R, do
 {
     const long long n = (long long)nx * ny;
     const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
-    const Ref rf{(float)rmid, (float)rhalf};
+    const Ref rf = make_lp_ref(rmid, rhalf);
     if (elem == 4)
         hipLaunchKernelGGL((lp_init_drop<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0,
                            v0, rf);
