@@ -285,7 +285,7 @@ int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
             const double f = r + b;
             const size_t j = 2 * (size_t)((y * Q + i) * nx + x);
             const float e = (float)((f != 0.0) ? (r - b) / f : 0.0);
-            a[j] = (float)((f - w) * (double)SCL[cl]) - (rf.mid(cl) + rf.half(cl) * e);
+            a[j] = (float)((f - w) * (double)SCL[cl]) - std::fma(rf.half(cl), e, rf.mid(cl));   // as the device
             a[j + 1] = e;
         }
     }
