@@ -60,20 +60,21 @@ template <> struct PairOf<float> {
     __device__ static __forceinline__ float e(raw r) { return r.y; }
     __device__ static __forceinline__ raw make(float a, float e) { return raw{a, e}; }
 };
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 template <> struct PairOf<_Float16> {
     using raw = unsigned int;
-    __device__ static __forceinline__ float a(raw r) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(r & 0xffffu)); }
+    __device__ static __forceinline__ float a(raw r) { return (float)__builtin_bit_cast(f16x2, r).x; }
+    // 1 - 2|m| >= 0, so the sign bit of the pair (that of m) is or-ed in
     __device__ static __forceinline__ float e(raw r)
     {
-        const float m = (float)__builtin_bit_cast(_Float16, (unsigned short)(r >> 16));
-        return __builtin_copysignf(1.0f - 2.0f * __builtin_fabsf(m), m);
+        const float t = __builtin_fmaf(-2.0f, __builtin_fabsf((float)__builtin_bit_cast(f16x2, r).y), 1.0f);
+        return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, t) | (r & 0x80000000u));
     }
     __device__ static __forceinline__ raw make(float a, float e)
     {
-        const unsigned lo = __builtin_bit_cast(unsigned short, (_Float16)a);
-        const float m = __builtin_fmaxf(0.5f * (1.0f - __builtin_fabsf(e)), 0.0f);
-        const unsigned hi = __builtin_bit_cast(unsigned short, (_Float16)__builtin_copysignf(m, e));
-        return lo | (hi << 16);
+        const float m = __builtin_fmaxf(__builtin_fmaf(-0.5f, __builtin_fabsf(e), 0.5f), 0.0f);
+        const float sm = __builtin_bit_cast(float, __builtin_bit_cast(unsigned, m) | (__builtin_bit_cast(unsigned, e) & 0x80000000u));
+        return __builtin_bit_cast(raw, f16x2{(_Float16)a, (_Float16)sm});
     }
 };
 
@@ -83,18 +84,40 @@ struct Row {
     float e[Q];   // colour fraction
 };
 
-// Element at a 32-bit byte offset from a uniform base: lets the compiler address with the base in
-// SGPRs and one 32-bit VGPR offset (no 64-bit per-lane address arithmetic).  A row of one case is
-// < 4 GiB, so the offset fits.
+// Buffer access to one row block of a case (the 19 velocity rows of one y): a resource holding the
+// block's base in scalar registers; an access gives the lane's byte offset (VGPR) and the velocity
+// row's byte offset (SGPR), so no per-lane 64-bit address arithmetic is needed.  The resource's
+// extent is the block, so an access outside it could not reach other memory.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int RSRC_WORD3 = 0x00020000;   // raw 32-bit data format (CDNA)
+
 template <typename T>
-__device__ __forceinline__ T at_byte(gcst<T>* base, unsigned off)
+__device__ __forceinline__ rsrc_t block_rsrc(__attribute__((address_space(1))) const T* base, unsigned bytes)
 {
-    return *(gcst<T>*)((__attribute__((address_space(1))) const char*)base + off);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(unsigned long long)base, (short)0, (int)bytes, RSRC_WORD3);
 }
-template <typename T>
-__device__ __forceinline__ void put_byte(gst<T>* base, unsigned off, T v)
+
+template <typename R> __device__ R buf_load(rsrc_t r, unsigned voff, unsigned soff);
+template <> __device__ __forceinline__ f32x2 buf_load<f32x2>(rsrc_t r, unsigned voff, unsigned soff)
 {
-    __builtin_nontemporal_store(v, (gst<T>*)((__attribute__((address_space(1))) char*)base + off));
+    return __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <> __device__ __forceinline__ unsigned buf_load<unsigned>(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+
+// nontemporal stores (aux bit 1): the populations are not read again by this launch
+constexpr int AUX_NT = 2;
+template <typename R> __device__ void buf_store(R v, rsrc_t r, unsigned voff, unsigned soff);
+template <> __device__ __forceinline__ void buf_store<f32x2>(f32x2 v, rsrc_t r, unsigned voff, unsigned soff)
+{
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX_NT);
+}
+template <> __device__ __forceinline__ void buf_store<unsigned>(unsigned v, rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, AUX_NT);
 }
 
 // Issue the pulls of site (col, y): raw pair i from row y - c_iy, column col - c_ix.
@@ -104,26 +127,28 @@ __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restr
 {
     using R = typename PairOf<ST>::raw;
     const long long pitch = (long long)Q * nx;
-    gcst<R>* rowm = in + wrapi(y - 1, ny) * pitch;
-    gcst<R>* row0 = in + wrapi(y, ny) * pitch;
-    gcst<R>* rowp = in + wrapi(y + 1, ny) * pitch;
-    // per-velocity bases are uniform (SGPRs); the lane part is one of three byte offsets
+    const unsigned bytes = (unsigned)(pitch * (long long)sizeof(R));
+    const rsrc_t rowm = block_rsrc(in + wrapi(y - 1, ny) * pitch, bytes);
+    const rsrc_t row0 = block_rsrc(in + wrapi(y, ny) * pitch, bytes);
+    const rsrc_t rowp = block_rsrc(in + wrapi(y + 1, ny) * pitch, bytes);
+    // the lane part is one of three byte offsets; the velocity row's offset is uniform
     const unsigned om = (unsigned)L.xm * (unsigned)sizeof(R), o0 = (unsigned)L.col * (unsigned)sizeof(R),
                    op = (unsigned)L.xp * (unsigned)sizeof(R);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : o0);
-        gcst<R>* base = (CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0)) + i * nx;
-        r[i] = at_byte(base, xs);
+        r[i] = buf_load<R>(CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0), xs, (unsigned)(i * nx * (int)sizeof(R)));
     }
 }
 
 using Ref = LpRef;
 
-// deviation s of a stored pair (a, e) of velocity class k
+// deviation s of a stored pair (a, e) of velocity class k: (a + c_k r(e)) 2^-k, written as
+// a 2^-k + (c_k r(e)) 2^-k, which rounds identically because the scalings are exact
 __device__ __forceinline__ float decode_s(float a, float e, int k, const Ref& rf)
 {
-    return (a + (rf.mid(k) + rf.half(k) * e)) * ISCL[k];
+    const float mid = rf.mid(k) * ISCL[k], half = rf.half(k) * ISCL[k];
+    return __builtin_fmaf(a, ISCL[k], __builtin_fmaf(half, e, mid));
 }
 
 template <typename ST>
@@ -140,7 +165,7 @@ template <typename ST>
 __device__ __forceinline__ typename PairOf<ST>::raw encode(int i, float s, float e, const Ref& rf)
 {
     const int k = vclass(i);
-    return PairOf<ST>::make(s * SCL[k] - (rf.mid(k) + rf.half(k) * e), e);
+    return PairOf<ST>::make(__builtin_fmaf(s, SCL[k], -__builtin_fmaf(rf.half(k), e, rf.mid(k))), e);
 }
 
 // density deviation, rho and phi of a site.  phi = sum e_i f_i / sum f_i is formed in FP64 (the
@@ -246,10 +271,10 @@ __device__ __forceinline__ void halo_pair_phase(typename PairOf<ST>::raw v, cons
     const bool on = j < Q;
     const int k = j == 0 ? 0 : (j < 7 ? 1 : 2);
     const float w = k == 0 ? WL0 : (k == 1 ? WL1 : WL2);
-    const float mid = rf.mid(k), half = rf.half(k);
     const float isc = k == 0 ? ISCL[0] : (k == 1 ? ISCL[1] : ISCL[2]);
+    const float mid = rf.mid(k) * isc, half = rf.half(k) * isc;
     const float e = PairOf<ST>::e(v);
-    const float si = (PairOf<ST>::a(v) + (mid + half * e)) * isc;
+    const float si = __builtin_fmaf(PairOf<ST>::a(v), isc, __builtin_fmaf(half, e, mid));
     const float f = si + w;
     const double D = half_wave_sum(on ? (double)e * (double)f : 0.0);
     const double F = half_wave_sum(on ? (double)f : 0.0);
@@ -388,12 +413,13 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
     const float kk = 2.f * k.beta * cR * cB * rho;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const float f = s[i] + WF[i];
         float e = phi;
-        if (i > 0 && has_g) {
+        if (i > 0) {   // without a gradient n = 0, so the correction is 0 (no branch per lane)
+            const float f = s[i] + WF[i];
             const float cn = (float)CX[i] * nx_ + (float)CY[i] * ny_;
             const float cosp = (i < 7) ? cn : cn * ISQ2;
-            e = (f != 0.f) ? (float)(phid - (double)(kk * WF[i] * cosp * __builtin_amdgcn_rcpf(f))) : phi;
+            const float corr = kk * WF[i] * cosp * __builtin_amdgcn_rcpf(f);
+            e = (float)(phid - (double)((f != 0.f) ? corr : 0.f));
         }
         out(i, s[i], e);
     }
@@ -584,9 +610,9 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             float ds, rho;
             double phi;
             row_phase(row, ds, rho, phi);
-            gst<R>* orow = out + (long long)y * ((long long)Q * nx);
+            const rsrc_t orow = block_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
             const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-            auto st = [&](int i, float s, float e) { put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf)); };
+            auto st = [&](int i, float s, float e) { buf_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
             if (rho > 1.0e-30f) {
                 collide_lp<true>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
             } else {
@@ -656,9 +682,9 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
         decode<ST>(cur, row, rf);
         float ds, rho;
         row_density(row, ds, rho);
-        gst<R>* orow = out + (long long)y * ((long long)Q * nx);
+        const rsrc_t orow = block_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
         const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-        auto st = [&](int i, float s, float e) { put_byte(orow + i * nx, ocol, encode<ST>(i, s, e, rf)); };
+        auto st = [&](int i, float s, float e) { buf_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
         if (rho > 1.0e-30f) {
             collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
         } else {
@@ -851,8 +877,8 @@ __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __r
         const int k = vclass(i);
         const auto r = pop[p];
         const float e = PairOf<ST>::e(r);
-        const float A = PairOf<ST>::a(r) + (from.mid(k) + from.half(k) * e);
-        pop[p] = PairOf<ST>::make(A - (to.mid(k) + to.half(k) * e), e);
+        const float A = PairOf<ST>::a(r) + __builtin_fmaf(from.half(k), e, from.mid(k));
+        pop[p] = PairOf<ST>::make(A - __builtin_fmaf(to.half(k), e, to.mid(k)), e);
     }
 }
 
@@ -885,7 +911,7 @@ __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __
             const double fR = g * phi, fB = g * (1.0 - phi), f = fR + fB;
             const float e = (float)((f != 0.0) ? (fR - fB) / f : 0.0);
             const int k = vclass(i);
-            const float h = (float)((f - w) * (double)SCL[k]) - (rf.mid(k) + rf.half(k) * e);
+            const float h = (float)((f - w) * (double)SCL[k]) - __builtin_fmaf(rf.half(k), e, rf.mid(k));
             pop[((long long)y * Q + i) * nx + x] = PairOf<ST>::make(h, e);
         }
     }
