@@ -78,47 +78,25 @@ template <> struct PairOf<_Float16> {
     }
 };
 
+// one stored pair through a buffer resource (8-byte FP32 pairs, 4-byte FP16 pairs)
+template <typename R>
+__device__ __forceinline__ R pair_load(rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(R) == 8) return buf_load64<R>(r, voff, soff);
+    else return buf_load<unsigned>(r, voff, soff);
+}
+template <typename R>
+__device__ __forceinline__ void pair_store(R v, rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(R) == 8) buf_store64(v, r, voff, soff);
+    else buf_store<unsigned>(v, r, voff, soff);
+}
+
 // pulled populations of one site in deviation form
 struct Row {
     float s[Q];   // s_i = f_i - w_i
     float e[Q];   // colour fraction
 };
-
-// Buffer access to one row block of a case (the 19 velocity rows of one y): a resource holding the
-// block's base in scalar registers; an access gives the lane's byte offset (VGPR) and the velocity
-// row's byte offset (SGPR), so no per-lane 64-bit address arithmetic is needed.  The resource's
-// extent is the block, so an access outside it could not reach other memory.
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-constexpr int RSRC_WORD3 = 0x00020000;   // raw 32-bit data format (CDNA)
-
-template <typename T>
-__device__ __forceinline__ rsrc_t block_rsrc(__attribute__((address_space(1))) const T* base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc((void*)(unsigned long long)base, (short)0, (int)bytes, RSRC_WORD3);
-}
-
-template <typename R> __device__ R buf_load(rsrc_t r, unsigned voff, unsigned soff);
-template <> __device__ __forceinline__ f32x2 buf_load<f32x2>(rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-template <> __device__ __forceinline__ unsigned buf_load<unsigned>(rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-}
-
-// nontemporal stores (aux bit 1): the populations are not read again by this launch
-constexpr int AUX_NT = 2;
-template <typename R> __device__ void buf_store(R v, rsrc_t r, unsigned voff, unsigned soff);
-template <> __device__ __forceinline__ void buf_store<f32x2>(f32x2 v, rsrc_t r, unsigned voff, unsigned soff)
-{
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX_NT);
-}
-template <> __device__ __forceinline__ void buf_store<unsigned>(unsigned v, rsrc_t r, unsigned voff, unsigned soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, AUX_NT);
-}
 
 // Issue the pulls of site (col, y): raw pair i from row y - c_iy, column col - c_ix.
 template <typename ST>
@@ -128,16 +106,16 @@ __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restr
     using R = typename PairOf<ST>::raw;
     const long long pitch = (long long)Q * nx;
     const unsigned bytes = (unsigned)(pitch * (long long)sizeof(R));
-    const rsrc_t rowm = block_rsrc(in + wrapi(y - 1, ny) * pitch, bytes);
-    const rsrc_t row0 = block_rsrc(in + wrapi(y, ny) * pitch, bytes);
-    const rsrc_t rowp = block_rsrc(in + wrapi(y + 1, ny) * pitch, bytes);
+    const rsrc_t rowm = make_rsrc(in + wrapi(y - 1, ny) * pitch, bytes);
+    const rsrc_t row0 = make_rsrc(in + wrapi(y, ny) * pitch, bytes);
+    const rsrc_t rowp = make_rsrc(in + wrapi(y + 1, ny) * pitch, bytes);
     // the lane part is one of three byte offsets; the velocity row's offset is uniform
     const unsigned om = (unsigned)L.xm * (unsigned)sizeof(R), o0 = (unsigned)L.col * (unsigned)sizeof(R),
                    op = (unsigned)L.xp * (unsigned)sizeof(R);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : o0);
-        r[i] = buf_load<R>(CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0), xs, (unsigned)(i * nx * (int)sizeof(R)));
+        r[i] = pair_load<R>(CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0), xs, (unsigned)(i * nx * (int)sizeof(R)));
     }
 }
 
@@ -610,9 +588,9 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             float ds, rho;
             double phi;
             row_phase(row, ds, rho, phi);
-            const rsrc_t orow = block_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
+            const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
             const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-            auto st = [&](int i, float s, float e) { buf_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
+            auto st = [&](int i, float s, float e) { pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
             if (rho > 1.0e-30f) {
                 collide_lp<true>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
             } else {
@@ -682,9 +660,9 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
         decode<ST>(cur, row, rf);
         float ds, rho;
         row_density(row, ds, rho);
-        const rsrc_t orow = block_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
+        const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
         const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-        auto st = [&](int i, float s, float e) { buf_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
+        auto st = [&](int i, float s, float e) { pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
         if (rho > 1.0e-30f) {
             collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
         } else {

@@ -123,6 +123,51 @@ __device__ __forceinline__ int wrap_row(int y, int ny)
     return (y >= ny) ? y - ny : y;
 }
 
+// ------------------------------------------------------------------ buffer-resource access
+// A buffer resource holds a uniform base in scalar registers; an access gives the lane's byte offset
+// (VGPR) and a uniform byte offset (SGPR), so no per-lane 64-bit address arithmetic is needed.  The
+// resource's extent bounds the accesses: one outside it returns zero / is dropped and cannot reach
+// other memory.  Offsets are 32-bit: one resource spans at most 4 GiB.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int RSRC_WORD3 = 0x00020000;   // raw 32-bit data format (CDNA)
+constexpr int AUX_NT = 2;                // nontemporal store: not read again by the launch
+
+template <typename T>
+__device__ __forceinline__ rsrc_t make_rsrc(__attribute__((address_space(1))) const T* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(unsigned long long)base, (short)0, (int)bytes, RSRC_WORD3);
+}
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+template <typename R> __device__ R buf_load(rsrc_t r, unsigned voff, unsigned soff);
+template <typename R> __device__ void buf_store(R v, rsrc_t r, unsigned voff, unsigned soff);
+template <typename R>
+__device__ __forceinline__ R buf_load64(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <typename R>
+__device__ __forceinline__ void buf_store64(R v, rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, AUX_NT);
+}
+template <> __device__ __forceinline__ double buf_load<double>(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return buf_load64<double>(r, voff, soff);
+}
+template <> __device__ __forceinline__ void buf_store<double>(double v, rsrc_t r, unsigned voff, unsigned soff)
+{
+    buf_store64(v, r, voff, soff);
+}
+template <> __device__ __forceinline__ unsigned buf_load<unsigned>(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+template <> __device__ __forceinline__ void buf_store<unsigned>(unsigned v, rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, AUX_NT);
+}
+
 // Value of lane `src` (a compile-time constant after unrolling) in every lane.
 __device__ __forceinline__ float lane_value(float v, int src)
 {
