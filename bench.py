@@ -14,7 +14,9 @@ storage forms are timed the same way and reported beside it.  Also reported:
   roofline       the dominant kernel (the P1/P3 family launch, 8 of the 14 cases) timed alone with HIP
                  events on its stream: algorithmic bytes per launch / average launch duration;
                  traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-                 (profiles/traffic_<form>.json), when present
+                 (profiles/traffic_<form>.json), when present; copy_GBs = a plain 1 GiB device copy
+                 (torch's copy kernel) timed the same way, the bandwidth a streaming kernel reaches
+                 on this GPU, and frac_of_copy = achieved / copy_GBs
   large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in the headline form
   sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
                  numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
@@ -119,6 +121,24 @@ def bench_large(m, timer, stream, form, steps):
     return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
             "steps": steps, "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
             "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS}
+
+
+def copy_ceiling(torch, timer, stream, reps=10):
+    """Plain device copy of 1 GiB (torch's copy kernel) on the same stream: read + write bandwidth a
+    simple streaming kernel reaches on this GPU, beside the nominal peak."""
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            y.copy_(x)
+
+        def run():
+            for _ in range(reps):
+                y.copy_(x)
+        gpu_s, _ = timer.run(run)
+    nbytes = 2 * x.numel() * x.element_size()
+    del x, y
+    return {"bytes_per_copy": nbytes, "GBs": nbytes * reps / gpu_s / 1e9}
 
 
 def sweep_cases(m):
@@ -227,6 +247,8 @@ def main():
     ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 case")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only the 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
@@ -238,6 +260,9 @@ def main():
     m = _pkg.load()
     stream = torch.cuda.Stream()
     timer = Timer(torch, stream)
+    if a.copy_only:
+        print(json.dumps({"copy": copy_ceiling(torch, timer, stream, reps=3)}), flush=True)
+        return
 
     forms = {}
     for form in [a.prec] + [f for f in a.forms.split(",") if f and f != a.prec]:
@@ -251,6 +276,8 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
+    say("bench: 1 GiB device copy")
+    copy = copy_ceiling(torch, timer, stream)
     out = {
         "metric": "MLUPS + % HBM roofline, 1 GPU; sweep cases/hour vs plain FP64 at same Δp error",
         "value": head["MLUPS"],
@@ -272,7 +299,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": head["dominant_kernel"], "achieved": head["achieved_GBs"],
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": head["frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": head["bytes_per_launch"],
-                     "avg_launch_us": head["kernel_avg_launch_us"]},
+                     "avg_launch_us": head["kernel_avg_launch_us"],
+                     "copy_GBs": copy["GBs"], "frac_of_copy": head["achieved_GBs"] / copy["GBs"]},
         "forms": forms,
     }
     if a.large_steps > 0:
