@@ -144,6 +144,13 @@ int lbm_sync(lbm_ctx* ctx);
  * reference counterpart: it lets a caller order the steps with its own work and time them. */
 int lbm_set_stream(lbm_ctx* ctx, void* hip_stream);
 
+/* lbm_step queues whole runs of steps_per_graph steps (even; default 32) as one captured HIP graph,
+ * replayed with a single launch, and the remaining steps launch by launch; 0 turns graphs off.  The
+ * results are the same either way.  The graph is captured on first use and again after the set of
+ * active cases, a tile shape or the parity of a family changes.  No reference counterpart (it cuts
+ * the per-step launch cost of small cases). */
+int lbm_set_graph(lbm_ctx* ctx, int steps_per_graph);
+
 /* Includes (active != 0, the initial state) or leaves out (0) a case from lbm_step, e.g. once a batch
  * member has finished or become unstable; lbm_step_case still advances it.  No reference
  * counterpart (the reference runs one case per process). */

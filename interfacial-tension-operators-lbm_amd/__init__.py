@@ -25,7 +25,7 @@ STEP_RECORD_AUX = 1
 
 # every symbol include/itolbm.h declares
 SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "lbm_upload_state",
-           "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active",
+           "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
            "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl"]
 
 
@@ -89,6 +89,7 @@ def lib():
         L.lbm_set_stream.argtypes = [P, C.c_void_p]
         L.lbm_set_reference.argtypes = [P, C.c_int, C.c_double, C.c_double]
         L.lbm_set_active.argtypes = [P, C.c_int, C.c_int]
+        L.lbm_set_graph.argtypes = [P, C.c_int]
         L.lbm_download_aux.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
         L.lbm_diag_drop.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(DropDiag)]
         L.lbm_diag_incl.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(InclDiag), C.c_void_p]
@@ -193,6 +194,10 @@ class Context:
 
     def sync(self):
         self._chk(lib().lbm_sync(self.h))
+
+    def set_graph(self, steps_per_graph):
+        """Steps per captured HIP graph in step() (even; 0 = launch by launch)."""
+        self._chk(lib().lbm_set_graph(self.h, int(steps_per_graph)))
 
     def set_active(self, active, case=0):
         """Includes or leaves out a case from step() over all cases."""

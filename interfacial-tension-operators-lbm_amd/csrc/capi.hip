@@ -66,6 +66,17 @@ struct Group {
     int cap = 0;                   // entries dtab can hold
     bool written = false;
     bool tuned = false;            // rows per segment chosen by timing (group_tune)
+    long long version = 0;         // bumped whenever the table is rewritten
+};
+
+// A captured sequence of `steps` steps of all active families (lbm_step), replayed with one
+// hipGraphLaunch.  It is valid while every family it covers still has the same table (version,
+// tile count) and parity; steps is even, so a replay returns every family to its parity.
+struct StepGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int steps = 0;
+    std::vector<long long> key;
 };
 
 }  // namespace
@@ -80,6 +91,8 @@ struct lbm_ctx {
     hipStream_t side[N_KINDS] = {};
     hipEvent_t join[N_KINDS] = {};
     bool autotune = true;
+    int graph_steps = 32;              // steps per captured graph in lbm_step (0: no graphs)
+    StepGraph graph;
     long long bytes = 0;
     Group family[N_KINDS];             // all cases of each kernel family (lbm_step)
     std::vector<Group> single;         // one group per case (lbm_step_case)
@@ -411,6 +424,8 @@ int lbm_destroy(lbm_ctx* c)
     };
     for (Group& g : c->family) free_group(g);
     for (Group& g : c->single) free_group(g);
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -628,6 +643,7 @@ int group_sync(lbm_ctx* c, Group& g)
     g.ntiles = tile0;
     g.k = 0;
     g.written = true;
+    g.version++;
     return LBM_OK;
 }
 
@@ -718,6 +734,81 @@ int group_step(lbm_ctx* c, Group& g, int n_steps, int flags, hipStream_t st, boo
 
 int group_step(lbm_ctx* c, Group& g, int n_steps, int flags) { return group_step(c, g, n_steps, flags, c->stream, false); }
 
+// Queues n steps of the active families (already prepared).  The families advance disjoint sets of
+// cases, so with several of them each runs its n steps on a stream of its own, forked from and
+// joined back to the context's stream: small launches of different families fill the GPU together.
+int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int flags)
+{
+    if (act.size() == 1) return group_step(c, *act[0], n_steps, flags, c->stream, true);
+    HIPCHK(c, hipEventRecord(c->fork, c->stream));
+    for (size_t j = 0; j < act.size(); ++j) {
+        hipStream_t st = c->side[j];
+        HIPCHK(c, hipStreamWaitEvent(st, c->fork, 0));
+        if (int r = group_step(c, *act[j], n_steps, flags, st, true)) return r;
+        HIPCHK(c, hipEventRecord(c->join[j], st));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[j], 0));
+    }
+    return LBM_OK;
+}
+
+// Host bookkeeping of n steps run by a graph replay (the launches themselves were captured).
+void advance(lbm_ctx* c, Group& g, int n_steps)
+{
+    g.k += n_steps;
+    for (int id : g.members) {
+        c->cases[id].cur ^= (n_steps & 1);
+        c->cases[id].aux_valid = false;
+    }
+}
+
+void drop_graph(lbm_ctx* c)
+{
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
+    c->graph = StepGraph{};
+}
+
+std::vector<long long> graph_key(const std::vector<Group*>& act)
+{
+    std::vector<long long> key;
+    for (const Group* g : act) {
+        key.push_back(g->kind);
+        key.push_back(g->version);
+        key.push_back(g->ntiles);
+        key.push_back(g->k & 1);
+    }
+    return key;
+}
+
+// Captures graph_steps steps of the active families from the context's stream (the side streams
+// join the capture through the fork / join events), unless the current graph already covers them.
+// Capturing queues nothing, so the host bookkeeping of the captured steps is undone afterwards.
+int ensure_graph(lbm_ctx* c, const std::vector<Group*>& act, int steps)
+{
+    std::vector<long long> key = graph_key(act);
+    if (c->graph.exec && c->graph.steps == steps && c->graph.key == key) return LBM_OK;
+    drop_graph(c);
+    std::vector<int> k0, cur0;
+    for (const Group* g : act) k0.push_back(g->k);
+    for (const CaseState& k : c->cases) cur0.push_back(k.cur);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    const int r = queue_steps(c, act, steps, 0);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    for (size_t j = 0; j < act.size(); ++j) act[j]->k = k0[j];
+    for (size_t i = 0; i < c->cases.size(); ++i) c->cases[i].cur = cur0[i];
+    if (r != LBM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, LBM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph.graph = graph;
+    HIPCHK(c, hipGraphInstantiate(&c->graph.exec, graph, nullptr, nullptr, 0));
+    c->graph.steps = steps;
+    c->graph.key = std::move(key);
+    return LBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -737,29 +828,38 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
     std::vector<Group*> act;
     for (Group& g : c->family)
         if (!g.members.empty()) act.push_back(&g);
-    if (act.size() == 1) return group_step(c, *act[0], n_steps, flags);
     if (act.empty()) return LBM_OK;
-    // The kernel families advance disjoint sets of cases, so each runs all n steps on a stream of
-    // its own (forked from and joined back to the context's stream): small launches of different
-    // families fill the GPU together.
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     for (Group* g : act)
         if (int r = group_prepare(c, *g, rec)) return r;
-    if (!c->fork) {
+    if (act.size() > 1 && !c->fork) {
         HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         for (int k = 0; k < N_KINDS; ++k) {
             HIPCHK(c, hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking));
             HIPCHK(c, hipEventCreateWithFlags(&c->join[k], hipEventDisableTiming));
         }
     }
-    HIPCHK(c, hipEventRecord(c->fork, c->stream));
-    for (size_t j = 0; j < act.size(); ++j) {
-        hipStream_t st = c->side[j];
-        HIPCHK(c, hipStreamWaitEvent(st, c->fork, 0));
-        if (int r = group_step(c, *act[j], n_steps, flags, st, true)) return r;
-        HIPCHK(c, hipEventRecord(c->join[j], st));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[j], 0));
+    // Whole graphs of graph_steps steps first (the last step stays outside when it records aux),
+    // then the remaining steps launch by launch.
+    int left = n_steps;
+    const int G = c->graph_steps;
+    if (G > 0 && n_steps - (rec ? 1 : 0) >= G) {
+        const int reps = (n_steps - (rec ? 1 : 0)) / G;
+        if (int r = ensure_graph(c, act, G)) return r;
+        for (int i = 0; i < reps; ++i) HIPCHK(c, hipGraphLaunch(c->graph.exec, c->stream));
+        for (Group* g : act) advance(c, *g, reps * G);
+        left -= reps * G;
     }
+    if (left > 0) return queue_steps(c, act, left, flags);
+    return LBM_OK;
+}
+
+int lbm_set_graph(lbm_ctx* c, int steps_per_graph)
+{
+    if (!c) return LBM_E_ARG;
+    if (steps_per_graph < 0 || (steps_per_graph & 1)) return fail(c, LBM_E_ARG, "steps per graph must be even and >= 0");
+    c->graph_steps = steps_per_graph;
+    drop_graph(c);
     return LBM_OK;
 }
 

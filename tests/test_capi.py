@@ -80,3 +80,4 @@ def test_create_rejects_bad_arguments_without_touching_a_device():
     assert L.lbm_step_case(None, 0, 1, 0) == -1
     assert L.lbm_set_reference(None, 0, 1.0, 1.0) == -1
     assert L.lbm_sync(None) == -1
+    assert L.lbm_set_graph(None, 4) == -1
