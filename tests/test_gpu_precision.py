@@ -2,9 +2,10 @@
 
 Batched (many cases per launch) FP64 must equal the oracle bit for bit, like the one-case path.
 
-The FP32 and FP16 forms compute in FP32 and store, per site and direction, the pair
-(s_i / w_i, e_i) with s_i = f_R,i + f_B,i - w_i and e_i = (f_R,i - f_B,i) / (f_R,i + f_B,i), in
-FP32 or FP16.  They are compared with the FP64 path (itself equal to the oracle, see
+The FP32 and FP16 forms compute in FP32 (colour sums in FP64) and store, per site and direction, the
+pair (h_i, e_i) with e_i = (f_R,i - f_B,i) / (f_R,i + f_B,i) and h_i the deviation
+s_i = f_R,i + f_B,i - w_i scaled by a power of two near 1 / w_i, less the case's reference density,
+in FP32 or FP16.  They are compared with the FP64 path (itself equal to the oracle, see
 test_gpu_parity.py) at the tolerances written in each test:
 
 * populations after 20 steps (70 x 45 drop, width 3.45): max |df| <= 5e-7 (FP32), 1e-4 (FP16);
