@@ -39,8 +39,20 @@ constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0), correctly rounded
 using gcdouble = __attribute__((address_space(1))) const double;
 using gdouble = __attribute__((address_space(1))) double;
 
+// Element at a 32-bit byte offset from a uniform base: the load or store is issued with the base in
+// scalar registers and one 32-bit VGPR offset, without per-lane 64-bit address arithmetic.  A
+// population array of one case is < 4 GiB (lbm_create bounds the site count), so the offset fits.
+__device__ __forceinline__ double at_byte(gcdouble* base, unsigned off)
+{
+    return *(gcdouble*)((__attribute__((address_space(1))) const char*)base + off);
+}
+__device__ __forceinline__ void put_byte(gdouble* base, unsigned off, double v)
+{
+    __builtin_nontemporal_store(v, (gdouble*)((__attribute__((address_space(1))) char*)base + off));
+}
+
 // Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).  Each population array is a
-// uniform base (p + i S) plus a 32-bit site offset.
+// uniform base (p + i S) plus a 32-bit byte offset of the site.
 __device__ __forceinline__ void load_row(gcdouble* __restrict__ pR, gcdouble* __restrict__ pB,
                                          long long S, int nx, int ny, int y, const Lane& L,
                                          double fR[Q], double fB[Q])
@@ -50,8 +62,9 @@ __device__ __forceinline__ void load_row(gcdouble* __restrict__ pR, gcdouble* __
     for (int i = 0; i < Q; ++i) {
         const unsigned xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
         const unsigned rs = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
-        fR[i] = (pR + i * S)[rs + xs];
-        fB[i] = (pB + i * S)[rs + xs];
+        const unsigned off = (rs + xs) * (unsigned)sizeof(double);
+        fR[i] = at_byte(pR + i * S, off);
+        fB[i] = at_byte(pB + i * S, off);
     }
 }
 
@@ -330,8 +343,8 @@ __device__ __forceinline__ void store_row(gdouble* __restrict__ oR, gdouble* __r
 {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        __builtin_nontemporal_store(fR[i], (oR + i * S) + site);
-        __builtin_nontemporal_store(fB[i], (oB + i * S) + site);
+        put_byte(oR + i * S, site * (unsigned)sizeof(double), fR[i]);
+        put_byte(oB + i * S, site * (unsigned)sizeof(double), fB[i]);
     }
 }
 
@@ -463,8 +476,8 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
             gdouble* __restrict__ oB = t.outB;
             const long long S = t.S;
             auto out = [=](int i, double r, double b) {
-                __builtin_nontemporal_store(r, (oR + i * S) + site);
-                __builtin_nontemporal_store(b, (oB + i * S) + site);
+                put_byte(oR + i * S, site * (unsigned)sizeof(double), r);
+                put_byte(oB + i * S, site * (unsigned)sizeof(double), b);
             };
             if (!s.skip) {
                 recolour(cc, s, gx, gy, gz, mod, out);
@@ -657,8 +670,8 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
                 gdouble* __restrict__ oB = t.outB;
                 const long long S = t.S;
                 recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) {
-                    __builtin_nontemporal_store(r, (oR + i * S) + site);
-                    __builtin_nontemporal_store(b, (oB + i * S) + site);
+                    put_byte(oR + i * S, site * (unsigned)sizeof(double), r);
+                    put_byte(oB + i * S, site * (unsigned)sizeof(double), b);
                 });
             } else {
                 store_row(t.outR, t.outB, t.S, site, fR, fB);
