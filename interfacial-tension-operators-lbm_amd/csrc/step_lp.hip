@@ -498,8 +498,8 @@ __device__ __forceinline__ void halo2_phase(typename PairOf<ST>::raw va, typenam
 // Aligned 64-column strip kernel for the two-row stencils (F1, F2): rows y-2 .. y+2 of the colour
 // field are kept for the own lanes (P) and the four halo columns (HW); the gradient and the n / T
 // field at the halo columns c0-1 and c0+64 are evaluated on the wavefront's uniform values with the
-// same functions as every lane's.  Row y's populations are read again (from L2) once its force is
-// known.
+// same functions as every lane's.  With FP32 pairs the loads of row y + 3 are issued while row y is
+// worked on.  Row y's populations are read again (from L2) for its collision once its force is known.
 template <typename ST, int KIND>
 __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
 {
@@ -517,14 +517,16 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     h.init(nx, t.strip);
     const int y0 = t.y0, y1 = t.y1;
 
-    R cur[Q];
+    R cur[Q], nxt[Q];
+    R hva, hvb;
     float P[5], HW[5][4];
+    double pd[3];   // FP64 colour field of rows y, y+1, y+2 (for their collision)
     auto phi_row = [&](int y, float (&hw)[4]) {
         load_raw<ST>(in, nx, ny, y, L, cur);
         R va, vb;
         h.load(in, nx, ny, y, va, vb);
         halo2_phase<ST>(va, vb, rf, hw);
-        return (float)raw_phase<ST>(cur, rf);
+        return raw_phase<ST>(cur, rf);
     };
     auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
         const float hl[3] = {HW[j][1], HW[j + 1][1], HW[j + 2][1]};
@@ -547,15 +549,36 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
         }
         return aux_of<KIND>(k, n);
     };
-    P[0] = phi_row(y0 - 2, HW[0]);
-    P[1] = phi_row(y0 - 1, HW[1]);
-    P[2] = phi_row(y0, HW[2]);
-    P[3] = phi_row(y0 + 1, HW[3]);
+    P[0] = (float)phi_row(y0 - 2, HW[0]);
+    P[1] = (float)phi_row(y0 - 1, HW[1]);
+    pd[1] = phi_row(y0, HW[2]);
+    P[2] = (float)pd[1];
+    pd[2] = phi_row(y0 + 1, HW[3]);
+    P[3] = (float)pd[2];
     AuxF am = aux_lanes(0), a0 = aux_lanes(1);
+    // FP32 pairs: row y + 2's loads are issued one iteration ahead (measured faster); FP16 pairs
+    // load it in the iteration, keeping registers for more wavefronts per SIMD
+    constexpr bool AHEAD = sizeof(R) == 8;
+    if constexpr (AHEAD) {
+        load_raw<ST>(in, nx, ny, y0 + 2, L, nxt);
+        h.load(in, nx, ny, y0 + 2, hva, hvb);
+    }
 
     for (int y = y0; y < y1; ++y) {
-        P[4] = phi_row(y + 2, HW[4]);
-        load_raw<ST>(in, nx, ny, y, L, cur);   // row y again, for its collision
+        pd[0] = pd[1];
+        pd[1] = pd[2];
+        if constexpr (AHEAD) {
+            pd[2] = raw_phase<ST>(nxt, rf);   // row y + 2 (waits for its loads)
+            halo2_phase<ST>(hva, hvb, rf, HW[4]);
+            if (y + 1 < y1) {   // the next iteration's row y + 3
+                load_raw<ST>(in, nx, ny, y + 3, L, nxt);
+                h.load(in, nx, ny, y + 3, hva, hvb);
+            }
+        } else {
+            pd[2] = phi_row(y + 2, HW[4]);
+        }
+        P[4] = (float)pd[2];
+        load_raw<ST>(in, nx, ny, y, L, cur);   // row y again (from L2), for its collision
         const AuxF ap = aux_lanes(2);
         AuxF hl[3], hr[3];
 #pragma unroll
@@ -586,13 +609,12 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             Row row;
             decode<ST>(cur, row, rf);
             float ds, rho;
-            double phi;
-            row_phase(row, ds, rho, phi);
+            row_density(row, ds, rho);
             const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
             const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
             auto st = [&](int i, float s, float e) { pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
             if (rho > 1.0e-30f) {
-                collide_lp<true>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+                collide_lp<true>(k, row, ds, rho, pd[0], gx, gy, Fx, Fy, st);
             } else {
 #pragma unroll
                 for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
