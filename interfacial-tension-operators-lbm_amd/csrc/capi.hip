@@ -31,7 +31,7 @@ struct CaseState {
     lbm_case_desc d{};
     CaseConst cc{};
     long long n = 0, S = 0;
-    double* pop[2] = {nullptr, nullptr};   // each [colour][19][S]
+    double* pop[2] = {nullptr, nullptr};   // FP64: row blocks [y][colour][19][nx]; FP32/FP16: pairs ((y*19 + i)*nx + x)
     int cur = 0;                            // buffer holding the latest post-collision state
     double* aux = nullptr;                  // [6][S] gm, F
     bool aux_valid = false;
@@ -326,15 +326,15 @@ int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
 // FP64 view [colour][19][S] of the current post-collision state (device pointer)
 int state_f64(lbm_ctx* c, CaseState& k, const double** out)
 {
-    if (k.elem == 8) {
-        *out = (const double*)k.pop[k.cur];
-        return LBM_OK;
-    }
     if (!k.f64) {
         if (int r = dev_alloc(c, (void**)&k.f64, sizeof(double) * 2 * Q * (size_t)k.S)) return r;
     }
-    HIPCHK(c, launch_lp_to_f64(k.elem, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
-                               k.f64 + Q * k.S, c->stream));
+    if (k.elem == 8) {
+        HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.d.nx, k.n, k.S, k.f64, c->stream));
+    } else {
+        HIPCHK(c, launch_lp_to_f64(k.elem, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
+                                   k.f64 + Q * k.S, c->stream));
+    }
     *out = k.f64;
     return LBM_OK;
 }
@@ -456,7 +456,14 @@ int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
             }
         }
     if (k.elem == 8) {
-        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+        // row blocks [y][colour][19][nx]
+        std::vector<double> rows(2 * Q * (size_t)k.n);
+        for (int y = 0; y < ny; ++y)
+            for (int j = 0; j < 2 * Q; ++j)
+                for (int x = 0; x < nx; ++x)
+                    rows[((size_t)y * 2 * Q + j) * nx + x] = h[(size_t)j * k.S + (size_t)y * nx + x];
+        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice,
+                                 c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     } else {
         if (int r = upload_lp(c, k, h)) return r;

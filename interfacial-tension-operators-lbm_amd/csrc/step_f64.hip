@@ -23,6 +23,8 @@
 // +0.0, where adding a signed zero never changes the value.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lattice.h"
 #include "step_kernels.h"
 #include "tile.h"
@@ -39,33 +41,39 @@ constexpr double SQRT2 = 1.4142135623730951;  // sqrt(2.0), correctly rounded
 using gcdouble = __attribute__((address_space(1))) const double;
 using gdouble = __attribute__((address_space(1))) double;
 
-// Element at a 32-bit byte offset from a uniform base: the load or store is issued with the base in
-// scalar registers and one 32-bit VGPR offset, without per-lane 64-bit address arithmetic.  A
-// population array of one case is < 4 GiB (lbm_create bounds the site count), so the offset fits.
-__device__ __forceinline__ double at_byte(gcdouble* base, unsigned off)
+// State layout: row blocks [y][colour][19][nx] (colour 0 red, 1 blue), so the 38 velocity rows of one
+// y are contiguous and one buffer resource (base in scalar registers) covers them; an access gives the
+// lane's byte offset x * 8 and the velocity row's byte offset as a scalar.
+__device__ __forceinline__ long long block_doubles(int nx) { return 2LL * Q * nx; }
+__device__ __forceinline__ unsigned block_bytes(int nx) { return (unsigned)(block_doubles(nx) * (long long)sizeof(double)); }
+__device__ __forceinline__ unsigned vrow_off(int colour, int i, int nx)
 {
-    return *(gcdouble*)((__attribute__((address_space(1))) const char*)base + off);
-}
-__device__ __forceinline__ void put_byte(gdouble* base, unsigned off, double v)
-{
-    __builtin_nontemporal_store(v, (gdouble*)((__attribute__((address_space(1))) char*)base + off));
+    return (unsigned)((colour * Q + i) * nx) * (unsigned)sizeof(double);
 }
 
-// Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).  Each population array is a
-// uniform base (p + i S) plus a 32-bit byte offset of the site.
-__device__ __forceinline__ void load_row(gcdouble* __restrict__ pR, gcdouble* __restrict__ pB,
-                                         long long S, int nx, int ny, int y, const Lane& L,
+// Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).
+__device__ __forceinline__ void load_row(gcdouble* __restrict__ in, int nx, int ny, int y, const Lane& L,
                                          double fR[Q], double fB[Q])
 {
-    const unsigned rm = wrapi(y - 1, ny) * nx, r0 = wrapi(y, ny) * nx, rp = wrapi(y + 1, ny) * nx;
+    const long long rb = block_doubles(nx);
+    const unsigned bytes = block_bytes(nx);
+    const rsrc_t rm = make_rsrc(in + wrapi(y - 1, ny) * rb, bytes);
+    const rsrc_t r0 = make_rsrc(in + wrapi(y, ny) * rb, bytes);
+    const rsrc_t rp = make_rsrc(in + wrapi(y + 1, ny) * rb, bytes);
+    const unsigned om = (unsigned)L.xm * 8u, o0 = (unsigned)L.col * 8u, op = (unsigned)L.xp * 8u;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const unsigned xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
-        const unsigned rs = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
-        const unsigned off = (rs + xs) * (unsigned)sizeof(double);
-        fR[i] = at_byte(pR + i * S, off);
-        fB[i] = at_byte(pB + i * S, off);
+        const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : o0);
+        const rsrc_t r = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
+        fR[i] = buf_load<double>(r, xs, vrow_off(0, i, nx));
+        fB[i] = buf_load<double>(r, xs, vrow_off(1, i, nx));
     }
+}
+
+// element index of population (colour, i) of site (x, y) in the row-block layout
+__device__ __forceinline__ unsigned long long elem_index(int nx, int y, int colour, int i, unsigned x)
+{
+    return (unsigned long long)y * (unsigned long long)block_doubles(nx) + (unsigned)((colour * Q + i) * nx) + x;
 }
 
 // density (LBM_functions.cpp:1581-1590): sequential sum
@@ -338,14 +346,16 @@ __device__ __forceinline__ void recolour(const CaseConst& cc, const Site& s, dou
     recol(18, 17, fw2, gy - gz, mod_d);
 }
 
-__device__ __forceinline__ void store_row(gdouble* __restrict__ oR, gdouble* __restrict__ oB, long long S,
-                                          unsigned site, const double fR[Q], const double fB[Q])
+// nontemporal stores of a site's 38 post-collision populations into row block `ro`
+__device__ __forceinline__ void store_pop(rsrc_t ro, int nx, unsigned xoff, int i, double r, double b)
+{
+    buf_store<double>(r, ro, xoff, vrow_off(0, i, nx));
+    buf_store<double>(b, ro, xoff, vrow_off(1, i, nx));
+}
+__device__ __forceinline__ void store_row(rsrc_t ro, int nx, unsigned xoff, const double fR[Q], const double fB[Q])
 {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        put_byte(oR + i * S, site * (unsigned)sizeof(double), fR[i]);
-        put_byte(oB + i * S, site * (unsigned)sizeof(double), fB[i]);
-    }
+    for (int i = 0; i < Q; ++i) store_pop(ro, nx, xoff, i, fR[i], fB[i]);
 }
 
 __device__ __forceinline__ void store_aux(gdouble* __restrict__ aux, long long S, unsigned site, double gx, double gy,
@@ -360,12 +370,10 @@ __device__ __forceinline__ double phase(double rR, double rB) { return (rR - rB)
 // One wavefront = one tile of one case: the tile -> case table gives the case, its entry the rest.
 struct TileCtx {
     CaseConst cc;
-    gcdouble* inR;
-    gcdouble* inB;
-    gdouble* outR;
-    gdouble* outB;
+    gcdouble* in;    // row blocks of the current state
+    gdouble* out;    // row blocks of the next state
     gdouble* aux;
-    long long S;
+    long long S;     // aux stride
     int strip, y0, y1;
 };
 
@@ -375,13 +383,9 @@ __device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtx& t)
     const int c = L.tile_case[tile];
     const CaseDev& d = L.tab[c];
     t.cc = d.cc;
-    gcdouble* in = (gcdouble*)d.pop[L.parity];
-    gdouble* out = (gdouble*)d.pop[L.parity ^ 1];
+    t.in = (gcdouble*)d.pop[L.parity];
+    t.out = (gdouble*)d.pop[L.parity ^ 1];
     t.S = d.S;
-    t.inR = in;
-    t.inB = in + Q * d.S;
-    t.outR = out;
-    t.outB = out + Q * d.S;
     t.aux = L.record ? (gdouble*)d.aux : nullptr;
     const int local = tile - d.tile0;
     t.strip = local % d.nstrip;
@@ -403,16 +407,14 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     const int nx = cc.nx, ny = cc.ny;
     const Lane L = make_lane_aligned(nx, t.strip);
     const Halo1 h = make_halo1(nx, t.strip);
-    gcdouble* const hR = t.inR + h.i * t.S;
-    gcdouble* const hB = t.inB + h.i * t.S;
     // every lane loads (lanes >= 38 re-read an in-range value that no one uses): no branch
     auto halo_load = [&](int y, double& vr, double& vb) {
         int yy = y + h.dy;
         yy = (yy < 0) ? yy + ny : yy;
         yy = (yy >= ny) ? yy - ny : yy;
-        const unsigned src = (unsigned)yy * (unsigned)nx + h.x;
-        vr = hR[src];
-        vb = hB[src];
+        const unsigned long long e = elem_index(nx, yy, 0, h.i, h.x);
+        vr = t.in[e];
+        vb = t.in[e + (unsigned)(Q * nx)];
     };
     auto halo_phase = [&](double vr, double vb, double& l, double& r) {
         l = phase(lane_dens(vr, 0), lane_dens(vb, 0));
@@ -422,11 +424,11 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     double cR_[Q], cB_[Q], nR[Q], nB[Q];
     double vr, vb;
     double ph[3], hl[3], hr[3];   // colour field of rows y-1, y, y+1 and of the two halo columns
-    load_row(t.inR, t.inB, t.S, nx, ny, t.y0 - 1, L, nR, nB);
+    load_row(t.in, nx, ny, t.y0 - 1, L, nR, nB);
     halo_load(t.y0 - 1, vr, vb);
     ph[0] = phase(dens(nR), dens(nB));
     halo_phase(vr, vb, hl[0], hr[0]);
-    load_row(t.inR, t.inB, t.S, nx, ny, t.y0, L, cR_, cB_);
+    load_row(t.in, nx, ny, t.y0, L, cR_, cB_);
     halo_load(t.y0, vr, vb);
     double rRc = dens(cR_), rBc = dens(cB_);
     ph[1] = phase(rRc, rBc);
@@ -435,7 +437,7 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     for (int y = t.y0; y < t.y1; ++y) {
         // next row's loads go out first; the gradient-independent part of this row's collision
         // (sums and, for P1/P3, the whole BGK relaxation) runs while they are in flight
-        load_row(t.inR, t.inB, t.S, nx, ny, y + 1, L, nR, nB);
+        load_row(t.in, nx, ny, y + 1, L, nR, nB);
         halo_load(y + 1, vr, vb);
         Site s;
         site_sums(s, cR_, cB_, rRc, rBc);
@@ -472,19 +474,15 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
 
         if (L.interior) {
             const unsigned site = (unsigned)(y * nx + L.col);
-            gdouble* __restrict__ oR = t.outR;
-            gdouble* __restrict__ oB = t.outB;
-            const long long S = t.S;
-            auto out = [=](int i, double r, double b) {
-                put_byte(oR + i * S, site * (unsigned)sizeof(double), r);
-                put_byte(oB + i * S, site * (unsigned)sizeof(double), b);
-            };
+            const rsrc_t ro = make_rsrc(t.out + (long long)y * block_doubles(nx), block_bytes(nx));
+            const unsigned xoff = (unsigned)L.col * 8u;
+            auto out = [=](int i, double r, double b) { store_pop(ro, nx, xoff, i, r, b); };
             if (!s.skip) {
                 recolour(cc, s, gx, gy, gz, mod, out);
             } else {   // left untouched (rare)
                 double uR[Q], uB[Q];
-                load_row(t.inR, t.inB, t.S, nx, ny, y, L, uR, uB);
-                store_row(t.outR, t.outB, t.S, site, uR, uB);
+                load_row(t.in, nx, ny, y, L, uR, uB);
+                store_row(ro, nx, xoff, uR, uB);
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
@@ -536,18 +534,15 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
     const int nx = cc.nx, ny = cc.ny;
     const Lane L = make_lane_aligned(nx, t.strip);
     const Halo2 h = make_halo2(nx, t.strip);
-    gcdouble* const hR0 = t.inR + h.i[0] * t.S;
-    gcdouble* const hB0 = t.inB + h.i[0] * t.S;
-    gcdouble* const hR1 = t.inR + h.i[1] * t.S;
-    gcdouble* const hB1 = t.inB + h.i[1] * t.S;
 
     double fR[Q], fB[Q];
     // colour field of a row: own lanes, and the four halo columns (c0-2, c0-1, c0+64, c0+65)
     auto phi_row = [&](int y, double hw[4]) {
-        load_row(t.inR, t.inB, t.S, nx, ny, y, L, fR, fB);
-        const unsigned s0 = (unsigned)wrap_row(y + h.dy[0], ny) * (unsigned)nx + h.x[0];
-        const unsigned s1 = (unsigned)wrap_row(y + h.dy[1], ny) * (unsigned)nx + h.x[1];
-        const double ra = hR0[s0], ba = hB0[s0], rb = hR1[s1], bb = hB1[s1];
+        load_row(t.in, nx, ny, y, L, fR, fB);
+        const unsigned long long e0 = elem_index(nx, wrap_row(y + h.dy[0], ny), 0, h.i[0], h.x[0]);
+        const unsigned long long e1 = elem_index(nx, wrap_row(y + h.dy[1], ny), 0, h.i[1], h.x[1]);
+        const unsigned cb = (unsigned)(Q * nx);
+        const double ra = t.in[e0], ba = t.in[e0 + cb], rb = t.in[e1], bb = t.in[e1 + cb];
 #pragma unroll
         for (int q = 0; q < 4; ++q) hw[q] = phase(halo2_dens(ra, rb, q), halo2_dens(ba, bb, q));
         return phase(dens(fR), dens(fB));
@@ -593,7 +588,7 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
         P[4] = phi_row(y + 2, HW[4]);
         // populations of row y again (read two rows ago for the colour field, now in L2); issued
         // before the force evaluation so that they arrive while it runs
-        load_row(t.inR, t.inB, t.S, nx, ny, y, L, fR, fB);
+        load_row(t.in, nx, ny, y, L, fR, fB);
         double gp0, gp1, gp2;
         grad_lanes(2, gp0, gp1, gp2);   // row y+1
         const Aux6 ap = aux_of<OP>(cc, gp0, gp1, gp2);
@@ -662,19 +657,15 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
 
         if (L.interior) {
             const unsigned site = (unsigned)(y * nx + L.col);
+            const rsrc_t ro = make_rsrc(t.out + (long long)y * block_doubles(nx), block_bytes(nx));
+            const unsigned xoff = (unsigned)L.col * 8u;
             Site s;
             site_sums(s, fR, fB, dens(fR), dens(fB));
             if (!s.skip) {
                 bgk<true>(cc, s, Fx, Fy, Fz);
-                gdouble* __restrict__ oR = t.outR;
-                gdouble* __restrict__ oB = t.outB;
-                const long long S = t.S;
-                recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) {
-                    put_byte(oR + i * S, site * (unsigned)sizeof(double), r);
-                    put_byte(oB + i * S, site * (unsigned)sizeof(double), b);
-                });
+                recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) { store_pop(ro, nx, xoff, i, r, b); });
             } else {
-                store_row(t.outR, t.outB, t.S, site, fR, fB);
+                store_row(ro, nx, xoff, fR, fB);
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
@@ -702,6 +693,27 @@ int step_kind(int op)
     case OP_F1: return KIND_F1;
     default: return KIND_F2;
     }
+}
+
+namespace {
+// row blocks [y][colour][19][nx] -> [colour][19][S] (the diagnostics' and transfers' view)
+__global__ void __launch_bounds__(256) f64_rows_to_soa(const double* __restrict__ pop, int nx, long long n, long long S,
+                                                       double* __restrict__ soa)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long y = p / nx, x = p - y * nx;
+        const double* row = pop + y * (2LL * Q * nx) + x;
+#pragma unroll
+        for (int j = 0; j < 2 * Q; ++j) soa[j * S + p] = row[(long long)j * nx];
+    }
+}
+}  // namespace
+
+hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream)
+{
+    const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(f64_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, nx, n, S, soa);
+    return hipGetLastError();
 }
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)

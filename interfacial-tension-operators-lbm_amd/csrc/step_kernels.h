@@ -32,6 +32,8 @@ int step_halo(int op);
 int step_kind(int op);
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
+// FP64 state in row blocks [y][colour][19][nx] -> [colour][19][S]
+hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream);
 
 // FP32-compute forms; storage FP32 (elem = 4) or FP16 (elem = 2): per population the pair
 // (s_i / w_i, e_i), s_i = fR_i + fB_i - w_i, e_i = (fR_i - fB_i) / (fR_i + fB_i), at ((y * 19 + i) nx + x)
