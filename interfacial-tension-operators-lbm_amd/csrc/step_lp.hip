@@ -2,13 +2,15 @@
 // kernels (pull-stream, colour field, mediator gradient, tension operator, BGK + Guo, recolouring),
 // reformulated in deviation variables so that single and half precision keep their digits.
 //
-// Storage: one pair (A_i, e_i) per site and velocity i, in rows: pair(i, x, y) at ((y * 19 + i) nx + x).
+// Storage: one pair (h_i, e_i) per site and velocity i, in rows: pair(i, x, y) at ((y * 19 + i) nx + x).
 // A row of one velocity is contiguous (coalesced pulls and stores) and the 19 velocities of a row sit
-// together (few pages per wavefront, so few translation misses):
-//   A_i = s_i / w_i,   s_i = fR_i + fB_i - w_i      deviation of the total population from w_i (rho0 = 1)
+// together (one buffer resource per row block, few pages per wavefront):
+//   s_i = fR_i + fB_i - w_i                         deviation of the total population from w_i (rho0 = 1)
+//   h_i = 2^k s_i - c_k r(e_i)                      stored: s_i scaled exactly (lattice.h), less the
+//                                                   case's reference density
 //   e_i = (fR_i - fB_i) / (fR_i + fB_i)             colour fraction of the population, exactly +-1 in a
 //                                                   pure phase
-// From these a site has f_i = w_i (1 + A_i), rho = 1 + sum_i s_i, rho_R - rho_B = sum_i e_i f_i and
+// From these a site has f_i = w_i + s_i, rho = 1 + sum_i s_i, rho_R - rho_B = sum_i e_i f_i and
 // phi = rho^N = (rho_R - rho_B) / rho; the colour populations are fR_i = f_i (1 + e_i) / 2 and
 // fB_i = f_i (1 - e_i) / 2.  Collision and recolouring (colisao_comum, Operadores_tensao.cpp:340-373;
 // coll_BGK, Collision_operators.cpp:11-49; recolloring, LBM_functions.cpp:1712-1746) act on s_i:
@@ -632,7 +634,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     }
 }
 
-// Halo of an aligned strip for the pair layout: one 38-lane load per row (Halo1).  The address is
+// Halo of an aligned strip for the pair layout: one wavefront load per row (halo_lane_lp).  The address is
 // computed from a per-lane row index, apart from the strip's own loads, whose row bases stay uniform
 // (scalar registers) that way.
 template <typename ST>
