@@ -47,8 +47,8 @@ R = 100.0
 PEAK_HBM_GBS = 8000.0
 B_LU = {"f64": 608, "f32": 304, "f16": 152}
 PREC_ID = {"f64": 0, "f32": 1, "f16": 2}
-KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P> (P1/P3 family)",
-          "f16": "lp_row1<_Float16,P> (P1/P3 family)"}
+KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth> (P1/P3 family)",
+          "f16": "lp_pipe<_Float16,P,depth> (P1/P3 family)"}
 
 
 def amplitude(kind, tau, sigma=SIGMA):

@@ -66,6 +66,7 @@ struct Group {
     int cap = 0;                   // entries dtab can hold
     bool written = false;
     bool tuned = false;            // rows per segment chosen by timing (group_tune)
+    int depth = 0;                 // pipeline depth of the FP32/FP16 one-row-stencil kernels (0 = default)
     long long version = 0;         // bumped whenever the table is rewritten
 };
 
@@ -661,17 +662,20 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
     a.tile_case = g.dtile;
     a.parity = parity;
     a.record = record;
+    a.depth = g.depth;
     const int elem = c->cases[g.members[0]].elem;
     HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, st) : launch_step_lp(elem, g.kind, a, g.ntiles, st));
     return LBM_OK;
 }
 int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_group(c, g, parity, record, c->stream); }
 
-// Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel).
-// Each candidate's launch is timed at the group's current parity: it reads the current state and
-// writes the other buffer, so repeating it leaves the simulation unchanged, and the results of a
-// step do not depend on the tile shape (every site is computed the same way), so the choice only
-// changes speed.  Short rows re-read more halo rows; long rows give fewer, longer wavefronts.
+// Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel)
+// and, for the FP32/FP16 one-row-stencil kernels, the pipeline depth (rows in flight per wavefront;
+// deeper pipelines hold more registers, so fewer wavefronts fit a SIMD).  Each candidate's launch is
+// timed at the group's current parity: it reads the current state and writes the other buffer, so
+// repeating it leaves the simulation unchanged, and the results of a step do not depend on the
+// shape or depth (every site is computed the same way), so the choice only changes speed.  Short
+// rows re-read more halo rows; long rows give fewer, longer wavefronts.
 int group_tune(lbm_ctx* c, Group& g)
 {
     g.tuned = true;
@@ -682,32 +686,40 @@ int group_tune(lbm_ctx* c, Group& g)
         HIPCHK(c, hipEventCreate(&c->ev[0]));
         HIPCHK(c, hipEventCreate(&c->ev[1]));
     }
+    const bool pipelined = c->cases[g.members[0]].elem != 8 && (g.kind == KIND_P || g.kind == KIND_F3);
+    const int depths[] = {1, 2, 3};
+    const int n_depth = pipelined ? 3 : 1;
     const int cand[] = {1, 2, 4, 8, 16, 32, 64};
-    int best = -1;
+    int best = -1, best_depth = 0;
     float best_ms = 0.f;
-    int last_eff = -1;
-    for (int ly : cand) {
-        const int eff = std::min(ly, ny_max);
-        if (eff == last_eff) continue;
-        last_eff = eff;
-        g.ly.assign(g.members.size(), ly);
-        g.written = false;
-        if (int r = group_sync(c, g)) return r;
-        const int parity = g.k & 1;
-        if (int r = launch_group(c, g, parity, 0)) return r;   // warm-up
-        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-        for (int k = 0; k < 3; ++k)
-            if (int r = launch_group(c, g, parity, 0)) return r;
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev[1]));
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-        if (best < 0 || ms < best_ms) {
-            best = ly;
-            best_ms = ms;
+    for (int di = 0; di < n_depth; ++di) {
+        g.depth = pipelined ? depths[di] : 0;
+        int last_eff = -1;
+        for (int ly : cand) {
+            const int eff = std::min(ly, ny_max);
+            if (eff == last_eff) continue;
+            last_eff = eff;
+            g.ly.assign(g.members.size(), ly);
+            g.written = false;
+            if (int r = group_sync(c, g)) return r;
+            const int parity = g.k & 1;
+            if (int r = launch_group(c, g, parity, 0)) return r;   // warm-up
+            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+            for (int k = 0; k < 3; ++k)
+                if (int r = launch_group(c, g, parity, 0)) return r;
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            if (best < 0 || ms < best_ms) {
+                best = ly;
+                best_depth = g.depth;
+                best_ms = ms;
+            }
         }
     }
     g.ly.assign(g.members.size(), best);
+    g.depth = best_depth;
     g.written = false;
     return group_sync(c, g);
 }

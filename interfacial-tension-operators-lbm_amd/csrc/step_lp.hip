@@ -697,52 +697,6 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
     }
 }
 
-// One-row-ahead kernel on aligned strips (P1, P3, F3): row y+1's loads (own and halo) are issued,
-// its colour field completes row y's gradient, row y is collided.
-template <typename ST, int KIND>
-__global__ void __launch_bounds__(64) lp_row1(StepLaunch launch)
-{
-    static_assert(KIND == KIND_P || KIND == KIND_F3, "one-row stencils only");
-    using R = typename PairOf<ST>::raw;
-    TileCtxLP t;
-    if (!tile_ctx(launch, t)) return;
-    const int nx = t.cc.nx, ny = t.cc.ny;
-    const ConstF k = to_float(t.cc);
-    const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
-    const Lane L = make_lane_aligned(nx, t.strip);
-    gcst<R>* in = (gcst<R>*)t.in;
-    gst<R>* out = (gst<R>*)t.out;
-    HaloLP<ST> h;
-    h.init(nx, t.strip);
-    const int y0 = t.y0, y1 = t.y1;
-
-    R cur[Q], nxt[Q];
-    float p[3], hl[3], hr[3];
-    load_raw<ST>(in, nx, ny, y0 - 1, L, nxt);
-    R hv = h.load(in, nx, ny, y0 - 1);
-    p[1] = (float)raw_phase<ST>(nxt, rf);
-    halo_phase<ST>(hv, rf, hl[1], hr[1]);
-    load_raw<ST>(in, nx, ny, y0, L, cur);
-    hv = h.load(in, nx, ny, y0);
-    double phn = raw_phase<ST>(cur, rf);   // phi of the next row to collide
-    p[2] = (float)phn;
-    halo_phase<ST>(hv, rf, hl[2], hr[2]);
-    for (int y = y0; y < y1; ++y) {
-        load_raw<ST>(in, nx, ny, y + 1, L, nxt);
-        hv = h.load(in, nx, ny, y + 1);
-        p[0] = p[1]; p[1] = p[2];
-        hl[0] = hl[1]; hl[1] = hl[2];
-        hr[0] = hr[1]; hr[1] = hr[2];
-        const double phc = phn;
-        phn = raw_phase<ST>(nxt, rf);
-        p[2] = (float)phn;
-        halo_phase<ST>(hv, rf, hl[2], hr[2]);
-        collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr, phc);
-#pragma unroll
-        for (int i = 0; i < Q; ++i) cur[i] = nxt[i];
-    }
-}
-
 __device__ __forceinline__ void opaque(f32x2& v)
 {
     float a = v.x, b = v.y;
@@ -751,38 +705,18 @@ __device__ __forceinline__ void opaque(f32x2& v)
 }
 __device__ __forceinline__ void opaque(unsigned& v) { asm volatile("" : "+v"(v)); }
 
-// Two-row-deep pipelined variant: three raw row buffers rotate, so while row y is collided the
-// loads of row y+2 are in flight.  The row loop is unrolled three times (buffer roles are
-// compile-time constants).
-template <typename ST, int KIND>
-__device__ __forceinline__ void pipe_row(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
-                                         const HaloLP<ST>& h, gcst<typename PairOf<ST>::raw>* in,
-                                         gst<typename PairOf<ST>::raw>* out, int y, typename PairOf<ST>::raw (&cur)[Q],
-                                         typename PairOf<ST>::raw (&nxt)[Q], typename PairOf<ST>::raw (&nn)[Q],
-                                         typename PairOf<ST>::raw& hnxt, typename PairOf<ST>::raw& hnn, float (&p)[3],
-                                         float (&hl)[3], float (&hr)[3], double& phn)
-{
-    const int nx = t.cc.nx, ny = t.cc.ny;
-    if (y + 2 <= t.y1) {   // rows up to y1 feed the gradients
-        load_raw<ST>(in, nx, ny, y + 2, L, nn);
-        hnn = h.load(in, nx, ny, y + 2);
-    }
-    p[0] = p[1]; p[1] = p[2];
-    hl[0] = hl[1]; hl[1] = hl[2];
-    hr[0] = hr[1]; hr[1] = hr[2];
-    const double phc = phn;
-    phn = raw_phase<ST>(nxt, rf);   // row y + 1 (waits for it)
-    p[2] = (float)phn;
-    halo_phase<ST>(hnxt, rf, hl[2], hr[2]);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) opaque(cur[i]);   // re-decode rather than keep decoded values alive
-    collide_row<ST, KIND>(t, k, rf, L, out, y, cur, p, hl, hr, phc);
-}
-
-template <typename ST, int KIND>
+// Aligned-strip kernel for the one-row stencils (P1, P3, F3), pipelined DEPTH rows deep: DEPTH + 1
+// raw row buffers rotate, so while row y is collided the loads of rows up to y + DEPTH are in
+// flight.  Row y + 1's colour field (own lanes and the two halo columns) completes row y's gradient.
+// The row loop is unrolled DEPTH + 1 times, so every buffer role is a compile-time constant (no
+// register copies).  DEPTH trades registers (occupancy) for bytes in flight per wavefront; the
+// library's tuner picks it with the row-segment length.
+template <typename ST, int KIND, int DEPTH>
 __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
 {
     static_assert(KIND == KIND_P || KIND == KIND_F3, "one-row stencils only");
+    static_assert(DEPTH >= 1 && DEPTH <= 3, "pipeline depth 1..3");
+    constexpr int NB = DEPTH + 1;
     using R = typename PairOf<ST>::raw;
     TileCtxLP t;
     if (!tile_ctx(launch, t)) return;
@@ -796,49 +730,70 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     h.init(nx, t.strip);
     const int y0 = t.y0, y1 = t.y1;
 
-    R ra[Q], rb[Q], rc[Q];
-    R ha, hb, hc;
+    // row r lives in buffer (r - y0) mod NB; row y0 - 1 in buffer NB - 1
+    R buf[NB][Q];
+    R hv[NB];
     float p[3], hl[3], hr[3];
-    load_raw<ST>(in, nx, ny, y0 - 1, L, rc);
-    hc = h.load(in, nx, ny, y0 - 1);
-    load_raw<ST>(in, nx, ny, y0, L, ra);
-    ha = h.load(in, nx, ny, y0);
-    load_raw<ST>(in, nx, ny, y0 + 1, L, rb);
-    hb = h.load(in, nx, ny, y0 + 1);
-    p[1] = (float)raw_phase<ST>(rc, rf);
-    halo_phase<ST>(hc, rf, hl[1], hr[1]);
-    double phn = raw_phase<ST>(ra, rf);   // phi of the next row to collide
-    p[2] = (float)phn;
-    halo_phase<ST>(ha, rf, hl[2], hr[2]);
-    for (int y = y0; y < y1; y += 3) {
-        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y, ra, rb, rc, hb, hc, p, hl, hr, phn);
-        if (y + 1 >= y1) break;
-        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 1, rb, rc, ra, hc, ha, p, hl, hr, phn);
-        if (y + 2 >= y1) break;
-        pipe_row<ST, KIND>(t, k, rf, L, h, in, out, y + 2, rc, ra, rb, ha, hb, p, hl, hr, phn);
+    load_raw<ST>(in, nx, ny, y0 - 1, L, buf[NB - 1]);
+    hv[NB - 1] = h.load(in, nx, ny, y0 - 1);
+#pragma unroll
+    for (int r = 0; r < DEPTH; ++r) {
+        load_raw<ST>(in, nx, ny, y0 + r, L, buf[r]);
+        hv[r] = h.load(in, nx, ny, y0 + r);
     }
+    p[1] = (float)raw_phase<ST>(buf[NB - 1], rf);
+    halo_phase<ST>(hv[NB - 1], rf, hl[1], hr[1]);
+    double phn = raw_phase<ST>(buf[0], rf);   // phi of the next row to collide
+    p[2] = (float)phn;
+    halo_phase<ST>(hv[0], rf, hl[2], hr[2]);
+    for (int y = y0; y < y1; y += NB) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int yy = y + j;
+            if (yy < y1) {
+                R(&cur)[Q] = buf[j];
+                R(&nxt)[Q] = buf[(j + 1) % NB];
+                R(&far)[Q] = buf[(j + DEPTH) % NB];   // held row yy - 1, free now
+                if (yy + DEPTH <= y1) {   // rows up to y1 feed the gradients
+                    load_raw<ST>(in, nx, ny, yy + DEPTH, L, far);
+                    hv[(j + DEPTH) % NB] = h.load(in, nx, ny, yy + DEPTH);
+                }
+                p[0] = p[1]; p[1] = p[2];
+                hl[0] = hl[1]; hl[1] = hl[2];
+                hr[0] = hr[1]; hr[1] = hr[2];
+                const double phc = phn;
+                phn = raw_phase<ST>(nxt, rf);   // row yy + 1 (waits for it)
+                p[2] = (float)phn;
+                halo_phase<ST>(hv[(j + 1) % NB], rf, hl[2], hr[2]);
+                if constexpr (DEPTH > 1) {
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) opaque(cur[i]);   // re-decode rather than keep decoded values alive
+                }
+                collide_row<ST, KIND>(t, k, rf, L, out, yy, cur, p, hl, hr, phc);
+            }
+        }
+    }
+}
+
+template <typename ST, int KIND>
+void launch_pipe(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
+{
+    const dim3 block(64);
+    if (depth <= 1) hipLaunchKernelGGL((lp_pipe<ST, KIND, 1>), grid, block, 0, stream, a);
+    else if (depth == 2) hipLaunchKernelGGL((lp_pipe<ST, KIND, 2>), grid, block, 0, stream, a);
+    else hipLaunchKernelGGL((lp_pipe<ST, KIND, 3>), grid, block, 0, stream, a);
 }
 
 template <typename ST>
 hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
 {
     const dim3 grid(ntiles), block(64);
-    // two-row pipeline for FP32 storage, one row ahead for FP16 storage (measured best on MI355X);
-    // ITOLBM_PIPE=0/1 forces one or the other
-    static const int pipe_env = [] {
-        const char* e = std::getenv("ITOLBM_PIPE");
-        return e ? std::atoi(e) : -1;
-    }();
-    const bool pipe = (pipe_env >= 0) ? pipe_env != 0 : sizeof(typename PairOf<ST>::raw) == 8;
+    // pipeline depth: the launch's (chosen by the tuner); 0 = default, 2 rows for FP32 storage and 1
+    // for FP16 storage (measured best on MI355X without tuning)
+    const int depth = a.depth > 0 ? a.depth : (sizeof(typename PairOf<ST>::raw) == 8 ? 2 : 1);
     switch (kind) {
-    case KIND_P:
-        if (pipe) hipLaunchKernelGGL((lp_pipe<ST, KIND_P>), grid, block, 0, stream, a);
-        else hipLaunchKernelGGL((lp_row1<ST, KIND_P>), grid, block, 0, stream, a);
-        break;
-    case KIND_F3:
-        if (pipe) hipLaunchKernelGGL((lp_pipe<ST, KIND_F3>), grid, block, 0, stream, a);
-        else hipLaunchKernelGGL((lp_row1<ST, KIND_F3>), grid, block, 0, stream, a);
-        break;
+    case KIND_P: launch_pipe<ST, KIND_P>(depth, a, grid, stream); break;
+    case KIND_F3: launch_pipe<ST, KIND_F3>(depth, a, grid, stream); break;
     case KIND_F1: hipLaunchKernelGGL((lp_f12<ST, KIND_F1>), grid, block, 0, stream, a); break;
     case KIND_F2: hipLaunchKernelGGL((lp_f12<ST, KIND_F2>), grid, block, 0, stream, a); break;
     default: return hipErrorInvalidValue;
