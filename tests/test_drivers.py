@@ -298,3 +298,27 @@ def test_vtk_fields_match_oracle(tmp_path):
     u = ((fR + fB) @ c.T - 0.5 * sim.F.reshape(-1, 3)[:, :2]) / rho[:, None]
     assert np.allclose(vel[:, :2], u, rtol=2e-5, atol=1e-10)
     assert np.abs(vel[:, 2]).max() <= 1e-15
+
+
+@pytest.mark.gpu
+def test_reduced_precision_forms_through_the_programs(tmp_path):
+    """The drop-in program with itolbm.txt selecting FP32 / FP16 storage (and re-centring after each
+    diagnostic): laplace.dat's last row (a 96^2 drop, R = 19.2, 6000 steps) against the FP64 run of
+    the same case.  Tolerances as in test_gpu_precision.py: delta_p (column 7) 2e-4 / 3e-2 relative,
+    radius (column 2) 1e-5 / 1e-3, colour masses (columns 12-13) 1e-5 / 5e-4."""
+    rows = {}
+    for prec in ("f64", "f32", "f16"):
+        say(f"[gpu] Imm_OPS 96^2 P3 0.5 drop, precisao {prec}")
+        d = str(tmp_path / prec)
+        K.make_case(d, 96, "P3 0.5", 1.0, 0.225, 6000, options={"precisao": prec, "referencia": 1})
+        r = K.run("Imm_OPS", [], d, timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        rows[prec] = [float(v) for v in K.read_table(os.path.join(d, "laplace.dat"))[1][-1]]
+    ref = rows["f64"]
+    for prec, tol in (("f32", (2e-4, 1e-5, 1e-5)), ("f16", (3e-2, 1e-3, 5e-4))):
+        got = rows[prec]
+        assert got[0] == ref[0]   # same step
+        assert abs(got[6] - ref[6]) <= tol[0] * abs(ref[6]), (prec, "delta_p", got[6], ref[6])
+        assert abs(got[1] - ref[1]) <= tol[1] * ref[1], (prec, "radius", got[1], ref[1])
+        for col in (11, 12):
+            assert abs(got[col] - ref[col]) <= tol[2] * ref[col], (prec, "mass", col, got[col], ref[col])
