@@ -386,6 +386,29 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
             }
         }
     }
+    // Mass and momentum restored to what the update conserves exactly (mass; momentum + F).  With
+    // the FP32 weights the second moment 3 sum_i w_i c_ix^2 is 1 - 6e-8 (1/3 has no binary
+    // representation), so the equilibrium lost a fraction 6e-8 / tau of the momentum every step
+    // (3e-3 of a translating drop's velocity over 40000 steps) and gained mass in proportion to
+    // u^2.  The residuals go to the rest population (mass) and to the +-x, +-y pairs (momentum).
+    {
+        float m = 0.f, px = 0.f, py = 0.f;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            m += s[i];
+            if (CX[i] > 0) px += s[i];
+            if (CX[i] < 0) px -= s[i];
+            if (CY[i] > 0) py += s[i];
+            if (CY[i] < 0) py -= s[i];
+        }
+        const float dx = 0.5f * ((FORCED ? jx + Fx : jx) - px);
+        const float dy = 0.5f * ((FORCED ? jy + Fy : jy) - py);
+        s[0] += ds - m;
+        s[1] += dx;
+        s[2] -= dx;
+        s[3] += dy;
+        s[4] -= dy;
+    }
     // recolouring: e_i = phi - 2 beta cR cB w_i rho cos_i / f_i, the difference taken in FP64 and
     // rounded once
     const float phi = (float)phid;
