@@ -472,12 +472,16 @@ __device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
     AuxF a;
     a.g0 = dx_iso(p);
     a.g1 = dy_iso(p);
-    const float m = sqrtf(a.g0 * a.g0 + a.g1 * a.g1);
+    // |g| and 1/|g| from one reciprocal square root (thresholds as the reference's on |g|:
+    // 1e-12 for F1, 1e-30 for F2, whose square underflows in FP32 so that the test is |g|^2 > 0)
+    const float m2 = a.g0 * a.g0 + a.g1 * a.g1;
+    const float r = __builtin_amdgcn_rsqf(m2);
     if constexpr (KIND == KIND_F1) {
-        const float inv = (m > 1.0e-12f) ? 1.f / m : 0.f;
+        const float inv = (m2 > 1.0e-24f) ? r : 0.f;
         a.a0 = a.g0 * inv; a.a1 = a.g1 * inv; a.a2 = 0.f;
     } else {
-        const float inv = (m > 1.0e-30f) ? 1.f / m : 0.f;
+        const float inv = (m2 > 0.f) ? r : 0.f;
+        const float m = m2 * inv;
         a.a0 = k.hs * (m - a.g0 * a.g0 * inv);
         a.a1 = k.hs * (m - a.g1 * a.g1 * inv);
         a.a2 = -k.hs * a.g0 * a.g1 * inv;
@@ -485,7 +489,6 @@ __device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
     return a;
 }
 
-// The four halo columns of an aligned strip for the pair layout (Halo2): two loads per row.
 // Two-ring halo of an aligned strip (two-row stencils): load A holds columns c0-2 and c0-1, load B
 // columns c0+64 and c0+65, in the lane layout of halo_lane_lp.
 template <typename ST>
