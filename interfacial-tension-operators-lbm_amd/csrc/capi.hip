@@ -687,40 +687,55 @@ int group_tune(lbm_ctx* c, Group& g)
         HIPCHK(c, hipEventCreate(&c->ev[1]));
     }
     const bool pipelined = c->cases[g.members[0]].elem != 8 && (g.kind == KIND_P || g.kind == KIND_F3);
-    const int depths[] = {1, 2, 3};
-    const int n_depth = pipelined ? 3 : 1;
-    const int cand[] = {1, 2, 4, 8, 16, 32, 64};
-    int best = -1, best_depth = 0;
-    float best_ms = 0.f;
-    for (int di = 0; di < n_depth; ++di) {
-        g.depth = pipelined ? depths[di] : 0;
+    struct Cand {
+        int ly, depth;
+        float ms;
+    };
+    std::vector<Cand> cands;
+    for (int depth : {1, 2, 3}) {
+        if (!pipelined && depth > 1) break;
         int last_eff = -1;
-        for (int ly : cand) {
+        for (int ly : {1, 2, 4, 8, 16, 32, 64}) {
             const int eff = std::min(ly, ny_max);
             if (eff == last_eff) continue;
             last_eff = eff;
-            g.ly.assign(g.members.size(), ly);
-            g.written = false;
-            if (int r = group_sync(c, g)) return r;
-            const int parity = g.k & 1;
-            if (int r = launch_group(c, g, parity, 0)) return r;   // warm-up
-            HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-            for (int k = 0; k < 3; ++k)
-                if (int r = launch_group(c, g, parity, 0)) return r;
-            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-            HIPCHK(c, hipEventSynchronize(c->ev[1]));
-            float ms = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-            if (best < 0 || ms < best_ms) {
-                best = ly;
-                best_depth = g.depth;
-                best_ms = ms;
-            }
+            cands.push_back({ly, pipelined ? depth : 0, 0.f});
         }
     }
-    g.ly.assign(g.members.size(), best);
-    g.depth = best_depth;
+    // average launch time of a candidate over `reps` back-to-back launches after one warm-up (the
+    // launches re-read the same input; a state smaller than the last-level cache is then partly
+    // served from it, which shifts all candidates alike)
+    auto time_of = [&](const Cand& cd, int reps, float& ms) -> int {
+        g.depth = cd.depth;
+        g.ly.assign(g.members.size(), cd.ly);
+        g.written = false;
+        if (int r = group_sync(c, g)) return r;
+        const int parity = g.k & 1;
+        if (int r = launch_group(c, g, parity, 0)) return r;
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        for (int k = 0; k < reps; ++k)
+            if (int r = launch_group(c, g, parity, 0)) return r;
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        ms /= (float)reps;
+        return LBM_OK;
+    };
+    // a first pass over every candidate, then the three fastest timed again, longer
+    for (Cand& cd : cands)
+        if (int r = time_of(cd, 3, cd.ms)) return r;
+    std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+    const size_t top = std::min<size_t>(3, cands.size());
+    for (size_t j = 0; j < top; ++j)
+        if (int r = time_of(cands[j], 8, cands[j].ms)) return r;
+    std::sort(cands.begin(), cands.begin() + top, [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+    g.ly.assign(g.members.size(), cands[0].ly);
+    g.depth = cands[0].depth;
     g.written = false;
+    if (std::getenv("ITOLBM_TUNE_LOG"))
+        std::fprintf(stderr, "itolbm tuner: family %d, %zu cases, element %d B: %d rows per segment, depth %d, %.1f us\n",
+                     g.kind, g.members.size(), c->cases[g.members[0]].elem, cands[0].ly, cands[0].depth,
+                     1000.0 * cands[0].ms);
     return group_sync(c, g);
 }
 
