@@ -23,7 +23,6 @@ using namespace itolbm;
 namespace {
 
 constexpr double PI = 3.14159265358979323846;
-constexpr double CS2 = 1. / 3.;
 constexpr int DIAG_THREADS = 256;
 constexpr int N_ANG = 256;
 
@@ -66,7 +65,7 @@ struct Group {
     int cap = 0;                   // entries dtab can hold
     bool written = false;
     bool tuned = false;            // rows per segment chosen by timing (group_tune)
-    int depth = 0;                 // pipeline depth of the FP32/FP16 one-row-stencil kernels (0 = default)
+    int depth = 0;                 // pipeline depth of the FP32/FP16 kernels (0 = default)
     long long version = 0;         // bumped whenever the table is rewritten
 };
 
@@ -670,7 +669,7 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
 int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_group(c, g, parity, record, c->stream); }
 
 // Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel)
-// and, for the FP32/FP16 one-row-stencil kernels, the pipeline depth (rows in flight per wavefront;
+// and, for the FP32/FP16 kernels, the pipeline depth (rows in flight per wavefront;
 // deeper pipelines hold more registers, so fewer wavefronts fit a SIMD).  Each candidate's launch is
 // timed at the group's current parity: it reads the current state and writes the other buffer, so
 // repeating it leaves the simulation unchanged, and the results of a step do not depend on the
@@ -679,14 +678,17 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_g
 int group_tune(lbm_ctx* c, Group& g)
 {
     g.tuned = true;
-    if (std::getenv("ITOLBM_LY")) return LBM_OK;
+    if (std::getenv("ITOLBM_LY")) {   // a fixed shape; ITOLBM_DEPTH fixes the depth too (0 = default)
+        if (const char* e = std::getenv("ITOLBM_DEPTH")) g.depth = std::max(0, std::min(3, std::atoi(e)));
+        return LBM_OK;
+    }
     int ny_max = 1;
     for (int id : g.members) ny_max = std::max(ny_max, c->cases[id].d.ny);
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
         HIPCHK(c, hipEventCreate(&c->ev[1]));
     }
-    const bool pipelined = c->cases[g.members[0]].elem != 8 && (g.kind == KIND_P || g.kind == KIND_F3);
+    const bool pipelined = c->cases[g.members[0]].elem != 8;   // the FP32/FP16 kernels of every family
     struct Cand {
         int ly, depth;
         float ms;

@@ -106,21 +106,9 @@ struct Nine {
     __device__ __forceinline__ double at(int dx, int dy) const { return v[dy + 1][dx + 1]; }
 };
 
-__device__ __forceinline__ Nine gather9(double rm, double r0, double rp, int lane)
-{
-    Nine n;
-    const double rows[3] = {rm, r0, rp};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        n.v[k][0] = shfl(rows[k], lane - 1);
-        n.v[k][1] = rows[k];
-        n.v[k][2] = shfl(rows[k], lane + 1);
-    }
-    return n;
-}
-
-// As gather9, with the columns left of lane 0 and right of lane 63 taken from the strip's halo
-// values (hl, hr per row), for aligned 64-column strips.
+// 3 x 3 neighbourhood of each lane's site from the lanes of rows y-1, y, y+1 (a row per lane), with the
+// columns left of lane 0 and right of lane 63 taken from the strip's halo values (hl, hr per row), for
+// aligned 64-column strips.
 __device__ __forceinline__ Nine gather9h(const double r[3], const double hl[3], const double hr[3], int lane)
 {
     Nine n;
@@ -409,10 +397,7 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     const Halo1 h = make_halo1(nx, t.strip);
     // every lane loads (lanes >= 38 re-read an in-range value that no one uses): no branch
     auto halo_load = [&](int y, double& vr, double& vb) {
-        int yy = y + h.dy;
-        yy = (yy < 0) ? yy + ny : yy;
-        yy = (yy >= ny) ? yy - ny : yy;
-        const unsigned long long e = elem_index(nx, yy, 0, h.i, h.x);
+        const unsigned long long e = elem_index(nx, wrap_row3(y, h.dy, ny), 0, h.i, h.x);
         vr = t.in[e];
         vb = t.in[e + (unsigned)(Q * nx)];
     };
@@ -539,8 +524,8 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
     // colour field of a row: own lanes, and the four halo columns (c0-2, c0-1, c0+64, c0+65)
     auto phi_row = [&](int y, double hw[4]) {
         load_row(t.in, nx, ny, y, L, fR, fB);
-        const unsigned long long e0 = elem_index(nx, wrap_row(y + h.dy[0], ny), 0, h.i[0], h.x[0]);
-        const unsigned long long e1 = elem_index(nx, wrap_row(y + h.dy[1], ny), 0, h.i[1], h.x[1]);
+        const unsigned long long e0 = elem_index(nx, wrap_row3(y, h.dy[0], ny), 0, h.i[0], h.x[0]);
+        const unsigned long long e1 = elem_index(nx, wrap_row3(y, h.dy[1], ny), 0, h.i[1], h.x[1]);
         const unsigned cb = (unsigned)(Q * nx);
         const double ra = t.in[e0], ba = t.in[e0 + cb], rb = t.in[e1], bb = t.in[e1 + cb];
 #pragma unroll

@@ -26,7 +26,7 @@ struct StepLaunch {
     const int* tile_case;    // tile -> case index
     int parity;              // which of pop[0] / pop[1] holds the input state
     int record;              // write aux for cases that have it
-    int depth;               // FP32/FP16 one-row stencils: rows in flight ahead (0 = default)
+    int depth;               // FP32/FP16 kernels: rows in flight ahead (0 = default)
 };
 
 int step_halo(int op);

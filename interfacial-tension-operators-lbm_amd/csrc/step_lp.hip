@@ -100,14 +100,18 @@ struct Row {
     float e[Q];   // colour fraction
 };
 
-// Issue the pulls of site (col, y): raw pair i from row y - c_iy, column col - c_ix.
+// Issue the pulls of site (col, y): raw pair i from row y - c_iy, column col - c_ix.  With live =
+// false the loads are still issued but through empty resources: they read no memory and return 0.
+// A row loop that issues its look-ahead loads on every path has the same loads outstanding on each,
+// so the compiler's waits for the current row (vector memory counts retire in issue order) do not
+// also wait for the rows ahead.
 template <typename ST>
 __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restrict__ in, int nx, int ny, int y,
-                                         const Lane& L, typename PairOf<ST>::raw r[Q])
+                                         const Lane& L, typename PairOf<ST>::raw r[Q], bool live = true)
 {
     using R = typename PairOf<ST>::raw;
     const long long pitch = (long long)Q * nx;
-    const unsigned bytes = (unsigned)(pitch * (long long)sizeof(R));
+    const unsigned bytes = live ? (unsigned)(pitch * (long long)sizeof(R)) : 0u;
     const rsrc_t rowm = make_rsrc(in + wrapi(y - 1, ny) * pitch, bytes);
     const rsrc_t row0 = make_rsrc(in + wrapi(y, ny) * pitch, bytes);
     const rsrc_t rowp = make_rsrc(in + wrapi(y + 1, ny) * pitch, bytes);
@@ -274,21 +278,9 @@ struct Nine {
     __device__ __forceinline__ float at(int dx, int dy) const { return v[dy + 1][dx + 1]; }
 };
 
-__device__ __forceinline__ Nine gather9(float rm, float r0, float rp, int lane)
-{
-    Nine n;
-    const float rows[3] = {rm, r0, rp};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        n.v[k][0] = shf(rows[k], lane - 1);
-        n.v[k][1] = rows[k];
-        n.v[k][2] = shf(rows[k], lane + 1);
-    }
-    return n;
-}
-
-// As gather9, with the columns left of lane 0 and right of lane 63 taken from the strip's halo
-// values (hl, hr per row), for aligned 64-column strips.
+// 3 x 3 neighbourhood of each lane's site from the lanes of rows y-1, y, y+1 (a row per lane), with the
+// columns left of lane 0 and right of lane 63 taken from the strip's halo values (hl, hr per row), for
+// aligned 64-column strips.
 __device__ __forceinline__ Nine gather9h(float rm, float r0, float rp, const float hl[3], const float hr[3], int lane)
 {
     Nine n;
@@ -337,11 +329,58 @@ __device__ __forceinline__ ConstF to_float(const CaseConst& c)
 
 // Collision of one site given its pulled row, phi, the mediator gradient gm = -grad(phi) and the
 // force (class F); writes the post-collision s_i and e_i through out(i, s_i, e_i).
+// c_i . (x, y, z) for a velocity index that is a compile-time constant (unrolled loops): only the
+// velocity's nonzero components enter (a product with a zero component cannot be folded away in
+// IEEE arithmetic), added in x, y, z order, so the value is that of sum_a c_ia v_a rounded once.
+__device__ __forceinline__ float cdot3(int i, float x, float y, float z)
+{
+    float r = 0.f;
+    bool any = false;
+    auto add = [&](int c, float v) {
+        if (c == 0) return;
+        const float t = c > 0 ? v : -v;
+        r = any ? r + t : t;
+        any = true;
+    };
+    add(CX[i], x);
+    add(CY[i], y);
+    add(CZ[i], z);
+    return r;
+}
+// the in-plane part c_ix x + c_iy y (fields with no z component)
+__device__ __forceinline__ float cdot2(int i, float x, float y)
+{
+    float r = 0.f;
+    bool any = false;
+    auto add = [&](int c, float v) {
+        if (c == 0) return;
+        const float t = c > 0 ? v : -v;
+        r = any ? r + t : t;
+        any = true;
+    };
+    add(CX[i], x);
+    add(CY[i], y);
+    return r;
+}
+// z moment sum_i c_iz v_i as a sum of z-mirror pair differences (i and i' differ in the sign of c_z
+// only): exactly 0 when the values are mirror-symmetric
+__device__ __forceinline__ float zmoment(const float v[Q])
+{
+    static_assert(CZ[5] == 1 && CZ[6] == -1 && CZ[11] == 1 && CZ[13] == -1 && CZ[14] == 1 && CZ[12] == -1 &&
+                      CZ[16] == 1 && CZ[18] == -1 && CZ[17] == 1 && CZ[15] == -1,
+                  "z-mirror pairs of the D3Q19 ordering");
+    static_assert(CX[11] == CX[13] && CX[14] == CX[12] && CY[16] == CY[18] && CY[17] == CY[15], "mirror pairs");
+    return (((v[5] - v[6]) + (v[11] - v[13])) + ((v[14] - v[12]) + (v[16] - v[18]))) + (v[17] - v[15]);
+}
+
 template <bool FORCED, class Out>
 __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float ds, float rho, double phid, float gx,
                                            float gy, float Fx, float Fy, Out out)
 {
-    // momentum (sum_i c_i w_i = 0, so it is the momentum of the deviations)
+    // momentum (sum_i c_i w_i = 0, so it is the momentum of the deviations).  The z component is 0
+    // in every state that is mirror-symmetric in z (all states the programs create: one node in z,
+    // so gradients and forces have no z component); it is kept for uploaded states that are not, and
+    // summed over z-mirror pairs, so that it is exactly 0 in a symmetric state.
     float jx = 0.f, jy = 0.f;
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
@@ -350,22 +389,24 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
         if (CY[i] > 0) jy += r.s[i];
         if (CY[i] < 0) jy -= r.s[i];
     }
+    const float jz = zmoment(r.s);
     const float irho = __builtin_amdgcn_rcpf(rho);
     float ux = jx * irho, uy = jy * irho;
+    const float uz = jz * irho;
     if constexpr (FORCED) {
         ux += 0.5f * Fx * irho;
         uy += 0.5f * Fy * irho;
     }
-    const float usq = ux * ux + uy * uy;
+    const float usq = ux * ux + uy * uy + uz * uz;
     const float vF = ux * Fx + uy * Fy;
     float s[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const float cu = (float)CX[i] * ux + (float)CY[i] * uy;
+        const float cu = cdot3(i, ux, uy, uz);
         const float seq = WF[i] * (ds + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
         s[i] = r.s[i] + (seq - r.s[i]) * k.oot;
         if constexpr (FORCED) {
-            const float cF = (float)CX[i] * Fx + (float)CY[i] * Fy;
+            const float cF = cdot2(i, Fx, Fy);
             const float sf = (i == 0) ? k.sf0 : (i < 7 ? k.sf1 : k.sf2);
             s[i] += sf * (cF + 3.f * cF * cu - vF);
         }
@@ -380,7 +421,7 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
             const float Am = k.A * mod2 * imod;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const float cn = (float)CX[i] * nx_ + (float)CY[i] * ny_;
+                const float cn = cdot2(i, nx_, ny_);
                 const float B = (i == 0) ? k.B0 : (i < 7 ? k.B1 : k.B2);
                 s[i] += Am * (WF[i] * cn * cn - B);
             }
@@ -390,7 +431,7 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
     // the FP32 weights the second moment 3 sum_i w_i c_ix^2 is 1 - 6e-8 (1/3 has no binary
     // representation), so the equilibrium lost a fraction 6e-8 / tau of the momentum every step
     // (3e-3 of a translating drop's velocity over 40000 steps) and gained mass in proportion to
-    // u^2.  The residuals go to the rest population (mass) and to the +-x, +-y pairs (momentum).
+    // u^2.  The residuals go to the rest population (mass) and to the +-x, +-y, +-z pairs (momentum).
     {
         float m = 0.f, px = 0.f, py = 0.f;
 #pragma unroll
@@ -401,13 +442,17 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
             if (CY[i] > 0) py += s[i];
             if (CY[i] < 0) py -= s[i];
         }
+        const float pz = zmoment(s);
         const float dx = 0.5f * ((FORCED ? jx + Fx : jx) - px);
         const float dy = 0.5f * ((FORCED ? jy + Fy : jy) - py);
+        const float dz = 0.5f * (jz - pz);
         s[0] += ds - m;
         s[1] += dx;
         s[2] -= dx;
         s[3] += dy;
         s[4] -= dy;
+        s[5] += dz;
+        s[6] -= dz;
     }
     // recolouring: e_i = phi - 2 beta cR cB w_i rho cos_i / f_i, the difference taken in FP64 and
     // rounded once
@@ -419,7 +464,7 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
         float e = phi;
         if (i > 0) {   // without a gradient n = 0, so the correction is 0 (no branch per lane)
             const float f = s[i] + WF[i];
-            const float cn = (float)CX[i] * nx_ + (float)CY[i] * ny_;
+            const float cn = cdot2(i, nx_, ny_);
             const float cosp = (i < 7) ? cn : cn * ISQ2;
             const float corr = kk * WF[i] * cosp * __builtin_amdgcn_rcpf(f);
             e = (float)(phid - (double)((f != 0.f) ? corr : 0.f));
@@ -509,8 +554,8 @@ struct Halo2LP {
     __device__ __forceinline__ void load(gcst<R>* in, int nx, int ny, int y, R& a, R& b) const
     {
         const unsigned long long pitch = (unsigned long long)Q * (unsigned)nx;
-        a = in[(unsigned long long)wrap_row(y + dy[0], ny) * pitch + off[0]];
-        b = in[(unsigned long long)wrap_row(y + dy[1], ny) * pitch + off[1]];
+        a = in[(unsigned long long)wrap_row3(y, dy[0], ny) * pitch + off[0]];
+        b = in[(unsigned long long)wrap_row3(y, dy[1], ny) * pitch + off[1]];
     }
 };
 
@@ -523,15 +568,51 @@ __device__ __forceinline__ void halo2_phase(typename PairOf<ST>::raw va, typenam
     halo_pair_phase<ST>(vb, rf, hw[2], hw[3]);
 }
 
+// Collision of one site row of an aligned strip and the stores of its 19 pairs (lanes inside the
+// case; a site without fluid, rho <= 1e-30, is stored unchanged), and the aux field on record steps.
+template <typename ST, bool FORCED>
+__device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
+                                              gst<typename PairOf<ST>::raw>* out, int y,
+                                              const typename PairOf<ST>::raw (&cur)[Q], double phi, float gx,
+                                              float gy, float Fx, float Fy)
+{
+    using R = typename PairOf<ST>::raw;
+    if (!L.interior) return;
+    const int nx = t.cc.nx;
+    Row row;
+    decode<ST>(cur, row, rf);
+    float ds, rho;
+    row_density(row, ds, rho);
+    const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
+    const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
+    auto st = [&](int i, float s, float e) {
+        pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R)));
+    };
+    if (rho > 1.0e-30f) {
+        collide_lp<FORCED>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
+    }
+    if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
+}
+
 // Aligned 64-column strip kernel for the two-row stencils (F1, F2): rows y-2 .. y+2 of the colour
 // field are kept for the own lanes (P) and the four halo columns (HW); the gradient and the n / T
 // field at the halo columns c0-1 and c0+64 are evaluated on the wavefront's uniform values with the
-// same functions as every lane's.  With FP32 pairs the loads of row y + 3 are issued while row y is
-// worked on.  Row y's populations are read again (from L2) for its collision once its force is known.
-template <typename ST, int KIND>
+// same functions as every lane's.  Row y's populations are read again (from L2) for its collision once
+// its force is known.  DEPTH sets the loads in flight: 1 loads row y + 2 in the iteration that needs
+// it; 2 and 3 issue its loads 1 and 2 iterations ahead (rotating buffers, the row loop unrolled so
+// every buffer role is a compile-time constant).  The re-read of row y is issued before the wait on
+// row y + 2, so that the collision's wait leaves the rows ahead in flight (vector memory counts
+// retire in issue order).
+template <typename ST, int KIND, int DEPTH>
 __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
 {
     static_assert(KIND == KIND_F1 || KIND == KIND_F2, "two-row stencils only");
+    static_assert(DEPTH >= 1 && DEPTH <= 3, "pipeline depth 1..3");
+    constexpr int NA = DEPTH - 1;         // rows loaded ahead of the iteration that uses them
+    constexpr int NU = NA > 0 ? NA : 1;   // unroll of the row loop
     using R = typename PairOf<ST>::raw;
     TileCtxLP t;
     if (!tile_ctx(launch, t)) return;
@@ -545,8 +626,9 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     h.init(nx, t.strip);
     const int y0 = t.y0, y1 = t.y1;
 
-    R cur[Q], nxt[Q];
-    R hva, hvb;
+    R cur[Q];
+    R ahead[NU][Q];   // row r + 2 in buffer (r - y0) mod NU (DEPTH > 1)
+    R ha[NU], hb[NU];
     float P[5], HW[5][4];
     double pd[3];   // FP64 colour field of rows y, y+1, y+2 (for their collision)
     auto phi_row = [&](int y, float (&hw)[4]) {
@@ -584,79 +666,73 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     pd[2] = phi_row(y0 + 1, HW[3]);
     P[3] = (float)pd[2];
     AuxF am = aux_lanes(0), a0 = aux_lanes(1);
-    // FP32 pairs: row y + 2's loads are issued one iteration ahead (measured faster); FP16 pairs
-    // load it in the iteration, keeping registers for more wavefronts per SIMD
-    constexpr bool AHEAD = sizeof(R) == 8;
-    if constexpr (AHEAD) {
-        load_raw<ST>(in, nx, ny, y0 + 2, L, nxt);
-        h.load(in, nx, ny, y0 + 2, hva, hvb);
+    // n / T at the halo columns of rows y-1 and y (carried down the rows; row y+1's is new each row)
+    AuxF hl[3], hr[3];
+    hl[0] = aux_halo(0, 0); hr[0] = aux_halo(0, 1);
+    hl[1] = aux_halo(1, 0); hr[1] = aux_halo(1, 1);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+        load_raw<ST>(in, nx, ny, y0 + 2 + r, L, ahead[r]);
+        h.load(in, nx, ny, y0 + 2 + r, ha[r], hb[r]);
     }
 
-    for (int y = y0; y < y1; ++y) {
-        pd[0] = pd[1];
-        pd[1] = pd[2];
-        if constexpr (AHEAD) {
-            pd[2] = raw_phase<ST>(nxt, rf);   // row y + 2 (waits for its loads)
-            halo2_phase<ST>(hva, hvb, rf, HW[4]);
-            if (y + 1 < y1) {   // the next iteration's row y + 3
-                load_raw<ST>(in, nx, ny, y + 3, L, nxt);
-                h.load(in, nx, ny, y + 3, hva, hvb);
-            }
-        } else {
-            pd[2] = phi_row(y + 2, HW[4]);
-        }
-        P[4] = (float)pd[2];
-        load_raw<ST>(in, nx, ny, y, L, cur);   // row y again (from L2), for its collision
-        const AuxF ap = aux_lanes(2);
-        AuxF hl[3], hr[3];
+    for (int y = y0; y < y1; y += NU) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            hl[r] = aux_halo(r, 0);
-            hr[r] = aux_halo(r, 1);
-        }
-        auto field = [&](float AuxF::*m) {
-            const float l3[3] = {hl[0].*m, hl[1].*m, hl[2].*m};
-            const float r3[3] = {hr[0].*m, hr[1].*m, hr[2].*m};
-            return gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
-        };
-        float Fx, Fy;
-        if constexpr (KIND == KIND_F1) {
-            const float divn = dx_iso(field(&AuxF::a0)) + dy_iso(field(&AuxF::a1));
-            Fx = -k.hs * divn * a0.g0;
-            Fy = -k.hs * divn * a0.g1;
-        } else {
-            const Nine txy = field(&AuxF::a2);
-            Fx = dx_iso(field(&AuxF::a0)) + dy_iso(txy);
-            Fy = dx_iso(txy) + dy_iso(field(&AuxF::a1));
-        }
-        const float hl1[3] = {HW[1][1], HW[2][1], HW[3][1]};
-        const float hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
-        const Nine p9 = gather9h(P[1], P[2], P[3], hl1, hr0, L.lane);
-        const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
-        if (L.interior) {
-            Row row;
-            decode<ST>(cur, row, rf);
-            float ds, rho;
-            row_density(row, ds, rho);
-            const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
-            const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-            auto st = [&](int i, float s, float e) { pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
-            if (rho > 1.0e-30f) {
-                collide_lp<true>(k, row, ds, rho, pd[0], gx, gy, Fx, Fy, st);
+        for (int j = 0; j < NU; ++j) {
+            const int yy = y + j;
+            if (yy >= y1) break;
+            pd[0] = pd[1];
+            pd[1] = pd[2];
+            if constexpr (NA > 0) {
+                load_raw<ST>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
+                __builtin_amdgcn_sched_barrier(0);      // issued before the wait on row yy + 2
+                pd[2] = raw_phase<ST>(ahead[j], rf);    // row yy + 2 (waits for its loads)
+                halo2_phase<ST>(ha[j], hb[j], rf, HW[4]);
+                // row yy + 2 + NA, used NA iterations on; past the segment's last row the loads are
+                // issued empty, so every path leaves the same loads outstanding
+                const bool live = yy + NA < y1;
+                load_raw<ST>(in, nx, ny, yy + 2 + NA, L, ahead[j], live);
+                h.load(in, nx, ny, live ? yy + 2 + NA : yy, ha[j], hb[j]);
+                __builtin_amdgcn_sched_barrier(0);
             } else {
-#pragma unroll
-                for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
+                pd[2] = phi_row(yy + 2, HW[4]);
+                load_raw<ST>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
             }
-            if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
-        }
+            P[4] = (float)pd[2];
+            const AuxF ap = aux_lanes(2);
+            hl[2] = aux_halo(2, 0);
+            hr[2] = aux_halo(2, 1);
+            auto field = [&](float AuxF::*m) {
+                const float l3[3] = {hl[0].*m, hl[1].*m, hl[2].*m};
+                const float r3[3] = {hr[0].*m, hr[1].*m, hr[2].*m};
+                return gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
+            };
+            float Fx, Fy;
+            if constexpr (KIND == KIND_F1) {
+                const float divn = dx_iso(field(&AuxF::a0)) + dy_iso(field(&AuxF::a1));
+                Fx = -k.hs * divn * a0.g0;
+                Fy = -k.hs * divn * a0.g1;
+            } else {
+                const Nine txy = field(&AuxF::a2);
+                Fx = dx_iso(field(&AuxF::a0)) + dy_iso(txy);
+                Fy = dx_iso(txy) + dy_iso(field(&AuxF::a1));
+            }
+            const float hl1[3] = {HW[1][1], HW[2][1], HW[3][1]};
+            const float hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
+            const Nine p9 = gather9h(P[1], P[2], P[3], hl1, hr0, L.lane);
+            const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
+            collide_store<ST, true>(t, k, rf, L, out, yy, cur, pd[0], gx, gy, Fx, Fy);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            P[r] = P[r + 1];
+            for (int r = 0; r < 4; ++r) {
+                P[r] = P[r + 1];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) HW[r][q] = HW[r + 1][q];
+                for (int q = 0; q < 4; ++q) HW[r][q] = HW[r + 1][q];
+            }
+            am = a0;
+            a0 = ap;
+            hl[0] = hl[1]; hl[1] = hl[2];
+            hr[0] = hr[1]; hr[1] = hr[2];
         }
-        am = a0;
-        a0 = ap;
     }
 }
 
@@ -678,9 +754,7 @@ struct HaloLP {
     }
     __device__ __forceinline__ R load(gcst<R>* in, int nx, int ny, int y) const
     {
-        int yy = y + dy;
-        yy = (yy < 0) ? yy + ny : yy;
-        yy = (yy >= ny) ? yy - ny : yy;
+        const int yy = wrap_row3(y, dy, ny);
         // every lane loads (the unused lanes re-read an in-range value): no branch
         return in[(unsigned long long)yy * ((unsigned long long)Q * (unsigned)nx) + off];
     }
@@ -694,8 +768,6 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
                                             const typename PairOf<ST>::raw (&cur)[Q], const float (&p)[3],
                                             const float (&hl)[3], const float (&hr)[3], double phi)
 {
-    using R = typename PairOf<ST>::raw;
-    const int nx = t.cc.nx;
     const Nine p9 = gather9h(p[0], p[1], p[2], hl, hr, L.lane);
     const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
     float Fx = 0.f, Fy = 0.f;
@@ -705,22 +777,7 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
         Fx = -mu * gx;
         Fy = -mu * gy;
     }
-    if (L.interior) {
-        Row row;
-        decode<ST>(cur, row, rf);
-        float ds, rho;
-        row_density(row, ds, rho);
-        const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
-        const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
-        auto st = [&](int i, float s, float e) { pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R))); };
-        if (rho > 1.0e-30f) {
-            collide_lp<KIND != KIND_P>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
-        } else {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
-        }
-        if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
-    }
+    collide_store<ST, KIND != KIND_P>(t, k, rf, L, out, y, cur, phi, gx, gy, Fx, Fy);
 }
 
 __device__ __forceinline__ void opaque(f32x2& v)
@@ -780,10 +837,12 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
                 R(&cur)[Q] = buf[j];
                 R(&nxt)[Q] = buf[(j + 1) % NB];
                 R(&far)[Q] = buf[(j + DEPTH) % NB];   // held row yy - 1, free now
-                if (yy + DEPTH <= y1) {   // rows up to y1 feed the gradients
-                    load_raw<ST>(in, nx, ny, yy + DEPTH, L, far);
-                    hv[(j + DEPTH) % NB] = h.load(in, nx, ny, yy + DEPTH);
-                }
+                // rows up to y1 feed the gradients; past it the loads are issued empty, so every
+                // path leaves the same loads outstanding
+                const bool live = yy + DEPTH <= y1;
+                load_raw<ST>(in, nx, ny, yy + DEPTH, L, far, live);
+                hv[(j + DEPTH) % NB] = h.load(in, nx, ny, live ? yy + DEPTH : yy);
+                __builtin_amdgcn_sched_barrier(0);
                 p[0] = p[1]; p[1] = p[2];
                 hl[0] = hl[1]; hl[1] = hl[2];
                 hr[0] = hr[1]; hr[1] = hr[2];
@@ -810,18 +869,27 @@ void launch_pipe(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
     else hipLaunchKernelGGL((lp_pipe<ST, KIND, 3>), grid, block, 0, stream, a);
 }
 
+template <typename ST, int KIND>
+void launch_f12(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
+{
+    const dim3 block(64);
+    if (depth <= 1) hipLaunchKernelGGL((lp_f12<ST, KIND, 1>), grid, block, 0, stream, a);
+    else if (depth == 2) hipLaunchKernelGGL((lp_f12<ST, KIND, 2>), grid, block, 0, stream, a);
+    else hipLaunchKernelGGL((lp_f12<ST, KIND, 3>), grid, block, 0, stream, a);
+}
+
 template <typename ST>
 hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
 {
-    const dim3 grid(ntiles), block(64);
+    const dim3 grid(ntiles);
     // pipeline depth: the launch's (chosen by the tuner); 0 = default, 2 rows for FP32 storage and 1
     // for FP16 storage (measured best on MI355X without tuning)
     const int depth = a.depth > 0 ? a.depth : (sizeof(typename PairOf<ST>::raw) == 8 ? 2 : 1);
     switch (kind) {
     case KIND_P: launch_pipe<ST, KIND_P>(depth, a, grid, stream); break;
     case KIND_F3: launch_pipe<ST, KIND_F3>(depth, a, grid, stream); break;
-    case KIND_F1: hipLaunchKernelGGL((lp_f12<ST, KIND_F1>), grid, block, 0, stream, a); break;
-    case KIND_F2: hipLaunchKernelGGL((lp_f12<ST, KIND_F2>), grid, block, 0, stream, a); break;
+    case KIND_F1: launch_f12<ST, KIND_F1>(depth, a, grid, stream); break;
+    case KIND_F2: launch_f12<ST, KIND_F2>(depth, a, grid, stream); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -117,10 +117,12 @@ __device__ __forceinline__ Halo2 make_halo2(int nx, int strip)
     return h;
 }
 
-__device__ __forceinline__ int wrap_row(int y, int ny)
+// Row y + dy (dy in {-1, 0, 1}, per lane) wrapped periodically, for any ny >= 1 and any y: the three
+// wrapped rows are uniform values (scalar arithmetic) and each lane selects its own.
+__device__ __forceinline__ int wrap_row3(int y, int dy, int ny)
 {
-    y = (y < 0) ? y + ny : y;
-    return (y >= ny) ? y - ny : y;
+    const int ym = wrapi(y - 1, ny), yc = wrapi(y, ny), yp = wrapi(y + 1, ny);
+    return dy < 0 ? ym : (dy > 0 ? yp : yc);
 }
 
 // ------------------------------------------------------------------ buffer-resource access

@@ -88,10 +88,16 @@ def test_step_bitwise_band(op, A):
     ctx.close()
 
 
-def test_tiny_box_wraps():
-    # a box narrower than one wavefront strip and shorter than a row segment
-    say("[gpu] step parity tiny 5x3 F2")
-    m, sim, ctx = pair(5, 3, "F2", 0.1)
+TINY = [(5, 3, "F2", 0.1), (7, 1, "F1", 0.1), (6, 2, "F2", 0.1), (9, 2, "P3 0.5", 0.225), (4, 1, "F3 1.569988", 0.1),
+        (3, 4, "F1", 0.1)]
+
+
+@pytest.mark.parametrize("nx,ny,op,A", TINY)
+def test_tiny_box_wraps(nx, ny, op, A):
+    # a box narrower than one wavefront strip and shorter than a row segment; with ny < 5 the rows of
+    # the two-row stencils and of the look-ahead loads wrap more than once
+    say(f"[gpu] step parity tiny {nx}x{ny} {op}")
+    m, sim, ctx = pair(nx, ny, op, A)
     rng = np.random.default_rng(3)
     sim.fR[:] = rng.uniform(0.0, 0.05, sim.fR.size)
     sim.fB[:] = rng.uniform(0.0, 0.05, sim.fB.size)

@@ -13,7 +13,8 @@ test_gpu_parity.py) at the tolerances written in each test:
   radius <= 1e-5 (FP32), 1e-3 (FP16);
 * colour masses drift by <= 1e-5 (FP32) / 5e-4 (FP16) relative over 1000 steps (exact in FP64);
 * a pure phase stays pure: e_i = +-1 is stored exactly and the colour-weighted sums use the
-  same order as the plain sums, so f_B = 0 inside a sharp drop is kept exactly.
+  same order as the plain sums, so f_B = 0 inside a sharp drop is kept exactly;
+* random non-equilibrium states (no symmetry in z), 3 steps: max |df| <= 5e-7 (FP32), 2e-4 (FP16).
 """
 import sys
 
@@ -113,6 +114,39 @@ def test_lp_step_close_to_oracle(prec, op, A):
         assert np.abs(F - sim.F).max() <= 50 * POP_TOL[prec] * max(1.0, np.abs(sim.F).max())
     nR, nB = ctx.download(m.STATE_POST_STREAM)
     assert np.abs(nR - sim.fR).max() <= POP_TOL[prec]
+    ctx.close()
+
+
+RANDOM_BOXES = [(5, 3, "F2", 0.1), (7, 1, "F1", 0.1), (6, 2, "F2", 0.1), (9, 2, "P3 0.5", 0.225),
+                (4, 1, "F3 1.569988", 0.1), (3, 4, "F1", 0.1), (64, 40, "F2", 0.1), (70, 45, "P3 0.5", 0.225)]
+RANDOM_TOL = {1: 5e-7, 2: 2e-4}
+
+
+@pytest.mark.parametrize("prec", [1, 2], ids=["f32", "f16"])
+@pytest.mark.parametrize("nx,ny,op,A", RANDOM_BOXES)
+def test_lp_random_states(prec, nx, ny, op, A):
+    """Populations within 10 % of the weights, independently random per direction, with a random colour:
+    no symmetry in z (momentum in z), strong gradients everywhere.  Boxes narrower than a strip and
+    shorter than the stencils (rows wrap more than once) and ordinary ones.  3 steps against the oracle:
+    max |df| <= 5e-7 (FP32), 2e-4 (FP16)."""
+    say(f"[gpu] precision {prec} random state {nx}x{ny} {op}")
+    m = _pkg.load()
+    sim = O.Sim(O.make_case(nx, ny, op, 0.8, A, 0.8))
+    rng = np.random.default_rng(7)
+    w = np.tile(np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12), nx * ny)
+    f = w * rng.uniform(0.9, 1.1, w.size)
+    c = rng.uniform(0.0, 1.0, w.size)
+    sim.fR[:] = f * c
+    sim.fB[:] = f * (1.0 - c)
+    ctx = m.Context(m.case_desc(nx, ny, op, 0.8, A, 0.8), precision=prec)
+    ctx.upload(sim.fR, sim.fB)
+    for _ in range(3):
+        sim.step()
+    ctx.step(3)
+    gR, gB = ctx.download(m.STATE_POST_COLLISION)
+    err = max(np.abs(gR - sim.post_R).max(), np.abs(gB - sim.post_B).max())
+    say(f"    max |df| = {err:.3e}")
+    assert err <= RANDOM_TOL[prec]
     ctx.close()
 
 
