@@ -63,6 +63,7 @@ struct Group {
     int k = 0;                     // launches since the table was written
     int ntiles = 0;
     int cap = 0;                   // entries dtab can hold
+    int tcap = 0;                  // entries dtile can hold
     bool written = false;
     bool tuned = false;            // rows per segment chosen by timing (group_tune)
     int depth = 0;                 // pipeline depth of the FP32/FP16 kernels (0 = default)
@@ -632,17 +633,20 @@ int group_sync(lbm_ctx* c, Group& g)
     if (g.dtab && nm > g.cap) {
         (void)hipFree(g.dtab);
         g.dtab = nullptr;
+        c->bytes -= (long long)sizeof(CaseDev) * g.cap;
     }
     if (!g.dtab) {
         if (int r = dev_alloc(c, (void**)&g.dtab, sizeof(CaseDev) * nm)) return r;
         g.cap = nm;
     }
-    if (g.dtile && tile0 > g.ntiles) {
+    if (g.dtile && tile0 > g.tcap) {
         (void)hipFree(g.dtile);
         g.dtile = nullptr;
+        c->bytes -= (long long)sizeof(int) * g.tcap;
     }
     if (!g.dtile) {
         if (int r = dev_alloc(c, (void**)&g.dtile, sizeof(int) * std::max(tile0, 1))) return r;
+        g.tcap = std::max(tile0, 1);
     }
     HIPCHK(c, hipMemcpyAsync(g.dtab, tab.data(), sizeof(CaseDev) * nm, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(g.dtile, tile_case.data(), sizeof(int) * tile0, hipMemcpyHostToDevice, c->stream));
@@ -668,6 +672,32 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
 }
 int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_group(c, g, parity, record, c->stream); }
 
+struct Cand {
+    int ly, depth;
+    float ms;
+};
+
+// Tile shapes a family's tuning tries: rows per segment (distinct after clamping to the tallest
+// member) and, for the FP32/FP16 kernels, the pipeline depth.
+std::vector<Cand> tile_candidates(const lbm_ctx* c, const Group& g)
+{
+    int ny_max = 1;
+    for (int id : g.members) ny_max = std::max(ny_max, c->cases[id].d.ny);
+    const bool pipelined = c->cases[g.members[0]].elem != 8;   // the FP32/FP16 kernels of every family
+    std::vector<Cand> cands;
+    for (int depth : {1, 2, 3}) {
+        if (!pipelined && depth > 1) break;
+        int last_eff = -1;
+        for (int ly : {1, 2, 4, 8, 16, 32, 64}) {
+            const int eff = std::min(ly, ny_max);
+            if (eff == last_eff) continue;
+            last_eff = eff;
+            cands.push_back({ly, pipelined ? depth : 0, 0.f});
+        }
+    }
+    return cands;
+}
+
 // Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel)
 // and, for the FP32/FP16 kernels, the pipeline depth (rows in flight per wavefront;
 // deeper pipelines hold more registers, so fewer wavefronts fit a SIMD).  Each candidate's launch is
@@ -682,31 +712,14 @@ int group_tune(lbm_ctx* c, Group& g)
         if (const char* e = std::getenv("ITOLBM_DEPTH")) g.depth = std::max(0, std::min(3, std::atoi(e)));
         return LBM_OK;
     }
-    int ny_max = 1;
-    for (int id : g.members) ny_max = std::max(ny_max, c->cases[id].d.ny);
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
         HIPCHK(c, hipEventCreate(&c->ev[1]));
     }
-    const bool pipelined = c->cases[g.members[0]].elem != 8;   // the FP32/FP16 kernels of every family
-    struct Cand {
-        int ly, depth;
-        float ms;
-    };
-    std::vector<Cand> cands;
-    for (int depth : {1, 2, 3}) {
-        if (!pipelined && depth > 1) break;
-        int last_eff = -1;
-        for (int ly : {1, 2, 4, 8, 16, 32, 64}) {
-            const int eff = std::min(ly, ny_max);
-            if (eff == last_eff) continue;
-            last_eff = eff;
-            cands.push_back({ly, pipelined ? depth : 0, 0.f});
-        }
-    }
+    std::vector<Cand> cands = tile_candidates(c, g);
     // average launch time of a candidate over `reps` back-to-back launches after one warm-up (the
-    // launches re-read the same input; a state smaller than the last-level cache is then partly
-    // served from it, which shifts all candidates alike)
+    // launches re-read the same input, which the last-level cache then partly serves: the times are
+    // 5-10 % below those of launches in real steps)
     auto time_of = [&](const Cand& cd, int reps, float& ms) -> int {
         g.depth = cd.depth;
         g.ly.assign(g.members.size(), cd.ly);
