@@ -8,11 +8,22 @@ F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (one laun
 family per step).  A bench step advances every case by one time step.  Inputs are synthetic
 (generated by the library's initial condition), resident in HBM before timing.
 
-The headline form (--prec, default f32) is timed over --steps steps after --warmup; the other two
-storage forms are timed the same way and reported beside it.  Also reported:
+The headline form (--prec, default f32z) is timed over --steps steps after --warmup; the other
+storage forms are timed the same way and reported beside it:
+
+  f64    FP64 compute and storage (the drop-in form, bit-equal to the reference's arithmetic)
+  f32    FP32 compute, FP32 storage of all 19 populations per site
+  f16    FP32 compute, FP16 storage of all 19
+  f32z   as f32, storing the 14 distinct populations of a state that is mirror-symmetric in z (with
+         one node in z every state the programs create is; the results are bitwise those of f32)
+  f16z   as f16, likewise
+
+Also reported:
 
   roofline       the dominant kernel (the P1/P3 family launch, 8 of the 14 cases) timed alone with HIP
-                 events on its stream: algorithmic bytes per launch / average launch duration;
+                 events on its stream: algorithmic bytes per launch (the form's stored bytes) / average
+                 launch duration; frac_at_304B = the kernel's lattice updates per second x 304 B (the
+                 full FP32 state's bytes per update) / peak, the MLUPS-based roofline fraction;
                  traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
                  (profiles/traffic_<form>.json), when present; copy_GBs = a plain 1 GiB device copy
                  (torch's copy kernel) timed the same way, the bandwidth a streaming kernel reaches
@@ -45,10 +56,18 @@ SIGMA = 0.1
 NX = NY = 512
 R = 100.0
 PEAK_HBM_GBS = 8000.0
-B_LU = {"f64": 608, "f32": 304, "f16": 152}
-PREC_ID = {"f64": 0, "f32": 1, "f16": 2}
-KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth> (P1/P3 family)",
-          "f16": "lp_pipe<_Float16,P,depth> (P1/P3 family)"}
+SWEEP_WARMUP = 320   # untimed sweep steps (the batched path tunes its tile shapes on them)
+B_LU = {"f64": 608, "f32": 304, "f16": 152, "f32z": 224, "f16z": 112}
+PREC_ID = {"f64": 0, "f32": 1, "f16": 2, "f32z": 3, "f16z": 4}
+KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth,false> (P1/P3 family)",
+          "f16": "lp_pipe<_Float16,P,depth,false> (P1/P3 family)",
+          "f32z": "lp_pipe<float,P,depth,true> (P1/P3 family)",
+          "f16z": "lp_pipe<_Float16,P,depth,true> (P1/P3 family)"}
+FORM_TEXT = {"f64": "f64 storage", "f32": "f32 storage of all 19 populations, fp32 compute",
+             "f16": "f16 storage of all 19 populations, fp32 compute",
+             "f32z": "f32 storage of the 14 distinct populations of the z-mirror-symmetric slab states "
+                     "(bitwise equal to full f32 storage), fp32 compute",
+             "f16z": "f16 storage of the 14 distinct populations of the z-mirror-symmetric slab states, fp32 compute"}
 
 
 def amplitude(kind, tau, sigma=SIGMA):
@@ -106,6 +125,7 @@ def bench_form(m, timer, stream, form, steps, warmup):
             "dominant_kernel": KERNEL[form], "kernel_cases_per_launch": nP,
             "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
             "achieved_GBs": bytes_launch / avg / 1e9, "frac": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
+            "frac_at_304B": 304 * NX * NY * nP / avg / 1e9 / PEAK_HBM_GBS,
             "bytes_per_lattice_update": B_LU[form]}
 
 
@@ -114,13 +134,14 @@ def bench_large(m, timer, stream, form, steps):
     ctx = m.Context(m.case_desc(n, n, "P3 0.5", 1.0, amplitude("P", 1.0), 0.8), precision=PREC_ID[form])
     ctx.set_stream(stream.cuda_stream)
     ctx.init_drop(1000.0, width=-1.0, v0=0.1)
-    ctx.step(3)
+    ctx.step(100)   # untimed: the library tunes the tile shape on these steps
     gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
     ctx.close()
     avg = gpu_s / steps
     return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
             "steps": steps, "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
-            "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS}
+            "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS,
+            "frac_at_304B": 304 * n * n / avg / 1e9 / PEAK_HBM_GBS}
 
 
 def copy_ceiling(torch, timer, stream, reps=10):
@@ -167,12 +188,12 @@ def bench_sweep(m, timer, stream, form, steps):
         ctx.set_stream(stream.cuda_stream)
         for i, (d, r) in enumerate(cases):
             ctx.init_drop(r, width=3.45, case=i)
-        # untimed warm-up (first launches, tile-shape tuning), the same 5 steps on both paths
+        # untimed warm-up (first launches, tile-shape tuning), the same steps on both paths
         if prec == 0:
             for i in range(len(descs)):
-                ctx.step(5, case=i)
+                ctx.step(SWEEP_WARMUP, case=i)
         else:
-            ctx.step(5)
+            ctx.step(SWEEP_WARMUP)
         ctx.sync()
         if prec == 0:
             # one launch per case and step (cases are independent, so case by case is the same work)
@@ -194,7 +215,7 @@ def bench_sweep(m, timer, stream, form, steps):
     good = (a["ok"] == 1) & (b["ok"] == 1) & np.isfinite(a["dp"]) & np.isfinite(b["dp"]) & (np.abs(a["dp"]) > 1e-9)
     err = np.abs(b["dp"][good] - a["dp"][good]) / np.abs(a["dp"][good])
     per_case_20k = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
-    return {"cases": len(descs), "sites": sites, "steps": steps, "warmup": 5, "form": form,
+    return {"cases": len(descs), "sites": sites, "steps": steps, "warmup": SWEEP_WARMUP, "form": form,
             "f64_per_case_ms_per_step": a["ms_per_step"], "batched_ms_per_step": b["ms_per_step"],
             "speedup": a["ms_per_step"] / b["ms_per_step"],
             "cases_per_hour_20k_steps": {"f64_per_case": per_case_20k(a["ms_per_step"]),
@@ -240,9 +261,10 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--prec", default="f32", choices=["f64", "f32", "f16"], help="headline storage form")
-    ap.add_argument("--forms", default="f64,f32,f16", help="forms measured beside the headline")
+    ap.add_argument("--warmup", type=int, default=400,
+                    help="untimed steps first (the library tunes its tile shapes on the first ~300 of them)")
+    ap.add_argument("--prec", default="f32z", choices=list(B_LU), help="headline storage form")
+    ap.add_argument("--forms", default="f64,f32,f16,f16z", help="forms measured beside the headline")
     ap.add_argument("--sweep-steps", type=int, default=1000, help="0 skips the 168-case sweep")
     ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 case")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -293,11 +315,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": "512x512x1 static sharp drop R=100; tau {0.6, 1.0} x 7 operators "
                                "(P1, P3 chi -0.4/0.5/2, F1, F2, F3 xi 1.569988), sigma 0.1, beta 0.8; 14 cases "
-                               f"batched; {a.prec} storage" + ("" if a.prec == "f64" else ", fp32 compute"),
+                               f"batched; {FORM_TEXT[a.prec]}",
                    "cases": 14, "sites_per_case": NX * NY, "storage": a.prec,
                    "bytes_per_lattice_update": B_LU[a.prec]},
         "roofline": {"bound": "hbm", "kernel": head["dominant_kernel"], "achieved": head["achieved_GBs"],
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": head["frac"], "traffic": traffic,
+                     "frac_at_304B": head["frac_at_304B"],
                      "algorithmic_bytes_per_launch": head["bytes_per_launch"],
                      "avg_launch_us": head["kernel_avg_launch_us"],
                      "copy_GBs": copy["GBs"], "frac_of_copy": head["achieved_GBs"] / copy["GBs"]},
@@ -305,9 +328,9 @@ def main():
     }
     if a.large_steps > 0:
         say(f"bench: 8192^2 {a.prec}, {a.large_steps} steps")
-        out["large"] = bench_large(m, timer, stream, a.prec if a.prec != "f64" else "f32", a.large_steps)
+        out["large"] = bench_large(m, timer, stream, a.prec if a.prec != "f64" else "f32z", a.large_steps)
     if a.sweep_steps > 0:
-        sform = a.prec if a.prec != "f64" else "f32"
+        sform = a.prec if a.prec != "f64" else "f32z"
         say(f"bench: 168-case sweep, {a.sweep_steps} steps, f64 per case vs {sform} batched")
         out["sweep"] = bench_sweep(m, timer, stream, sform, a.sweep_steps)
     if not a.no_cpu_baseline:

@@ -32,8 +32,16 @@ enum { LBM_OP_P1 = 1, LBM_OP_P3 = 3, LBM_OP_F1 = 11, LBM_OP_F2 = 12, LBM_OP_F3 =
 enum {
     LBM_PREC_F64 = 0, /* FP64 compute and storage: the drop-in form, bit-equal to the reference's arithmetic */
     LBM_PREC_F32 = 1, /* FP32 compute, FP32 storage of s_i / w_i and e_i (304 B per lattice update) */
-    LBM_PREC_F16 = 2  /* FP32 compute, FP16 storage of s_i / w_i and of the minority colour fraction (152 B per lattice update) */
+    LBM_PREC_F16 = 2, /* FP32 compute, FP16 storage of s_i / w_i and of the minority colour fraction (152 B per lattice update) */
+    LBM_PREC_F32_ZSYM = 3, /* as LBM_PREC_F32 for states mirror-symmetric in z: 14 of the 19 populations stored (224 B) */
+    LBM_PREC_F16_ZSYM = 4  /* as LBM_PREC_F16 for states mirror-symmetric in z: 14 of the 19 populations stored (112 B) */
 };
+/* z-symmetric storage (no counterpart in the reference, whose D3Q19 slab stores all 19): with one node
+ * in z, velocity i and its z mirror pull from the same site, and every state the reference's programs
+ * create is symmetric under z -> -z (f_6 = f_5, f_13 = f_11, f_14 = f_12, f_17 = f_15, f_18 = f_16), a
+ * symmetry the FP32/FP16 collision keeps exactly.  Only the 14 distinct populations are stored; the
+ * results equal those of LBM_PREC_F32 / LBM_PREC_F16 on the same symmetric state.  lbm_upload_state
+ * rejects (LBM_E_ARG) a state whose mirror populations differ beyond 1e-12 relative. */
 /* In the FP32/FP16 forms each population is stored as A_i = s_i / w_i, s_i = fR_i + fB_i - w_i (deviation
  * from the rest state rho0 = 1, scaled per direction by 1 / w_i) and e_i = (fR_i - fB_i) / (fR_i + fB_i)
  * (colour fraction, exactly +-1 in a pure phase).  Transfers and diagnostics convert to and from FP64. */

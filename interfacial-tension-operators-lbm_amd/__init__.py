@@ -19,6 +19,8 @@ OPS = {"P1": 1, "P3": 3, "F1": 11, "F2": 12, "F3": 13}
 PREC_F64 = 0
 PREC_F32 = 1
 PREC_F16 = 2
+PREC_F32_ZSYM = 3   # FP32 storage of the 14 distinct populations of a z-mirror-symmetric state
+PREC_F16_ZSYM = 4
 STATE_POST_STREAM = 0
 STATE_POST_COLLISION = 1
 STEP_RECORD_AUX = 1

@@ -30,12 +30,14 @@ struct CaseState {
     lbm_case_desc d{};
     CaseConst cc{};
     long long n = 0, S = 0;
-    double* pop[2] = {nullptr, nullptr};   // FP64: row blocks [y][colour][19][nx]; FP32/FP16: pairs ((y*19 + i)*nx + x)
+    double* pop[2] = {nullptr, nullptr};   // FP64: row blocks [y][colour][19][nx]; FP32/FP16: pairs ((y*NS + slot)*nx + x)
     int cur = 0;                            // buffer holding the latest post-collision state
     double* aux = nullptr;                  // [6][S] gm, F
     bool aux_valid = false;
     int kind = 0;                           // kernel family (KIND_*)
     int elem = 8;                           // bytes per stored value: 8 (FP64), 4 (FP32), 2 (FP16)
+    bool sym = false;                       // FP32/FP16 z-symmetric storage: 14 of the 19 populations
+    int nslot = Q;                          // stored pairs per site (19, or 14 with sym)
     bool active = true;                     // advanced by lbm_step
     double* f64 = nullptr;                  // [2][19][S] FP64 copy of a low-precision state (diagnostics)
     // diagnostics scratch (allocated on first use)
@@ -281,23 +283,44 @@ long long gcdll(long long a, long long b)
     return a;
 }
 
-// (fR, fB) [colour][19][S] FP64 at own site -> pairs (h_i, e_i) at ((y * 19 + i) nx + x) in the case's
-// storage type: e_i = (fR - fB) / (fR + fB), h_i = 2^k (f_i - w_i) - c_k r(e_i) with the case's reference
+// (fR, fB) [colour][19][S] FP64 at own site -> pairs (h_i, e_i) at ((y * NS + slot) nx + x) in the case's
+// storage type (the z-symmetric storage keeps the mirror pairs' mean after checking that they agree): e_i = (fR - fB) / (fR + fB), h_i = 2^k (f_i - w_i) - c_k r(e_i) with the case's reference
 // density deviation r(e) = ref_mid + ref_half e (lattice.h).  The FP16 form keeps h as a half and e as the signed
 // minority fraction copysign((1 - |e|) / 2, e) as a half (the device encoding of the same pair).
 int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
 {
-    const int nx = k.d.nx;
+    const int nx = k.d.nx, ns = k.nslot;
     const LpRef rf = make_lp_ref(k.cc.ref_mid, k.cc.ref_half);
-    std::vector<float> a(2 * (size_t)Q * k.n);
+    std::vector<float> a(2 * (size_t)ns * k.n);
+    // z-symmetric storage: a state whose mirror populations differ beyond rounding cannot be held
+    if (k.sym)
+        for (int i = 0; i < Q; ++i) {
+            const int m = MIRROR_OF[i];
+            if (m == i) continue;
+            for (int col = 0; col < 2; ++col)
+                for (long long p = 0; p < k.n; ++p) {
+                    const double u = h[(size_t)(col * Q + i) * k.S + p], v = h[(size_t)(col * Q + m) * k.S + p];
+                    if (!(std::fabs(u - v) <= 1e-12 * std::fmax(std::fabs(u), std::fabs(v)) + 1e-300))
+                        return fail(c, LBM_E_ARG, "state is not mirror-symmetric in z (z-symmetric storage holds 14 of "
+                                                  "the 19 populations); use LBM_PREC_F32 or LBM_PREC_F16");
+                }
+        }
     for (long long p = 0; p < k.n; ++p) {
         const long long y = p / nx, x = p - y * nx;
         for (int i = 0; i < Q; ++i) {
+            if (!is_stored(k.sym, i)) continue;
             const int cl = vclass(i);
             const double w = (i == 0) ? WL0 : (i < 7 ? WL1 : WL2);   // the weights the device decodes with
-            const double r = h[(size_t)i * k.S + p], b = h[(size_t)(Q + i) * k.S + p];
+            double r = h[(size_t)i * k.S + p], b = h[(size_t)(Q + i) * k.S + p];
+            if (k.sym) {
+                for (int m = 0; m < Q; ++m)
+                    if (m != i && MIRROR_OF[m] == i) {
+                        r = 0.5 * (r + h[(size_t)m * k.S + p]);
+                        b = 0.5 * (b + h[(size_t)(Q + m) * k.S + p]);
+                    }
+            }
             const double f = r + b;
-            const size_t j = 2 * (size_t)((y * Q + i) * nx + x);
+            const size_t j = 2 * (size_t)((y * ns + slot_of(k.sym, i)) * nx + x);
             const float e = (float)((f != 0.0) ? (r - b) / f : 0.0);
             a[j] = (float)((f - w) * (double)SCL[cl]) - std::fma(rf.half(cl), e, rf.mid(cl));   // as the device
             a[j + 1] = e;
@@ -333,7 +356,7 @@ int state_f64(lbm_ctx* c, CaseState& k, const double** out)
     if (k.elem == 8) {
         HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.d.nx, k.n, k.S, k.f64, c->stream));
     } else {
-        HIPCHK(c, launch_lp_to_f64(k.elem, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
+        HIPCHK(c, launch_lp_to_f64(k.elem, k.sym, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
                                    k.f64 + Q * k.S, c->stream));
     }
     *out = k.f64;
@@ -357,8 +380,9 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
     if (!out) return LBM_E_ARG;
     *out = nullptr;
     if (!cases || n_cases < 1) return LBM_E_ARG;
-    if (precision != LBM_PREC_F64 && precision != LBM_PREC_F32 && precision != LBM_PREC_F16) return LBM_E_ARG;
-    const int elem = (precision == LBM_PREC_F64) ? 8 : (precision == LBM_PREC_F32 ? 4 : 2);
+    if (precision < LBM_PREC_F64 || precision > LBM_PREC_F16_ZSYM) return LBM_E_ARG;
+    const int elem = (precision == LBM_PREC_F64) ? 8 : ((precision == LBM_PREC_F32 || precision == LBM_PREC_F32_ZSYM) ? 4 : 2);
+    const bool sym = precision == LBM_PREC_F32_ZSYM || precision == LBM_PREC_F16_ZSYM;
     lbm_ctx* c = new (std::nothrow) lbm_ctx();
     if (!c) return LBM_E_NOMEM;
     int dev_count = 0;
@@ -387,9 +411,11 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.S = (k.n + 63) / 64 * 64 + stride_pad();
         k.kind = step_kind(d.op);
         k.elem = elem;
+        k.sym = sym;
+        k.nslot = n_slots(sym);
         for (int b = 0; b < 2; ++b) {
-            if (dev_alloc(c, (void**)&k.pop[b], (size_t)elem * 2 * Q * (size_t)k.S) ||
-                hipMemsetAsync(k.pop[b], 0, (size_t)elem * 2 * Q * (size_t)k.S, c->stream) != hipSuccess) {
+            if (dev_alloc(c, (void**)&k.pop[b], (size_t)elem * 2 * k.nslot * (size_t)k.S) ||
+                hipMemsetAsync(k.pop[b], 0, (size_t)elem * 2 * k.nslot * (size_t)k.S, c->stream) != hipSuccess) {
                 lbm_destroy(c);
                 return LBM_E_NOMEM;
             }
@@ -506,7 +532,7 @@ int lbm_init_drop(lbm_ctx* c, int id, double R, double width, int x0, int y0, do
         // reduced-precision forms: computed on the device in the stored form (no host arrays, so
         // boxes of 10^8 sites initialise in milliseconds); FP64 keeps the host path below, whose
         // libm exp gives the reference's bits
-        HIPCHK(c, launch_lp_init_drop(k.elem, k.pop[k.cur], nx, ny, R, k_perfil, x0, y0, v0, k.cc.ref_mid,
+        HIPCHK(c, launch_lp_init_drop(k.elem, k.sym, k.pop[k.cur], nx, ny, R, k_perfil, x0, y0, v0, k.cc.ref_mid,
                                       k.cc.ref_half, c->stream));
         k.aux_valid = false;
         return LBM_OK;
@@ -666,8 +692,9 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
     a.parity = parity;
     a.record = record;
     a.depth = g.depth;
-    const int elem = c->cases[g.members[0]].elem;
-    HIPCHK(c, elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, st) : launch_step_lp(elem, g.kind, a, g.ntiles, st));
+    const CaseState& k0 = c->cases[g.members[0]];   // a context has one storage form
+    HIPCHK(c, k0.elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, st)
+                           : launch_step_lp(k0.elem, k0.sym, g.kind, a, g.ntiles, st));
     return LBM_OK;
 }
 int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_group(c, g, parity, record, c->stream); }
@@ -755,13 +782,13 @@ int group_tune(lbm_ctx* c, Group& g)
 }
 
 // Tables, aux buffers and tile shape of a group, before its launches are queued.
-int group_prepare(lbm_ctx* c, Group& g, bool rec)
+int group_prepare(lbm_ctx* c, Group& g, bool rec, bool tune = true)
 {
     if (rec)
         for (int id : g.members)
             if (int r = ensure_aux(c, c->cases[id])) return r;
     if (int r = group_sync(c, g)) return r;
-    if (!g.tuned && c->autotune)
+    if (tune && !g.tuned && c->autotune)
         if (int r = group_tune(c, g)) return r;
     return LBM_OK;
 }
@@ -796,6 +823,86 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
         if (int r = group_step(c, *act[j], n_steps, flags, st, true)) return r;
         HIPCHK(c, hipEventRecord(c->join[j], st));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[j], 0));
+    }
+    return LBM_OK;
+}
+
+// Tile shapes of lbm_step's families, tuned on real steps.  Every shape computes every site the same
+// way, so candidate shapes can be timed on steps the caller asked for: the simulation advances
+// exactly as it would otherwise.  Real steps alternate the state buffers as stepping does (launches
+// repeated on one buffer pair re-read an input that the last-level cache then partly holds, which
+// favoured the wrong shapes), and all active families run together, as they do in a step, so a
+// family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
+// fixed: the pipeline depth (FP32/FP16 kernels) at the current rows per segment, then the rows per
+// segment (4 .. 64) at that depth, each candidate over W untimed + T timed steps; the two fastest are
+// timed again over 2T steps.  The steps used are subtracted from `left`; nothing is done unless
+// `left` covers the whole search.
+int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
+{
+    constexpr int W = 1, T = 6;
+    constexpr int LYS[] = {4, 8, 16, 32, 64};
+    auto depths_of = [&](const Group& g) { return c->cases[g.members[0]].elem == 8 ? 1 : 3; };
+    long long budget = 0;
+    for (const Group* g : act) budget += (long long)(depths_of(*g) + 4) * (W + T) + 2 * (W + 2 * T);
+    if (budget > left) return LBM_OK;
+    if (!c->ev[0]) {
+        HIPCHK(c, hipEventCreate(&c->ev[0]));
+        HIPCHK(c, hipEventCreate(&c->ev[1]));
+    }
+    auto run = [&](int steps, float* ms) -> int {
+        if (ms) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        if (int r = queue_steps(c, act, steps, 0)) return r;
+        left -= steps;
+        if (ms) {
+            HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev[1]));
+            HIPCHK(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+            *ms /= (float)steps;
+        }
+        return LBM_OK;
+    };
+    for (Group* g : act) {
+        auto apply = [&](const Cand& cd) -> int {
+            g->depth = cd.depth;
+            g->ly.assign(g->members.size(), cd.ly);
+            g->written = false;
+            return group_sync(c, *g);
+        };
+        auto time_of = [&](Cand& cd, int t) -> int {
+            if (int r = apply(cd)) return r;
+            if (int r = run(W, nullptr)) return r;
+            return run(t, &cd.ms);
+        };
+        auto by_time = [](const Cand& x, const Cand& y) { return x.ms < y.ms; };
+        // starting rows per segment: the current one, rounded to a candidate
+        int ly0 = g->ly.empty() ? 16 : g->ly[0];
+        int start = LYS[0];
+        for (int v : LYS)
+            if (v <= ly0) start = v;
+        std::vector<Cand> tried;
+        const bool lp = depths_of(*g) > 1;
+        for (int d = 1; d <= depths_of(*g); ++d) {
+            Cand cd{start, lp ? d : 0, 0.f};
+            if (int r = time_of(cd, T)) return r;
+            tried.push_back(cd);
+        }
+        const int best_depth = std::min_element(tried.begin(), tried.end(), by_time)->depth;
+        for (int v : LYS) {
+            if (v == start) continue;
+            Cand cd{v, best_depth, 0.f};
+            if (int r = time_of(cd, T)) return r;
+            tried.push_back(cd);
+        }
+        std::sort(tried.begin(), tried.end(), by_time);
+        for (int j = 0; j < 2; ++j)
+            if (int r = time_of(tried[j], 2 * T)) return r;
+        std::sort(tried.begin(), tried.begin() + 2, by_time);
+        if (int r = apply(tried[0])) return r;
+        g->tuned = true;
+        if (std::getenv("ITOLBM_TUNE_LOG"))
+            std::fprintf(stderr, "itolbm step tuner: family %d, %zu cases, element %d B: %d rows per segment, depth %d, step %.1f us\n",
+                         g->kind, g->members.size(), c->cases[g->members[0]].elem, tried[0].ly, tried[0].depth,
+                         1000.0 * tried[0].ms);
     }
     return LBM_OK;
 }
@@ -879,8 +986,18 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
         if (!g.members.empty()) act.push_back(&g);
     if (act.empty()) return LBM_OK;
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
-    for (Group* g : act)
-        if (int r = group_prepare(c, *g, rec)) return r;
+    bool untuned = false;
+    for (Group* g : act) {
+        if (int r = group_prepare(c, *g, rec, false)) return r;
+        if (!g->tuned && c->autotune) {
+            if (std::getenv("ITOLBM_LY")) {   // a fixed shape; ITOLBM_DEPTH fixes the depth too (0 = default)
+                if (const char* e = std::getenv("ITOLBM_DEPTH")) g->depth = std::max(0, std::min(3, std::atoi(e)));
+                g->tuned = true;
+            } else {
+                untuned = true;
+            }
+        }
+    }
     if (act.size() > 1 && !c->fork) {
         HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         for (int k = 0; k < N_KINDS; ++k) {
@@ -889,11 +1006,18 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
         }
     }
     // Whole graphs of graph_steps steps first (the last step stays outside when it records aux),
-    // then the remaining steps launch by launch.
+    // then the remaining steps launch by launch.  Families without a tuned shape are tuned first on
+    // the call's own steps when there are enough of them (the recording step excepted).
     int left = n_steps;
+    if (untuned) {
+        int avail = left - (rec ? 1 : 0);
+        const int before = avail;
+        if (int r = online_tune(c, act, avail)) return r;
+        left -= before - avail;
+    }
     const int G = c->graph_steps;
-    if (G > 0 && n_steps - (rec ? 1 : 0) >= G) {
-        const int reps = (n_steps - (rec ? 1 : 0)) / G;
+    if (G > 0 && left - (rec ? 1 : 0) >= G) {
+        const int reps = (left - (rec ? 1 : 0)) / G;
         if (int r = ensure_graph(c, act, G)) return r;
         for (int i = 0; i < reps; ++i) HIPCHK(c, hipGraphLaunch(c->graph.exec, c->stream));
         for (Group* g : act) advance(c, *g, reps * G);
@@ -951,7 +1075,7 @@ int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
     CaseState& k = c->cases[id];
     if (k.elem == 8) return LBM_OK;   // FP64 stores the populations themselves
     const double mid = 0.5 * (rho_red + rho_blue) - 1.0, half = 0.5 * (rho_red - rho_blue);
-    HIPCHK(c, launch_lp_recentre(k.elem, k.pop[k.cur], (long long)Q * k.n, k.d.nx, k.cc.ref_mid, k.cc.ref_half, mid, half,
+    HIPCHK(c, launch_lp_recentre(k.elem, k.sym, k.pop[k.cur], (long long)k.nslot * k.n, k.d.nx, k.cc.ref_mid, k.cc.ref_half, mid, half,
                                  c->stream));
     k.cc.ref_mid = mid;
     k.cc.ref_half = half;

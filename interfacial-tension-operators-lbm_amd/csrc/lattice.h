@@ -44,6 +44,35 @@ static_assert(CWL[1] == CWL[2], "axis and diagonal classes share the weight unit
 // velocity class of population i: 0 rest, 1 axis, 2 diagonal
 __host__ __device__ constexpr int vclass(int i) { return i == 0 ? 0 : (i < 7 ? 1 : 2); }
 
+// z-mirror-symmetric storage of the FP32 / FP16 forms.  With one node in z, velocity i and its z
+// mirror (c_z -> -c_z) pull from the same site, and a state that is symmetric under z -> -z has
+// f_6 = f_5, f_13 = f_11, f_14 = f_12, f_17 = f_15, f_18 = f_16.  Every state the programs create is
+// (gradients and forces have no z component), and the FP32 / FP16 collision keeps the symmetry
+// exactly (its z moment is summed over mirror pairs), so 14 of the 19 populations are stored.
+constexpr int NSYM = 14;
+constexpr int MIRROR_OF[Q] = {0, 1, 2, 3, 4, 5, 5, 7, 8, 9, 10, 11, 12, 11, 12, 15, 16, 15, 16};   // stored partner
+constexpr int SLOT_SYM[Q] = {0, 1, 2, 3, 4, 5, 5, 6, 7, 8, 9, 10, 11, 10, 11, 12, 13, 12, 13};     // slot of i
+constexpr int VEL_SYM[NSYM] = {0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 15, 16};                     // velocity of a slot
+
+constexpr bool check_mirrors()
+{
+    for (int i = 0; i < Q; ++i) {
+        const int m = MIRROR_OF[i];
+        if (VEL_SYM[SLOT_SYM[i]] != m || MIRROR_OF[m] != m) return false;
+        if (CX[i] != CX[m] || CY[i] != CY[m] || vclass(i) != vclass(m)) return false;
+        if (m != i && CZ[i] != -CZ[m]) return false;
+    }
+    for (int s = 0; s < NSYM; ++s)
+        if (SLOT_SYM[VEL_SYM[s]] != s) return false;
+    return true;
+}
+static_assert(check_mirrors(), "z-mirror table of the D3Q19 ordering");
+
+// stored slots per site and the slot of velocity i: all 19 (full storage) or 14 (z-symmetric)
+__host__ __device__ constexpr int n_slots(bool sym) { return sym ? NSYM : Q; }
+__host__ __device__ constexpr int slot_of(bool sym, int i) { return sym ? SLOT_SYM[i] : i; }
+__host__ __device__ constexpr bool is_stored(bool sym, int i) { return !sym || MIRROR_OF[i] == i; }
+
 // c_k r(e) = mid(k) + half(k) e, formed identically on the host and the device (c_1 = c_2, so the
 // axis and diagonal classes share one pair; scalar fields keep it in registers)
 struct LpRef {

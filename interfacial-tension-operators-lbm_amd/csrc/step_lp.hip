@@ -105,12 +105,12 @@ struct Row {
 // A row loop that issues its look-ahead loads on every path has the same loads outstanding on each,
 // so the compiler's waits for the current row (vector memory counts retire in issue order) do not
 // also wait for the rows ahead.
-template <typename ST>
+template <typename ST, bool SYM>
 __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restrict__ in, int nx, int ny, int y,
                                          const Lane& L, typename PairOf<ST>::raw r[Q], bool live = true)
 {
     using R = typename PairOf<ST>::raw;
-    const long long pitch = (long long)Q * nx;
+    const long long pitch = (long long)n_slots(SYM) * nx;
     const unsigned bytes = live ? (unsigned)(pitch * (long long)sizeof(R)) : 0u;
     const rsrc_t rowm = make_rsrc(in + wrapi(y - 1, ny) * pitch, bytes);
     const rsrc_t row0 = make_rsrc(in + wrapi(y, ny) * pitch, bytes);
@@ -120,9 +120,14 @@ __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restr
                    op = (unsigned)L.xp * (unsigned)sizeof(R);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+        if (!is_stored(SYM, i)) continue;
         const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : o0);
-        r[i] = pair_load<R>(CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0), xs, (unsigned)(i * nx * (int)sizeof(R)));
+        r[i] = pair_load<R>(CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0), xs,
+                            (unsigned)(slot_of(SYM, i) * nx * (int)sizeof(R)));
     }
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+        if (!is_stored(SYM, i)) r[i] = r[MIRROR_OF[i]];   // the z mirror pulls from the same site
 }
 
 using Ref = LpRef;
@@ -373,7 +378,7 @@ __device__ __forceinline__ float zmoment(const float v[Q])
     return (((v[5] - v[6]) + (v[11] - v[13])) + ((v[14] - v[12]) + (v[16] - v[18]))) + (v[17] - v[15]);
 }
 
-template <bool FORCED, class Out>
+template <bool FORCED, bool ZSYM, class Out>
 __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float ds, float rho, double phid, float gx,
                                            float gy, float Fx, float Fy, Out out)
 {
@@ -389,20 +394,21 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
         if (CY[i] > 0) jy += r.s[i];
         if (CY[i] < 0) jy -= r.s[i];
     }
-    const float jz = zmoment(r.s);
+    // z-symmetric storage: the state is symmetric by construction and the z moment is 0
+    const float jz = ZSYM ? 0.f : zmoment(r.s);
     const float irho = __builtin_amdgcn_rcpf(rho);
     float ux = jx * irho, uy = jy * irho;
-    const float uz = jz * irho;
+    const float uz = ZSYM ? 0.f : jz * irho;
     if constexpr (FORCED) {
         ux += 0.5f * Fx * irho;
         uy += 0.5f * Fy * irho;
     }
-    const float usq = ux * ux + uy * uy + uz * uz;
+    const float usq = ZSYM ? ux * ux + uy * uy : ux * ux + uy * uy + uz * uz;
     const float vF = ux * Fx + uy * Fy;
     float s[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const float cu = cdot3(i, ux, uy, uz);
+        const float cu = ZSYM ? cdot2(i, ux, uy) : cdot3(i, ux, uy, uz);
         const float seq = WF[i] * (ds + rho * (3.f * cu + 4.5f * cu * cu - 1.5f * usq));
         s[i] = r.s[i] + (seq - r.s[i]) * k.oot;
         if constexpr (FORCED) {
@@ -442,7 +448,7 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
             if (CY[i] > 0) py += s[i];
             if (CY[i] < 0) py -= s[i];
         }
-        const float pz = zmoment(s);
+        const float pz = ZSYM ? 0.f : zmoment(s);
         const float dx = 0.5f * ((FORCED ? jx + Fx : jx) - px);
         const float dy = 0.5f * ((FORCED ? jy + Fy : jy) - py);
         const float dz = 0.5f * (jz - pz);
@@ -451,8 +457,10 @@ __device__ __forceinline__ void collide_lp(const ConstF& k, const Row& r, float 
         s[2] -= dx;
         s[3] += dy;
         s[4] -= dy;
-        s[5] += dz;
-        s[6] -= dz;
+        if constexpr (!ZSYM) {
+            s[5] += dz;
+            s[6] -= dz;
+        }
     }
     // recolouring: e_i = phi - 2 beta cR cB w_i rho cos_i / f_i, the difference taken in FP64 and
     // rounded once
@@ -536,7 +544,7 @@ __device__ __forceinline__ AuxF aux_of(const ConstF& k, const Nine& p)
 
 // Two-ring halo of an aligned strip (two-row stencils): load A holds columns c0-2 and c0-1, load B
 // columns c0+64 and c0+65, in the lane layout of halo_lane_lp.
-template <typename ST>
+template <typename ST, bool SYM>
 struct Halo2LP {
     using R = typename PairOf<ST>::raw;
     unsigned long long off[2];   // element offset of (velocity i, column x) within a row block
@@ -547,13 +555,13 @@ struct Halo2LP {
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const HaloLaneLP h = s ? halo_lane_lp(nx, c0 + 64, c0 + 65) : halo_lane_lp(nx, c0 - 2, c0 - 1);
-            off[s] = (unsigned long long)h.i * (unsigned)nx + h.x;
+            off[s] = (unsigned long long)slot_of(SYM, h.i) * (unsigned)nx + h.x;
             dy[s] = h.dy;
         }
     }
     __device__ __forceinline__ void load(gcst<R>* in, int nx, int ny, int y, R& a, R& b) const
     {
-        const unsigned long long pitch = (unsigned long long)Q * (unsigned)nx;
+        const unsigned long long pitch = (unsigned long long)n_slots(SYM) * (unsigned)nx;
         a = in[(unsigned long long)wrap_row3(y, dy[0], ny) * pitch + off[0]];
         b = in[(unsigned long long)wrap_row3(y, dy[1], ny) * pitch + off[1]];
     }
@@ -570,7 +578,7 @@ __device__ __forceinline__ void halo2_phase(typename PairOf<ST>::raw va, typenam
 
 // Collision of one site row of an aligned strip and the stores of its 19 pairs (lanes inside the
 // case; a site without fluid, rho <= 1e-30, is stored unchanged), and the aux field on record steps.
-template <typename ST, bool FORCED>
+template <typename ST, bool FORCED, bool SYM>
 __device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
                                               gst<typename PairOf<ST>::raw>* out, int y,
                                               const typename PairOf<ST>::raw (&cur)[Q], double phi, float gx,
@@ -583,13 +591,15 @@ __device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& 
     decode<ST>(cur, row, rf);
     float ds, rho;
     row_density(row, ds, rho);
-    const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)Q * nx), (unsigned)(Q * nx * (int)sizeof(R)));
+    constexpr int NS = n_slots(SYM);
+    const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)NS * nx), (unsigned)(NS * nx * (int)sizeof(R)));
     const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
     auto st = [&](int i, float s, float e) {
-        pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(i * nx * (int)sizeof(R)));
+        if (!is_stored(SYM, i)) return;   // equal to its z mirror's
+        pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(slot_of(SYM, i) * nx * (int)sizeof(R)));
     };
     if (rho > 1.0e-30f) {
-        collide_lp<FORCED>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+        collide_lp<FORCED, SYM>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
@@ -606,7 +616,7 @@ __device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& 
 // every buffer role is a compile-time constant).  The re-read of row y is issued before the wait on
 // row y + 2, so that the collision's wait leaves the rows ahead in flight (vector memory counts
 // retire in issue order).
-template <typename ST, int KIND, int DEPTH>
+template <typename ST, int KIND, int DEPTH, bool SYM>
 __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
 {
     static_assert(KIND == KIND_F1 || KIND == KIND_F2, "two-row stencils only");
@@ -622,7 +632,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
-    Halo2LP<ST> h;
+    Halo2LP<ST, SYM> h;
     h.init(nx, t.strip);
     const int y0 = t.y0, y1 = t.y1;
 
@@ -632,7 +642,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     float P[5], HW[5][4];
     double pd[3];   // FP64 colour field of rows y, y+1, y+2 (for their collision)
     auto phi_row = [&](int y, float (&hw)[4]) {
-        load_raw<ST>(in, nx, ny, y, L, cur);
+        load_raw<ST, SYM>(in, nx, ny, y, L, cur);
         R va, vb;
         h.load(in, nx, ny, y, va, vb);
         halo2_phase<ST>(va, vb, rf, hw);
@@ -672,7 +682,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     hl[1] = aux_halo(1, 0); hr[1] = aux_halo(1, 1);
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
-        load_raw<ST>(in, nx, ny, y0 + 2 + r, L, ahead[r]);
+        load_raw<ST, SYM>(in, nx, ny, y0 + 2 + r, L, ahead[r]);
         h.load(in, nx, ny, y0 + 2 + r, ha[r], hb[r]);
     }
 
@@ -684,19 +694,19 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             pd[0] = pd[1];
             pd[1] = pd[2];
             if constexpr (NA > 0) {
-                load_raw<ST>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
+                load_raw<ST, SYM>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
                 __builtin_amdgcn_sched_barrier(0);      // issued before the wait on row yy + 2
                 pd[2] = raw_phase<ST>(ahead[j], rf);    // row yy + 2 (waits for its loads)
                 halo2_phase<ST>(ha[j], hb[j], rf, HW[4]);
                 // row yy + 2 + NA, used NA iterations on; past the segment's last row the loads are
                 // issued empty, so every path leaves the same loads outstanding
                 const bool live = yy + NA < y1;
-                load_raw<ST>(in, nx, ny, yy + 2 + NA, L, ahead[j], live);
+                load_raw<ST, SYM>(in, nx, ny, yy + 2 + NA, L, ahead[j], live);
                 h.load(in, nx, ny, live ? yy + 2 + NA : yy, ha[j], hb[j]);
                 __builtin_amdgcn_sched_barrier(0);
             } else {
                 pd[2] = phi_row(yy + 2, HW[4]);
-                load_raw<ST>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
+                load_raw<ST, SYM>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
             }
             P[4] = (float)pd[2];
             const AuxF ap = aux_lanes(2);
@@ -721,7 +731,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             const float hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
             const Nine p9 = gather9h(P[1], P[2], P[3], hl1, hr0, L.lane);
             const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
-            collide_store<ST, true>(t, k, rf, L, out, yy, cur, pd[0], gx, gy, Fx, Fy);
+            collide_store<ST, true, SYM>(t, k, rf, L, out, yy, cur, pd[0], gx, gy, Fx, Fy);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 P[r] = P[r + 1];
@@ -739,7 +749,7 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
 // Halo of an aligned strip for the pair layout: one wavefront load per row (halo_lane_lp).  The address is
 // computed from a per-lane row index, apart from the strip's own loads, whose row bases stay uniform
 // (scalar registers) that way.
-template <typename ST>
+template <typename ST, bool SYM>
 struct HaloLP {
     using R = typename PairOf<ST>::raw;
     unsigned long long off;   // element offset of (velocity i, column x) within a row block
@@ -748,7 +758,7 @@ struct HaloLP {
     __device__ __forceinline__ void init(int nx, int strip)
     {
         const HaloLaneLP h = halo_lane_lp(nx, strip * 64 - 1, strip * 64 + 64);
-        off = (unsigned long long)h.i * (unsigned)nx + h.x;
+        off = (unsigned long long)slot_of(SYM, h.i) * (unsigned)nx + h.x;
         dy = h.dy;
         on = h.on;
     }
@@ -756,13 +766,13 @@ struct HaloLP {
     {
         const int yy = wrap_row3(y, dy, ny);
         // every lane loads (the unused lanes re-read an in-range value): no branch
-        return in[(unsigned long long)yy * ((unsigned long long)Q * (unsigned)nx) + off];
+        return in[(unsigned long long)yy * ((unsigned long long)n_slots(SYM) * (unsigned)nx) + off];
     }
 };
 
 // Collision of row y of an aligned strip, given the colour field of rows y-1, y, y+1 (own lanes and
 // the two halo columns).
-template <typename ST, int KIND>
+template <typename ST, int KIND, bool SYM>
 __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
                                             gst<typename PairOf<ST>::raw>* out, int y,
                                             const typename PairOf<ST>::raw (&cur)[Q], const float (&p)[3],
@@ -777,7 +787,7 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
         Fx = -mu * gx;
         Fy = -mu * gy;
     }
-    collide_store<ST, KIND != KIND_P>(t, k, rf, L, out, y, cur, phi, gx, gy, Fx, Fy);
+    collide_store<ST, KIND != KIND_P, SYM>(t, k, rf, L, out, y, cur, phi, gx, gy, Fx, Fy);
 }
 
 __device__ __forceinline__ void opaque(f32x2& v)
@@ -794,7 +804,7 @@ __device__ __forceinline__ void opaque(unsigned& v) { asm volatile("" : "+v"(v))
 // The row loop is unrolled DEPTH + 1 times, so every buffer role is a compile-time constant (no
 // register copies).  DEPTH trades registers (occupancy) for bytes in flight per wavefront; the
 // library's tuner picks it with the row-segment length.
-template <typename ST, int KIND, int DEPTH>
+template <typename ST, int KIND, int DEPTH, bool SYM>
 __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
 {
     static_assert(KIND == KIND_P || KIND == KIND_F3, "one-row stencils only");
@@ -809,7 +819,7 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
-    HaloLP<ST> h;
+    HaloLP<ST, SYM> h;
     h.init(nx, t.strip);
     const int y0 = t.y0, y1 = t.y1;
 
@@ -817,11 +827,11 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     R buf[NB][Q];
     R hv[NB];
     float p[3], hl[3], hr[3];
-    load_raw<ST>(in, nx, ny, y0 - 1, L, buf[NB - 1]);
+    load_raw<ST, SYM>(in, nx, ny, y0 - 1, L, buf[NB - 1]);
     hv[NB - 1] = h.load(in, nx, ny, y0 - 1);
 #pragma unroll
     for (int r = 0; r < DEPTH; ++r) {
-        load_raw<ST>(in, nx, ny, y0 + r, L, buf[r]);
+        load_raw<ST, SYM>(in, nx, ny, y0 + r, L, buf[r]);
         hv[r] = h.load(in, nx, ny, y0 + r);
     }
     p[1] = (float)raw_phase<ST>(buf[NB - 1], rf);
@@ -840,7 +850,7 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
                 // rows up to y1 feed the gradients; past it the loads are issued empty, so every
                 // path leaves the same loads outstanding
                 const bool live = yy + DEPTH <= y1;
-                load_raw<ST>(in, nx, ny, yy + DEPTH, L, far, live);
+                load_raw<ST, SYM>(in, nx, ny, yy + DEPTH, L, far, live);
                 hv[(j + DEPTH) % NB] = h.load(in, nx, ny, live ? yy + DEPTH : yy);
                 __builtin_amdgcn_sched_barrier(0);
                 p[0] = p[1]; p[1] = p[2];
@@ -854,31 +864,31 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
 #pragma unroll
                     for (int i = 0; i < Q; ++i) opaque(cur[i]);   // re-decode rather than keep decoded values alive
                 }
-                collide_row<ST, KIND>(t, k, rf, L, out, yy, cur, p, hl, hr, phc);
+                collide_row<ST, KIND, SYM>(t, k, rf, L, out, yy, cur, p, hl, hr, phc);
             }
         }
     }
 }
 
-template <typename ST, int KIND>
+template <typename ST, int KIND, bool SYM>
 void launch_pipe(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
 {
     const dim3 block(64);
-    if (depth <= 1) hipLaunchKernelGGL((lp_pipe<ST, KIND, 1>), grid, block, 0, stream, a);
-    else if (depth == 2) hipLaunchKernelGGL((lp_pipe<ST, KIND, 2>), grid, block, 0, stream, a);
-    else hipLaunchKernelGGL((lp_pipe<ST, KIND, 3>), grid, block, 0, stream, a);
+    if (depth <= 1) hipLaunchKernelGGL((lp_pipe<ST, KIND, 1, SYM>), grid, block, 0, stream, a);
+    else if (depth == 2) hipLaunchKernelGGL((lp_pipe<ST, KIND, 2, SYM>), grid, block, 0, stream, a);
+    else hipLaunchKernelGGL((lp_pipe<ST, KIND, 3, SYM>), grid, block, 0, stream, a);
 }
 
-template <typename ST, int KIND>
+template <typename ST, int KIND, bool SYM>
 void launch_f12(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
 {
     const dim3 block(64);
-    if (depth <= 1) hipLaunchKernelGGL((lp_f12<ST, KIND, 1>), grid, block, 0, stream, a);
-    else if (depth == 2) hipLaunchKernelGGL((lp_f12<ST, KIND, 2>), grid, block, 0, stream, a);
-    else hipLaunchKernelGGL((lp_f12<ST, KIND, 3>), grid, block, 0, stream, a);
+    if (depth <= 1) hipLaunchKernelGGL((lp_f12<ST, KIND, 1, SYM>), grid, block, 0, stream, a);
+    else if (depth == 2) hipLaunchKernelGGL((lp_f12<ST, KIND, 2, SYM>), grid, block, 0, stream, a);
+    else hipLaunchKernelGGL((lp_f12<ST, KIND, 3, SYM>), grid, block, 0, stream, a);
 }
 
-template <typename ST>
+template <typename ST, bool SYM>
 hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
 {
     const dim3 grid(ntiles);
@@ -886,10 +896,10 @@ hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t s
     // for FP16 storage (measured best on MI355X without tuning)
     const int depth = a.depth > 0 ? a.depth : (sizeof(typename PairOf<ST>::raw) == 8 ? 2 : 1);
     switch (kind) {
-    case KIND_P: launch_pipe<ST, KIND_P>(depth, a, grid, stream); break;
-    case KIND_F3: launch_pipe<ST, KIND_F3>(depth, a, grid, stream); break;
-    case KIND_F1: launch_f12<ST, KIND_F1>(depth, a, grid, stream); break;
-    case KIND_F2: launch_f12<ST, KIND_F2>(depth, a, grid, stream); break;
+    case KIND_P: launch_pipe<ST, KIND_P, SYM>(depth, a, grid, stream); break;
+    case KIND_F3: launch_pipe<ST, KIND_F3, SYM>(depth, a, grid, stream); break;
+    case KIND_F1: launch_f12<ST, KIND_F1, SYM>(depth, a, grid, stream); break;
+    case KIND_F2: launch_f12<ST, KIND_F2, SYM>(depth, a, grid, stream); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -897,8 +907,8 @@ hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t s
 
 // ------------------------------------------------------------------ conversions for transfers / diagnostics
 // (h, e) pairs in rows -> fR, fB in FP64, [19][S] each (own site, post-collision); s is decoded as
-// the step kernels decode it, f = w + s in FP64
-template <typename ST>
+// the step kernels decode it, f = w + s in FP64 (a z mirror of the symmetric storage from its slot)
+template <typename ST, bool SYM>
 __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw* __restrict__ pop, int nx, long long S,
                                                  long long n, Ref rf, double* __restrict__ fR, double* __restrict__ fB)
 {
@@ -906,7 +916,7 @@ __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw*
         const long long y = p / nx, x = p - y * nx;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const auto r = pop[(y * Q + i) * nx + x];
+            const auto r = pop[(y * n_slots(SYM) + slot_of(SYM, i)) * nx + x];
             const float e = PairOf<ST>::e(r);
             const float s = decode_s(PairOf<ST>::a(r), e, vclass(i), rf);
             const double f = (double)WF[i] + (double)s;
@@ -917,14 +927,15 @@ __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw*
 }
 
 // Re-express a stored state against a new reference density, in place:
-// h' = h + c_k r_old(e) - c_k r_new(e).  Pair p is velocity (p / nx) mod 19 of its row block.
-template <typename ST>
+// h' = h + c_k r_old(e) - c_k r_new(e).  Pair p is slot (p / nx) mod (19 or 14) of its row block.
+template <typename ST, bool SYM>
 __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __restrict__ pop, long long npairs,
                                                    int nx, Ref from, Ref to)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs;
          p += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)((p / nx) % Q);
+        const int sl = (int)((p / nx) % n_slots(SYM));
+        const int i = SYM ? VEL_SYM[sl] : sl;
         const int k = vclass(i);
         const auto r = pop[p];
         const float e = PairOf<ST>::e(r);
@@ -937,7 +948,7 @@ __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __r
 // Initial_conditions.cpp:37-169, EQ_CLASSICO, cylinder, rho_R = rho_B = 1): f_R = f^eq(phi rho,
 // (v0, 0, 0)), f_B = f^eq((1 - phi) rho, (v0, 0, 0)).  The stored value at site x for velocity i is
 // the population that streams into x, i.e. the initial value at x + c_i (the kernel pulls).
-template <typename ST>
+template <typename ST, bool SYM>
 __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __restrict__ pop, int nx, int ny,
                                                     double R, double kp, int x0, int y0, double v0, Ref rf)
 {
@@ -946,6 +957,7 @@ __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __
         const int y = (int)(p / nx), x = (int)(p - (long long)y * nx);
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
+            if (!is_stored(SYM, i)) continue;   // its z mirror's value
             const int xs = wrapi(x + CX[i], nx), ys = wrapi(y + CY[i], ny);
             const double dx = (double)(xs - x0), dy = (double)(ys - y0);
             const double r = sqrt(dx * dx + dy * dy);
@@ -963,59 +975,70 @@ __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __
             const float e = (float)((f != 0.0) ? (fR - fB) / f : 0.0);
             const int k = vclass(i);
             const float h = (float)((f - w) * (double)SCL[k]) - __builtin_fmaf(rf.half(k), e, rf.mid(k));
-            pop[((long long)y * Q + i) * nx + x] = PairOf<ST>::make(h, e);
+            pop[((long long)y * n_slots(SYM) + slot_of(SYM, i)) * nx + x] = PairOf<ST>::make(h, e);
         }
     }
 }
 
 }  // namespace
 
-hipError_t launch_step_lp(int elem, int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
+hipError_t launch_step_lp(int elem, bool sym, int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
 {
     if (ntiles <= 0) return hipSuccess;
-    if (elem == 4) return launch_typed<float>(kind, a, ntiles, stream);
-    if (elem == 2) return launch_typed<_Float16>(kind, a, ntiles, stream);
+    if (elem == 4) return sym ? launch_typed<float, true>(kind, a, ntiles, stream) : launch_typed<float, false>(kind, a, ntiles, stream);
+    if (elem == 2)
+        return sym ? launch_typed<_Float16, true>(kind, a, ntiles, stream) : launch_typed<_Float16, false>(kind, a, ntiles, stream);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_lp_to_f64(int elem, const void* pop, int nx, long long S, long long n, double rmid, double rhalf,
-                            double* fR, double* fB, hipStream_t stream)
+hipError_t launch_lp_to_f64(int elem, bool sym, const void* pop, int nx, long long S, long long n, double rmid,
+                            double rhalf, double* fR, double* fB, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
     const Ref rf = make_lp_ref(rmid, rhalf);
-    if (elem == 4)
-        hipLaunchKernelGGL((lp_to_f64<float>), dim3(nb), dim3(256), 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
-    else
-        hipLaunchKernelGGL((lp_to_f64<_Float16>), dim3(nb), dim3(256), 0, stream, (const unsigned*)pop, nx, S, n, rf,
-                           fR, fB);
+    const dim3 g(nb), b(256);
+    if (elem == 4) {
+        if (sym) hipLaunchKernelGGL((lp_to_f64<float, true>), g, b, 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
+        else hipLaunchKernelGGL((lp_to_f64<float, false>), g, b, 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
+    } else {
+        if (sym) hipLaunchKernelGGL((lp_to_f64<_Float16, true>), g, b, 0, stream, (const unsigned*)pop, nx, S, n, rf, fR, fB);
+        else hipLaunchKernelGGL((lp_to_f64<_Float16, false>), g, b, 0, stream, (const unsigned*)pop, nx, S, n, rf, fR, fB);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_lp_recentre(int elem, void* pop, long long npairs, int nx, double mid0, double half0, double mid1,
-                              double half1, hipStream_t stream)
+hipError_t launch_lp_recentre(int elem, bool sym, void* pop, long long npairs, int nx, double mid0, double half0,
+                              double mid1, double half1, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((npairs + 255) / 256, 8192);
     const Ref from = make_lp_ref(mid0, half0), to = make_lp_ref(mid1, half1);
-    if (elem == 4)
-        hipLaunchKernelGGL((lp_recentre<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, npairs, nx, from, to);
-    else
-        hipLaunchKernelGGL((lp_recentre<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, npairs, nx, from,
-                           to);
+    const dim3 g(nb), b(256);
+    if (elem == 4) {
+        if (sym) hipLaunchKernelGGL((lp_recentre<float, true>), g, b, 0, stream, (f32x2*)pop, npairs, nx, from, to);
+        else hipLaunchKernelGGL((lp_recentre<float, false>), g, b, 0, stream, (f32x2*)pop, npairs, nx, from, to);
+    } else {
+        if (sym) hipLaunchKernelGGL((lp_recentre<_Float16, true>), g, b, 0, stream, (unsigned*)pop, npairs, nx, from, to);
+        else hipLaunchKernelGGL((lp_recentre<_Float16, false>), g, b, 0, stream, (unsigned*)pop, npairs, nx, from, to);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_lp_init_drop(int elem, void* pop, int nx, int ny, double R, double kp, int x0, int y0, double v0,
-                               double rmid, double rhalf, hipStream_t stream)
+hipError_t launch_lp_init_drop(int elem, bool sym, void* pop, int nx, int ny, double R, double kp, int x0, int y0,
+                               double v0, double rmid, double rhalf, hipStream_t stream)
 {
     const long long n = (long long)nx * ny;
     const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
     const Ref rf = make_lp_ref(rmid, rhalf);
-    if (elem == 4)
-        hipLaunchKernelGGL((lp_init_drop<float>), dim3(nb), dim3(256), 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0,
-                           v0, rf);
-    else
-        hipLaunchKernelGGL((lp_init_drop<_Float16>), dim3(nb), dim3(256), 0, stream, (unsigned*)pop, nx, ny, R, kp,
-                           x0, y0, v0, rf);
+    const dim3 g(nb), b(256);
+    if (elem == 4) {
+        if (sym) hipLaunchKernelGGL((lp_init_drop<float, true>), g, b, 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+        else hipLaunchKernelGGL((lp_init_drop<float, false>), g, b, 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+    } else {
+        if (sym)
+            hipLaunchKernelGGL((lp_init_drop<_Float16, true>), g, b, 0, stream, (unsigned*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+        else
+            hipLaunchKernelGGL((lp_init_drop<_Float16, false>), g, b, 0, stream, (unsigned*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+    }
     return hipGetLastError();
 }
 

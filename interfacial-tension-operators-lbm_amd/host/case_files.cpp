@@ -145,6 +145,18 @@ bool read_box(const std::string& path, int& nx, int& ny, int& nz, long long& n_f
     return true;
 }
 
+const char* precision_name(int precision)
+{
+    switch (precision) {
+    case LBM_PREC_F64: return "f64";
+    case LBM_PREC_F32: return "f32";
+    case LBM_PREC_F16: return "f16";
+    case LBM_PREC_F32_ZSYM: return "f32z";
+    case LBM_PREC_F16_ZSYM: return "f16z";
+    default: return "?";
+    }
+}
+
 bool read_options(const std::string& dir, Options& o, std::string& err)
 {
     std::ifstream in(join(dir, "itolbm.txt"));
@@ -165,6 +177,8 @@ bool read_options(const std::string& dir, Options& o, std::string& err)
                 if (val == "f64") o.precision = LBM_PREC_F64;
                 else if (val == "f32") o.precision = LBM_PREC_F32;
                 else if (val == "f16") o.precision = LBM_PREC_F16;
+                else if (val == "f32z") o.precision = LBM_PREC_F32_ZSYM;
+                else if (val == "f16z") o.precision = LBM_PREC_F16_ZSYM;
                 else throw std::invalid_argument(val);
             } else if (key == "raio") {
                 o.radius = std::stod(val);

@@ -46,7 +46,7 @@ bool read_box(const std::string& path, int& nx, int& ny, int& nz, long long& n_f
 
 // Optional <dir>/itolbm.txt, one "key value" per line, for what the reference fixes at compile
 // time.  Absent keys keep the reference's values.
-//   precisao        f64 | f32 | f16        storage form (default f64)
+//   precisao        f64 | f32 | f16 | f32z | f16z   storage form (default f64; z: z-symmetric storage)
 //   raio            initial drop radius    (default 0: nx / 4)
 //   espessura_ini   initial 10-90 width    (default 3.45; <= 0: sharp colour field)
 //   passos_diag     diagnostic interval    (default 500)
@@ -62,6 +62,8 @@ struct Options {
     bool recentre = true;
 };
 bool read_options(const std::string& dir, Options& o, std::string& err);
+// the itolbm.txt name of a storage form
+const char* precision_name(int precision);
 
 // ---------------------------------------------------------------- writers
 void laplace_header(std::ostream& out);

@@ -99,7 +99,7 @@ struct DropCase : Member {
         o << "   fat_R_B           = " << A_in << (op.force() ? "   ( = sigma )" : "   ( = A )") << "\n";
         o << "   sigma nominal     = " << sigma_nominal << "\n";
         o << "   v0                = " << v0 << "\n";
-        o << "   precisao          = " << (opt.precision == LBM_PREC_F64 ? "f64" : opt.precision == LBM_PREC_F32 ? "f32" : "f16")
+        o << "   precisao          = " << precision_name(opt.precision)
           << std::endl;
         f_lap.open(join(dir, "laplace.dat"));
         f_lap << std::setprecision(opt.digits);

@@ -229,7 +229,8 @@ def main():
     ap.add_argument("--saida", default="Lotes", help="root folder of the cases (default: Lotes)")
     ap.add_argument("--filtro", default="", help="only cases whose path contains this text")
     ap.add_argument("--passos", type=int, default=0, help="forces the number of steps (quick test)")
-    ap.add_argument("--precisao", default="f64", choices=["f64", "f32", "f16"], help="storage form on the GPU")
+    ap.add_argument("--precisao", default="f64", choices=["f64", "f32", "f16", "f32z", "f16z"],
+                    help="storage form on the GPU (f32z / f16z: z-symmetric storage)")
     ap.add_argument("--grupo", type=int, default=0, help="at most this many cases per GPU invocation (0: all)")
     ap.add_argument("--lista", action="store_true", help="only list the cases")
     ap.add_argument("--refaz", action="store_true", help="redo cases already done")

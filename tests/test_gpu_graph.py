@@ -40,7 +40,7 @@ def states(ctx):
     return [ctx.download(m.STATE_POST_COLLISION, case=i) for i in range(len(CASES))]
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 def test_graph_replay_equals_eager(prec):
     m = _pkg.load()
     say(f"graph vs eager: precision {prec}")

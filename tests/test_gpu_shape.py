@@ -1,8 +1,8 @@
 """GPU: a step's result does not depend on the tile shape or the pipeline depth.
 
 The autotuner picks, per kernel family, the rows per wavefront segment and (FP32 / FP16 storage) the
-number of rows whose loads are in flight, by timing; that is sound only if every shape computes every
-site the same way.  Contexts with fixed shapes (ITOLBM_LY rows per segment, ITOLBM_DEPTH depth) are
+number of rows whose loads are in flight, by timing real steps of the caller's; that is sound only if
+every shape computes every site the same way.  Contexts with fixed shapes (ITOLBM_LY rows per segment, ITOLBM_DEPTH depth) are
 compared bit for bit with one whose shapes the tuner chose.  The cases cover the four kernel families,
 widths that are not a multiple of the 64-column strip (lanes outside the case) and heights that are
 not a multiple of the segment (short last segments, look-ahead rows past a segment's end), and the
@@ -28,7 +28,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run(m, prec, shape):
+def run(m, prec, shape, steps=4):
     saved = {k: os.environ.get(k) for k in ("ITOLBM_LY", "ITOLBM_DEPTH")}
     try:
         if shape:
@@ -41,7 +41,7 @@ def run(m, prec, shape):
         try:
             for i, d in enumerate(descs):
                 ctx.init_drop(min(d.nx, d.ny) / 4, width=3.45, case=i)
-            ctx.step(4)
+            ctx.step(steps)
             ctx.step(1, record=True)
             out = []
             for i in range(len(CASES)):
@@ -58,7 +58,7 @@ def run(m, prec, shape):
                 os.environ[k] = v
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3, 4])
 def test_result_independent_of_tile_shape(prec):
     m = _pkg.load()
     say(f"tile shapes: precision {prec}, tuned")
@@ -68,3 +68,15 @@ def test_result_independent_of_tile_shape(prec):
         got = run(m, prec, shape)
         for j, (x, y) in enumerate(zip(ref, got)):
             assert np.array_equal(x, y), f"shape {shape}, array {j}"
+
+
+@pytest.mark.parametrize("prec", [0, 1, 3])
+def test_online_tuning_keeps_results(prec):
+    """A call long enough for the library to tune its tile shapes on the call's own steps (every
+    family's candidate shapes timed in turn) ends in the same state as a fixed shape."""
+    m = _pkg.load()
+    say(f"online tuning: precision {prec}, 340 steps")
+    ref = run(m, prec, (8, 2), steps=340)
+    got = run(m, prec, None, steps=340)
+    for j, (x, y) in enumerate(zip(ref, got)):
+        assert np.array_equal(x, y), f"array {j}"
