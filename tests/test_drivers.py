@@ -305,15 +305,21 @@ def test_reduced_precision_forms_through_the_programs(tmp_path):
     """The drop-in program with itolbm.txt selecting FP32 / FP16 storage (and re-centring after each
     diagnostic): laplace.dat's last row (a 96^2 drop, R = 19.2, 6000 steps) against the FP64 run of
     the same case.  Tolerances as in test_gpu_precision.py: delta_p (column 7) 2e-4 / 3e-2 relative,
-    radius (column 2) 1e-5 / 1e-3, colour masses (columns 12-13) 1e-5 / 5e-4."""
+    radius (column 2) 1e-5 / 1e-3, colour masses (columns 12-13) 1e-5 / 5e-4.  The z-symmetric forms
+    (f32z, f16z) must write the same bytes as f32 / f16."""
     rows = {}
-    for prec in ("f64", "f32", "f16"):
+    for prec in ("f64", "f32", "f16", "f32z", "f16z"):
         say(f"[gpu] Imm_OPS 96^2 P3 0.5 drop, precisao {prec}")
         d = str(tmp_path / prec)
         K.make_case(d, 96, "P3 0.5", 1.0, 0.225, 6000, options={"precisao": prec, "referencia": 1})
         r = K.run("Imm_OPS", [], d, timeout=600)
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
         rows[prec] = [float(v) for v in K.read_table(os.path.join(d, "laplace.dat"))[1][-1]]
+        if prec.endswith("z"):
+            # z-symmetric storage: every output file byte-equal to the full form's
+            for name in ("laplace.dat", "galilean.dat", "iso.dat"):
+                with open(os.path.join(d, name), "rb") as fz, open(os.path.join(str(tmp_path / prec[:-1]), name), "rb") as ff:
+                    assert fz.read() == ff.read(), (prec, name)
     ref = rows["f64"]
     for prec, tol in (("f32", (2e-4, 1e-5, 1e-5)), ("f16", (3e-2, 1e-3, 5e-4))):
         got = rows[prec]
