@@ -835,15 +835,17 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
 // family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
 // fixed: the pipeline depth (FP32/FP16 kernels) at the current rows per segment, then the rows per
 // segment (4 .. 64) at that depth, each candidate over W untimed + T timed steps; the two fastest are
-// timed again over 2T steps.  The steps used are subtracted from `left`; nothing is done unless
-// `left` covers the whole search.
+// timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
+// members changed).  The steps used are subtracted from `left`; nothing is done unless `left`
+// covers the whole search.
 int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
 {
     constexpr int W = 1, T = 6;
     constexpr int LYS[] = {4, 8, 16, 32, 64};
     auto depths_of = [&](const Group& g) { return c->cases[g.members[0]].elem == 8 ? 1 : 3; };
     long long budget = 0;
-    for (const Group* g : act) budget += (long long)(depths_of(*g) + 4) * (W + T) + 2 * (W + 2 * T);
+    for (const Group* g : act)
+        if (!g->tuned) budget += (long long)(depths_of(*g) + 4) * (W + T) + 2 * (W + 2 * T);
     if (budget > left) return LBM_OK;
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
@@ -862,6 +864,7 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         return LBM_OK;
     };
     for (Group* g : act) {
+        if (g->tuned) continue;   // a family whose members did not change keeps its shape
         auto apply = [&](const Cand& cd) -> int {
             g->depth = cd.depth;
             g->ly.assign(g->members.size(), cd.ly);
