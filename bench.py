@@ -56,7 +56,7 @@ SIGMA = 0.1
 NX = NY = 512
 R = 100.0
 PEAK_HBM_GBS = 8000.0
-SWEEP_WARMUP = 320   # untimed sweep steps (the batched path tunes its tile shapes on them)
+SWEEP_WARMUP = 400   # untimed sweep steps (the batched path tunes its tile shapes on them)
 B_LU = {"f64": 608, "f32": 304, "f16": 152, "f32z": 224, "f16z": 112}
 PREC_ID = {"f64": 0, "f32": 1, "f16": 2, "f32z": 3, "f16z": 4}
 KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth,false> (P1/P3 family)",
@@ -261,8 +261,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=400,
-                    help="untimed steps first (the library tunes its tile shapes on the first ~300 of them)")
+    ap.add_argument("--warmup", type=int, default=450,
+                    help="untimed steps first (the library tunes its tile shapes on the first ~360 of them)")
     ap.add_argument("--prec", default="f32z", choices=list(B_LU), help="headline storage form")
     ap.add_argument("--forms", default="f64,f32,f16,f16z", help="forms measured beside the headline")
     ap.add_argument("--sweep-steps", type=int, default=1000, help="0 skips the 168-case sweep")

@@ -67,7 +67,7 @@ struct Group {
     int cap = 0;                   // entries dtab can hold
     int tcap = 0;                  // entries dtile can hold
     bool written = false;
-    bool tuned = false;            // rows per segment chosen by timing (group_tune)
+    bool tuned = false;            // tile shape chosen by timing (online_tune)
     int depth = 0;                 // pipeline depth of the FP32/FP16 kernels (0 = default)
     long long version = 0;         // bumped whenever the table is rewritten
 };
@@ -697,108 +697,40 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
                            : launch_step_lp(k0.elem, k0.sym, g.kind, a, g.ntiles, st));
     return LBM_OK;
 }
-int launch_group(lbm_ctx* c, Group& g, int parity, int record) { return launch_group(c, g, parity, record, c->stream); }
 
+// a tile shape and its measured time per step
 struct Cand {
     int ly, depth;
     float ms;
 };
 
-// Tile shapes a family's tuning tries: rows per segment (distinct after clamping to the tallest
-// member) and, for the FP32/FP16 kernels, the pipeline depth.
-std::vector<Cand> tile_candidates(const lbm_ctx* c, const Group& g)
-{
-    int ny_max = 1;
-    for (int id : g.members) ny_max = std::max(ny_max, c->cases[id].d.ny);
-    const bool pipelined = c->cases[g.members[0]].elem != 8;   // the FP32/FP16 kernels of every family
-    std::vector<Cand> cands;
-    for (int depth : {1, 2, 3}) {
-        if (!pipelined && depth > 1) break;
-        int last_eff = -1;
-        for (int ly : {1, 2, 4, 8, 16, 32, 64}) {
-            const int eff = std::min(ly, ny_max);
-            if (eff == last_eff) continue;
-            last_eff = eff;
-            cands.push_back({ly, pipelined ? depth : 0, 0.f});
-        }
-    }
-    return cands;
-}
-
-// Autotuner of the tile shape: rows per wavefront segment (the strip width is fixed by the kernel)
-// and, for the FP32/FP16 kernels, the pipeline depth (rows in flight per wavefront;
-// deeper pipelines hold more registers, so fewer wavefronts fit a SIMD).  Each candidate's launch is
-// timed at the group's current parity: it reads the current state and writes the other buffer, so
-// repeating it leaves the simulation unchanged, and the results of a step do not depend on the
-// shape or depth (every site is computed the same way), so the choice only changes speed.  Short
-// rows re-read more halo rows; long rows give fewer, longer wavefronts.
-int group_tune(lbm_ctx* c, Group& g)
-{
-    g.tuned = true;
-    if (std::getenv("ITOLBM_LY")) {   // a fixed shape; ITOLBM_DEPTH fixes the depth too (0 = default)
-        if (const char* e = std::getenv("ITOLBM_DEPTH")) g.depth = std::max(0, std::min(3, std::atoi(e)));
-        return LBM_OK;
-    }
-    if (!c->ev[0]) {
-        HIPCHK(c, hipEventCreate(&c->ev[0]));
-        HIPCHK(c, hipEventCreate(&c->ev[1]));
-    }
-    std::vector<Cand> cands = tile_candidates(c, g);
-    // average launch time of a candidate over `reps` back-to-back launches after one warm-up (the
-    // launches re-read the same input, which the last-level cache then partly serves: the times are
-    // 5-10 % below those of launches in real steps)
-    auto time_of = [&](const Cand& cd, int reps, float& ms) -> int {
-        g.depth = cd.depth;
-        g.ly.assign(g.members.size(), cd.ly);
-        g.written = false;
-        if (int r = group_sync(c, g)) return r;
-        const int parity = g.k & 1;
-        if (int r = launch_group(c, g, parity, 0)) return r;
-        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-        for (int k = 0; k < reps; ++k)
-            if (int r = launch_group(c, g, parity, 0)) return r;
-        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-        HIPCHK(c, hipEventSynchronize(c->ev[1]));
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-        ms /= (float)reps;
-        return LBM_OK;
-    };
-    // a first pass over every candidate, then the three fastest timed again, longer
-    for (Cand& cd : cands)
-        if (int r = time_of(cd, 3, cd.ms)) return r;
-    std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
-    const size_t top = std::min<size_t>(3, cands.size());
-    for (size_t j = 0; j < top; ++j)
-        if (int r = time_of(cands[j], 8, cands[j].ms)) return r;
-    std::sort(cands.begin(), cands.begin() + top, [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
-    g.ly.assign(g.members.size(), cands[0].ly);
-    g.depth = cands[0].depth;
-    g.written = false;
-    if (std::getenv("ITOLBM_TUNE_LOG"))
-        std::fprintf(stderr, "itolbm tuner: family %d, %zu cases, element %d B: %d rows per segment, depth %d, %.1f us\n",
-                     g.kind, g.members.size(), c->cases[g.members[0]].elem, cands[0].ly, cands[0].depth,
-                     1000.0 * cands[0].ms);
-    return group_sync(c, g);
-}
-
 // Tables, aux buffers and tile shape of a group, before its launches are queued.
-int group_prepare(lbm_ctx* c, Group& g, bool rec, bool tune = true)
+int group_prepare(lbm_ctx* c, Group& g, bool rec)
 {
     if (rec)
         for (int id : g.members)
             if (int r = ensure_aux(c, c->cases[id])) return r;
-    if (int r = group_sync(c, g)) return r;
-    if (tune && !g.tuned && c->autotune)
-        if (int r = group_tune(c, g)) return r;
-    return LBM_OK;
+    return group_sync(c, g);
 }
 
-int group_step(lbm_ctx* c, Group& g, int n_steps, int flags, hipStream_t st, bool prepared)
+// Whether a group still needs its tile shape tuned: ITOLBM_LY (and ITOLBM_DEPTH, 0 = the kernel's
+// default) fix the shape instead.
+bool needs_tuning(lbm_ctx* c, Group& g)
+{
+    if (g.tuned || !c->autotune) return false;
+    if (std::getenv("ITOLBM_LY")) {
+        if (const char* e = std::getenv("ITOLBM_DEPTH")) g.depth = std::max(0, std::min(3, std::atoi(e)));
+        g.tuned = true;
+        return false;
+    }
+    return true;
+}
+
+// Queues n steps of a prepared group on stream st (the last one recording aux when asked).
+int group_step(lbm_ctx* c, Group& g, int n_steps, int flags, hipStream_t st)
 {
     if (g.members.empty() || n_steps <= 0) return LBM_OK;
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
-    if (!prepared)
-        if (int r = group_prepare(c, g, rec)) return r;
     for (int s = 0; s < n_steps; ++s) {
         if (int r = launch_group(c, g, g.k & 1, (rec && s == n_steps - 1) ? 1 : 0, st)) return r;
         g.k++;
@@ -808,19 +740,17 @@ int group_step(lbm_ctx* c, Group& g, int n_steps, int flags, hipStream_t st, boo
     return LBM_OK;
 }
 
-int group_step(lbm_ctx* c, Group& g, int n_steps, int flags) { return group_step(c, g, n_steps, flags, c->stream, false); }
-
 // Queues n steps of the active families (already prepared).  The families advance disjoint sets of
 // cases, so with several of them each runs its n steps on a stream of its own, forked from and
 // joined back to the context's stream: small launches of different families fill the GPU together.
 int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int flags)
 {
-    if (act.size() == 1) return group_step(c, *act[0], n_steps, flags, c->stream, true);
+    if (act.size() == 1) return group_step(c, *act[0], n_steps, flags, c->stream);
     HIPCHK(c, hipEventRecord(c->fork, c->stream));
     for (size_t j = 0; j < act.size(); ++j) {
         hipStream_t st = c->side[j];
         HIPCHK(c, hipStreamWaitEvent(st, c->fork, 0));
-        if (int r = group_step(c, *act[j], n_steps, flags, st, true)) return r;
+        if (int r = group_step(c, *act[j], n_steps, flags, st)) return r;
         HIPCHK(c, hipEventRecord(c->join[j], st));
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[j], 0));
     }
@@ -834,18 +764,18 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
 // favoured the wrong shapes), and all active families run together, as they do in a step, so a
 // family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
 // fixed: the pipeline depth (FP32/FP16 kernels) at the current rows per segment, then the rows per
-// segment (4 .. 64) at that depth, each candidate over W untimed + T timed steps; the two fastest are
+// segment (1 .. 64) at that depth, each candidate over W untimed + T timed steps; the two fastest are
 // timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
 // members changed).  The steps used are subtracted from `left`; nothing is done unless `left`
 // covers the whole search.
 int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
 {
     constexpr int W = 1, T = 6;
-    constexpr int LYS[] = {4, 8, 16, 32, 64};
+    constexpr int LYS[] = {1, 2, 4, 8, 16, 32, 64};   // short segments suit small cases (more wavefronts)
     auto depths_of = [&](const Group& g) { return c->cases[g.members[0]].elem == 8 ? 1 : 3; };
     long long budget = 0;
     for (const Group* g : act)
-        if (!g->tuned) budget += (long long)(depths_of(*g) + 4) * (W + T) + 2 * (W + 2 * T);
+        if (!g->tuned) budget += (long long)(depths_of(*g) + 6) * (W + T) + 2 * (W + 2 * T);
     if (budget > left) return LBM_OK;
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
@@ -976,7 +906,19 @@ int lbm_step_case(lbm_ctx* c, int id, int n_steps, int flags)
 {
     if (int r = check_case(c, id)) return r;
     if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
-    return group_step(c, c->single[id], n_steps, flags);
+    Group& g = c->single[id];
+    if (n_steps == 0 || g.members.empty()) return LBM_OK;
+    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    if (int r = group_prepare(c, g, rec)) return r;
+    int left = n_steps;
+    if (needs_tuning(c, g)) {   // as lbm_step: the tile shape is tuned on the call's own steps
+        int avail = left - (rec ? 1 : 0);
+        const int before = avail;
+        std::vector<Group*> one{&g};
+        if (int r = online_tune(c, one, avail)) return r;
+        left -= before - avail;
+    }
+    return group_step(c, g, left, flags, c->stream);
 }
 
 int lbm_step(lbm_ctx* c, int n_steps, int flags)
@@ -991,15 +933,8 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     bool untuned = false;
     for (Group* g : act) {
-        if (int r = group_prepare(c, *g, rec, false)) return r;
-        if (!g->tuned && c->autotune) {
-            if (std::getenv("ITOLBM_LY")) {   // a fixed shape; ITOLBM_DEPTH fixes the depth too (0 = default)
-                if (const char* e = std::getenv("ITOLBM_DEPTH")) g->depth = std::max(0, std::min(3, std::atoi(e)));
-                g->tuned = true;
-            } else {
-                untuned = true;
-            }
-        }
+        if (int r = group_prepare(c, *g, rec)) return r;
+        if (needs_tuning(c, *g)) untuned = true;
     }
     if (act.size() > 1 && !c->fork) {
         HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
