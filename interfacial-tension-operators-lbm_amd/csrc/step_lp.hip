@@ -488,6 +488,7 @@ struct TileCtxLP {
     gdouble* aux;
     long long S;
     int strip, y0, y1;
+    bool rev;   // odd segments march from y1 - 1 down to y0 (see lp_pipe)
 };
 
 __device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtxLP& t)
@@ -505,6 +506,7 @@ __device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtxLP& t)
     const int seg = local / d.nstrip;
     t.y0 = seg * d.ly;
     t.y1 = min(t.y0 + d.ly, d.cc.ny);
+    t.rev = (seg & 1) != 0;
     return t.y0 < d.cc.ny;
 }
 
@@ -616,16 +618,16 @@ __device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& 
 // every buffer role is a compile-time constant).  The re-read of row y is issued before the wait on
 // row y + 2, so that the collision's wait leaves the rows ahead in flight (vector memory counts
 // retire in issue order).
-template <typename ST, int KIND, int DEPTH, bool SYM>
-__global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
+// REV marches from y1 - 1 down to y0 (odd segments), so that adjacent segments reach the rows they
+// share at the same time (as in lp_pipe).  The window P / HW / pd holds rows in march order ("ahead"
+// is y - 1 .. y - 2 then); every three-row stencil reads it through G, which gives the window index
+// of geometric rows y - 1, y, y + 1, so the arithmetic per site is that of the forward march.
+template <typename ST, int KIND, int DEPTH, bool SYM, bool REV>
+__device__ __forceinline__ void f12_rows(const TileCtxLP& t)
 {
-    static_assert(KIND == KIND_F1 || KIND == KIND_F2, "two-row stencils only");
-    static_assert(DEPTH >= 1 && DEPTH <= 3, "pipeline depth 1..3");
     constexpr int NA = DEPTH - 1;         // rows loaded ahead of the iteration that uses them
     constexpr int NU = NA > 0 ? NA : 1;   // unroll of the row loop
     using R = typename PairOf<ST>::raw;
-    TileCtxLP t;
-    if (!tile_ctx(launch, t)) return;
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
     const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
@@ -634,13 +636,15 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     gst<R>* out = (gst<R>*)t.out;
     Halo2LP<ST, SYM> h;
     h.init(nx, t.strip);
-    const int y0 = t.y0, y1 = t.y1;
+    const int y0 = t.y0, y1 = t.y1, n = y1 - y0;
+    auto row = [&](int kk) { return REV ? y1 - 1 - kk : y0 + kk; };
+    auto G = [](int r) { return REV ? 2 - r : r; };
 
     R cur[Q];
-    R ahead[NU][Q];   // row r + 2 in buffer (r - y0) mod NU (DEPTH > 1)
+    R ahead[NU][Q];   // row(k + 2) in buffer k mod NU (DEPTH > 1)
     R ha[NU], hb[NU];
-    float P[5], HW[5][4];
-    double pd[3];   // FP64 colour field of rows y, y+1, y+2 (for their collision)
+    float P[5], HW[5][4];   // rows row(k - 2) .. row(k + 2)
+    double pd[3];   // FP64 colour field of rows row(k), row(k + 1), row(k + 2) (for their collision)
     auto phi_row = [&](int y, float (&hw)[4]) {
         load_raw<ST, SYM>(in, nx, ny, y, L, cur);
         R va, vb;
@@ -649,63 +653,65 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
         return raw_phase<ST>(cur, rf);
     };
     auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
-        const float hl[3] = {HW[j][1], HW[j + 1][1], HW[j + 2][1]};
-        const float hr[3] = {HW[j][2], HW[j + 1][2], HW[j + 2][2]};
-        return aux_of<KIND>(k, gather9h(P[j], P[j + 1], P[j + 2], hl, hr, L.lane));
+        const float hl[3] = {HW[j + G(0)][1], HW[j + 1][1], HW[j + G(2)][1]};
+        const float hr[3] = {HW[j + G(0)][2], HW[j + 1][2], HW[j + G(2)][2]};
+        return aux_of<KIND>(k, gather9h(P[j + G(0)], P[j + 1], P[j + G(2)], hl, hr, L.lane));
     };
     auto aux_halo = [&](int j, int side) {   // n / T at c0-1 (side 0) or c0+64 (side 1)
-        Nine n;
+        Nine nn;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
+            const int w = j + G(r);
             if (side == 0) {
-                n.v[r][0] = HW[j + r][0];
-                n.v[r][1] = HW[j + r][1];
-                n.v[r][2] = lane_value(P[j + r], 0);
+                nn.v[r][0] = HW[w][0];
+                nn.v[r][1] = HW[w][1];
+                nn.v[r][2] = lane_value(P[w], 0);
             } else {
-                n.v[r][0] = lane_value(P[j + r], 63);
-                n.v[r][1] = HW[j + r][2];
-                n.v[r][2] = HW[j + r][3];
+                nn.v[r][0] = lane_value(P[w], 63);
+                nn.v[r][1] = HW[w][2];
+                nn.v[r][2] = HW[w][3];
             }
         }
-        return aux_of<KIND>(k, n);
+        return aux_of<KIND>(k, nn);
     };
-    P[0] = (float)phi_row(y0 - 2, HW[0]);
-    P[1] = (float)phi_row(y0 - 1, HW[1]);
-    pd[1] = phi_row(y0, HW[2]);
+    P[0] = (float)phi_row(row(-2), HW[0]);
+    P[1] = (float)phi_row(row(-1), HW[1]);
+    pd[1] = phi_row(row(0), HW[2]);
     P[2] = (float)pd[1];
-    pd[2] = phi_row(y0 + 1, HW[3]);
+    pd[2] = phi_row(row(1), HW[3]);
     P[3] = (float)pd[2];
     AuxF am = aux_lanes(0), a0 = aux_lanes(1);
-    // n / T at the halo columns of rows y-1 and y (carried down the rows; row y+1's is new each row)
+    // n / T at the halo columns of rows row(k-1) and row(k) (carried along; row(k+1)'s is new each row)
     AuxF hl[3], hr[3];
     hl[0] = aux_halo(0, 0); hr[0] = aux_halo(0, 1);
     hl[1] = aux_halo(1, 0); hr[1] = aux_halo(1, 1);
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
-        load_raw<ST, SYM>(in, nx, ny, y0 + 2 + r, L, ahead[r]);
-        h.load(in, nx, ny, y0 + 2 + r, ha[r], hb[r]);
+        load_raw<ST, SYM>(in, nx, ny, row(2 + r), L, ahead[r]);
+        h.load(in, nx, ny, row(2 + r), ha[r], hb[r]);
     }
 
-    for (int y = y0; y < y1; y += NU) {
+    for (int k0 = 0; k0 < n; k0 += NU) {
 #pragma unroll
         for (int j = 0; j < NU; ++j) {
-            const int yy = y + j;
-            if (yy >= y1) break;
+            const int kk = k0 + j;
+            if (kk >= n) break;
+            const int yy = row(kk);
             pd[0] = pd[1];
             pd[1] = pd[2];
             if constexpr (NA > 0) {
                 load_raw<ST, SYM>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
-                __builtin_amdgcn_sched_barrier(0);      // issued before the wait on row yy + 2
-                pd[2] = raw_phase<ST>(ahead[j], rf);    // row yy + 2 (waits for its loads)
+                __builtin_amdgcn_sched_barrier(0);      // issued before the wait on row(kk + 2)
+                pd[2] = raw_phase<ST>(ahead[j], rf);    // row(kk + 2) (waits for its loads)
                 halo2_phase<ST>(ha[j], hb[j], rf, HW[4]);
-                // row yy + 2 + NA, used NA iterations on; past the segment's last row the loads are
+                // row(kk + 2 + NA), used NA iterations on; past the segment's last row the loads are
                 // issued empty, so every path leaves the same loads outstanding
-                const bool live = yy + NA < y1;
-                load_raw<ST, SYM>(in, nx, ny, yy + 2 + NA, L, ahead[j], live);
-                h.load(in, nx, ny, live ? yy + 2 + NA : yy, ha[j], hb[j]);
+                const bool live = kk + NA < n;
+                load_raw<ST, SYM>(in, nx, ny, row(kk + 2 + NA), L, ahead[j], live);
+                h.load(in, nx, ny, live ? row(kk + 2 + NA) : yy, ha[j], hb[j]);
                 __builtin_amdgcn_sched_barrier(0);
             } else {
-                pd[2] = phi_row(yy + 2, HW[4]);
+                pd[2] = phi_row(row(kk + 2), HW[4]);
                 load_raw<ST, SYM>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
             }
             P[4] = (float)pd[2];
@@ -713,9 +719,10 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             hl[2] = aux_halo(2, 0);
             hr[2] = aux_halo(2, 1);
             auto field = [&](float AuxF::*m) {
-                const float l3[3] = {hl[0].*m, hl[1].*m, hl[2].*m};
-                const float r3[3] = {hr[0].*m, hr[1].*m, hr[2].*m};
-                return gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
+                const float l3[3] = {hl[G(0)].*m, hl[1].*m, hl[G(2)].*m};
+                const float r3[3] = {hr[G(0)].*m, hr[1].*m, hr[G(2)].*m};
+                return REV ? gather9h(ap.*m, a0.*m, am.*m, l3, r3, L.lane)
+                          : gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
             };
             float Fx, Fy;
             if constexpr (KIND == KIND_F1) {
@@ -727,9 +734,9 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
                 Fx = dx_iso(field(&AuxF::a0)) + dy_iso(txy);
                 Fy = dx_iso(txy) + dy_iso(field(&AuxF::a1));
             }
-            const float hl1[3] = {HW[1][1], HW[2][1], HW[3][1]};
-            const float hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
-            const Nine p9 = gather9h(P[1], P[2], P[3], hl1, hr0, L.lane);
+            const float hl1[3] = {HW[1 + G(0)][1], HW[2][1], HW[1 + G(2)][1]};
+            const float hr0[3] = {HW[1 + G(0)][2], HW[2][2], HW[1 + G(2)][2]};
+            const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0, L.lane);
             const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
             collide_store<ST, true, SYM>(t, k, rf, L, out, yy, cur, pd[0], gx, gy, Fx, Fy);
 #pragma unroll
@@ -744,6 +751,19 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
             hr[0] = hr[1]; hr[1] = hr[2];
         }
     }
+}
+
+template <typename ST, int KIND, int DEPTH, bool SYM>
+__global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
+{
+    static_assert(KIND == KIND_F1 || KIND == KIND_F2, "two-row stencils only");
+    static_assert(DEPTH >= 1 && DEPTH <= 3, "pipeline depth 1..3");
+    TileCtxLP t;
+    if (!tile_ctx(launch, t)) return;
+    // FP16: the second body's registers cost more than the shared rows save (measured), so FP16
+    // segments all march forward; the sites' results are the same either way
+    if (std::is_same<ST, float>::value && t.rev) f12_rows<ST, KIND, DEPTH, SYM, std::is_same<ST, float>::value>(t);
+    else f12_rows<ST, KIND, DEPTH, SYM, false>(t);
 }
 
 // Halo of an aligned strip for the pair layout: one wavefront load per row (halo_lane_lp).  The address is
@@ -821,50 +841,62 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     gst<R>* out = (gst<R>*)t.out;
     HaloLP<ST, SYM> h;
     h.init(nx, t.strip);
-    const int y0 = t.y0, y1 = t.y1;
+    const int y0 = t.y0, y1 = t.y1, n = y1 - y0;
+    // Segments march in alternating directions: even ones from y0 to y1 - 1, odd ones from y1 - 1 to
+    // y0.  Two adjacent segments then reach the rows they share (each reads one row beyond its own)
+    // at the same time, both at the start of their march or both at its end, while the rows are in
+    // the L2 both use (adjacent tiles run on one XCD); marching the same way, one read them at its
+    // start and the other a whole march later.  The k-th row visited is row(k); row(-1) and row(n)
+    // are the rows just outside the segment.
+    const bool rev = t.rev;
+    auto row = [&](int kk) { return rev ? y1 - 1 - kk : y0 + kk; };
 
-    // row r lives in buffer (r - y0) mod NB; row y0 - 1 in buffer NB - 1
+    // visited row k lives in buffer k mod NB; row(-1) in buffer NB - 1
     R buf[NB][Q];
     R hv[NB];
-    float p[3], hl[3], hr[3];
-    load_raw<ST, SYM>(in, nx, ny, y0 - 1, L, buf[NB - 1]);
-    hv[NB - 1] = h.load(in, nx, ny, y0 - 1);
+    float p[3], hl[3], hr[3];   // colour field of rows row(k-1), row(k), row(k+1)
+    load_raw<ST, SYM>(in, nx, ny, row(-1), L, buf[NB - 1]);
+    hv[NB - 1] = h.load(in, nx, ny, row(-1));
 #pragma unroll
     for (int r = 0; r < DEPTH; ++r) {
-        load_raw<ST, SYM>(in, nx, ny, y0 + r, L, buf[r]);
-        hv[r] = h.load(in, nx, ny, y0 + r);
+        load_raw<ST, SYM>(in, nx, ny, row(r), L, buf[r]);
+        hv[r] = h.load(in, nx, ny, row(r));
     }
     p[1] = (float)raw_phase<ST>(buf[NB - 1], rf);
     halo_phase<ST>(hv[NB - 1], rf, hl[1], hr[1]);
     double phn = raw_phase<ST>(buf[0], rf);   // phi of the next row to collide
     p[2] = (float)phn;
     halo_phase<ST>(hv[0], rf, hl[2], hr[2]);
-    for (int y = y0; y < y1; y += NB) {
+    for (int k0 = 0; k0 < n; k0 += NB) {
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const int yy = y + j;
-            if (yy < y1) {
+            const int kk = k0 + j;
+            if (kk < n) {
                 R(&cur)[Q] = buf[j];
                 R(&nxt)[Q] = buf[(j + 1) % NB];
-                R(&far)[Q] = buf[(j + DEPTH) % NB];   // held row yy - 1, free now
-                // rows up to y1 feed the gradients; past it the loads are issued empty, so every
+                R(&far)[Q] = buf[(j + DEPTH) % NB];   // held row(kk - 1), free now
+                // rows up to row(n) feed the gradients; past it the loads are issued empty, so every
                 // path leaves the same loads outstanding
-                const bool live = yy + DEPTH <= y1;
-                load_raw<ST, SYM>(in, nx, ny, yy + DEPTH, L, far, live);
-                hv[(j + DEPTH) % NB] = h.load(in, nx, ny, live ? yy + DEPTH : yy);
+                const bool live = kk + DEPTH <= n;
+                load_raw<ST, SYM>(in, nx, ny, row(kk + DEPTH), L, far, live);
+                hv[(j + DEPTH) % NB] = h.load(in, nx, ny, live ? row(kk + DEPTH) : row(kk));
                 __builtin_amdgcn_sched_barrier(0);
                 p[0] = p[1]; p[1] = p[2];
                 hl[0] = hl[1]; hl[1] = hl[2];
                 hr[0] = hr[1]; hr[1] = hr[2];
                 const double phc = phn;
-                phn = raw_phase<ST>(nxt, rf);   // row yy + 1 (waits for it)
+                phn = raw_phase<ST>(nxt, rf);   // row(kk + 1) (waits for it)
                 p[2] = (float)phn;
                 halo_phase<ST>(hv[(j + 1) % NB], rf, hl[2], hr[2]);
                 if constexpr (DEPTH > 1) {
 #pragma unroll
                     for (int i = 0; i < Q; ++i) opaque(cur[i]);   // re-decode rather than keep decoded values alive
                 }
-                collide_row<ST, KIND, SYM>(t, k, rf, L, out, yy, cur, p, hl, hr, phc);
+                // the stencil takes rows y - 1, y, y + 1
+                const float pg[3] = {rev ? p[2] : p[0], p[1], rev ? p[0] : p[2]};
+                const float hlg[3] = {rev ? hl[2] : hl[0], hl[1], rev ? hl[0] : hl[2]};
+                const float hrg[3] = {rev ? hr[2] : hr[0], hr[1], rev ? hr[0] : hr[2]};
+                collide_row<ST, KIND, SYM>(t, k, rf, L, out, row(kk), cur, pg, hlg, hrg, phc);
             }
         }
     }
