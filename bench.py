@@ -171,7 +171,8 @@ def sweep_cases(m):
         for sigma in (0.01, 0.05, 0.1, 0.5):
             for op, kind in OPS:
                 r = int(radii[k])
-                cases.append((m.case_desc(4 * r, 4 * r, op, tau, amplitude(kind, tau, sigma), 0.8), float(r)))
+                cases.append((m.case_desc(4 * r, 4 * r, op, tau, amplitude(kind, tau, sigma), 0.8), float(r),
+                              f"{op} tau={tau} sigma={sigma} R={r}"))
                 k += 1
     return cases
 
@@ -180,13 +181,13 @@ def bench_sweep(m, timer, stream, form, steps):
     """168 cases: plain FP64 one launch per case and step vs the batched form, `steps` steps each."""
     import numpy as np
     cases = sweep_cases(m)
-    descs = [c for c, _ in cases]
+    descs = [c for c, _, _ in cases]
     sites = sum(d.nx * d.ny for d in descs)
     res = {}
     for label, prec in (("f64_per_case", 0), (f"{form}_batched", PREC_ID[form])):
         ctx = m.Context(descs, precision=prec)
         ctx.set_stream(stream.cuda_stream)
-        for i, (d, r) in enumerate(cases):
+        for i, (d, r, _) in enumerate(cases):
             ctx.init_drop(r, width=3.45, case=i)
         # untimed warm-up (first launches, tile-shape tuning), the same steps on both paths
         if prec == 0:
@@ -214,6 +215,7 @@ def bench_sweep(m, timer, stream, form, steps):
     a, b = res["f64_per_case"], res[f"{form}_batched"]
     good = (a["ok"] == 1) & (b["ok"] == 1) & np.isfinite(a["dp"]) & np.isfinite(b["dp"]) & (np.abs(a["dp"]) > 1e-9)
     err = np.abs(b["dp"][good] - a["dp"][good]) / np.abs(a["dp"][good])
+    labels = [lab for (_, _, lab), g in zip(cases, good) if g]
     per_case_20k = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
     return {"cases": len(descs), "sites": sites, "steps": steps, "warmup": SWEEP_WARMUP, "form": form,
             "f64_per_case_ms_per_step": a["ms_per_step"], "batched_ms_per_step": b["ms_per_step"],
@@ -222,7 +224,9 @@ def bench_sweep(m, timer, stream, form, steps):
                                          f"{form}_batched": per_case_20k(b["ms_per_step"])},
             "dp_compared_cases": int(good.sum()),
             "dp_rel_err_median": float(np.median(err)) if err.size else None,
-            "dp_rel_err_max": float(err.max()) if err.size else None}
+            "dp_rel_err_max": float(err.max()) if err.size else None,
+            "dp_within_1e-4": int((err <= 1e-4).sum()),
+            "dp_worst_cases": [[labels[j], float(err[j])] for j in np.argsort(err)[::-1][:3]] if err.size else []}
 
 
 def cpu_baseline(max_threads):
