@@ -89,3 +89,30 @@ def test_zsym_upload(zsym):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("full,zsym", PAIRS, ids=["f32", "f16"])
+def test_zsym_band_equals_full(full, zsym):
+    """The inclined-band program's state (lbm_init_band, Imm_INCL.cpp:309-334) in both storages: the
+    same populations after each call and the same inclined-interface diagnostics."""
+    m = _pkg.load()
+    cases = [(96, 96, "P3 0.5", 1.0, 0.225, 2, 1), (80, 80, "F2", 0.8, 0.1, 1, 1), (128, 64, "F1", 1.0, 0.1, 1, 0)]
+    say(f"z-symmetric storage {zsym} vs {full}: inclined band")
+    descs = [m.case_desc(nx, ny, op, tau, A) for nx, ny, op, tau, A, _, _ in cases]
+    a, b = m.Context(descs, precision=full), m.Context(descs, precision=zsym)
+    try:
+        for i, c in enumerate(cases):
+            a.init_band(c[5], c[6], case=i)
+            b.init_band(c[5], c[6], case=i)
+        for call, n in enumerate([3, 20]):
+            a.step(n, record=True)
+            b.step(n, record=True)
+            for i, c in enumerate(cases):
+                for x, y in zip(a.download(m.STATE_POST_COLLISION, case=i), b.download(m.STATE_POST_COLLISION, case=i)):
+                    assert np.array_equal(x, y), f"populations, call {call}, case {i}"
+                da, db = a.diag_incl(c[5], c[6], case=i).as_dict(), b.diag_incl(c[5], c[6], case=i).as_dict()
+                for k, va in da.items():   # equal, or both NaN (a fit that did not converge)
+                    assert va == db[k] or (va != va and db[k] != db[k]), f"diagnostic {k}, call {call}, case {i}"
+    finally:
+        a.close()
+        b.close()
