@@ -188,7 +188,7 @@ def test_diag_incl_matches_oracle(op, A):
     ctx.close()
 
 
-def test_north_star_drop_1000_steps():
+def test_drop_100x100_1000_steps():
     """100^2 sharp drop R = 20, P3 chi = 0.5, tau = 1, A = 0.225, beta = 0.8: delta_p, radius and the
     interface Fourier modes after 1000 steps within 1e-10 of the oracle (the populations are equal)."""
     say("[gpu] 100x100 sharp drop R=20, 1000 steps")
