@@ -56,7 +56,7 @@ SIGMA = 0.1
 NX = NY = 512
 R = 100.0
 PEAK_HBM_GBS = 8000.0
-SWEEP_WARMUP = 450   # untimed sweep steps (the batched path tunes its tile shapes on them)
+SWEEP_WARMUP = 400   # untimed sweep steps (the batched path tunes its tile shapes on them)
 B_LU = {"f64": 608, "f32": 304, "f16": 152, "f32z": 224, "f16z": 112}
 PREC_ID = {"f64": 0, "f32": 1, "f16": 2, "f32z": 3, "f16z": 4}
 KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth,false> (P1/P3 family)",
@@ -134,7 +134,7 @@ def bench_large(m, timer, stream, form, steps):
     ctx = m.Context(m.case_desc(n, n, "P3 0.5", 1.0, amplitude("P", 1.0), 0.8), precision=PREC_ID[form])
     ctx.set_stream(stream.cuda_stream)
     ctx.init_drop(1000.0, width=-1.0, v0=0.1)
-    ctx.step(150)   # untimed: the library tunes the tile shape on these steps (~105 of them)
+    ctx.step(100)   # untimed: the library tunes the tile shape on these steps
     gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
     ctx.close()
     avg = gpu_s / steps

@@ -69,7 +69,6 @@ struct Group {
     bool written = false;
     bool tuned = false;            // tile shape chosen by timing (online_tune)
     int depth = 0;                 // pipeline depth of the FP32/FP16 kernels (0 = default)
-    int waves = 1;                 // FP32/FP16 kernels: wavefronts (adjacent strips) per workgroup
     long long version = 0;         // bumped whenever the table is rewritten
 };
 
@@ -650,9 +649,8 @@ int group_sync(lbm_ctx* c, Group& g)
         d.ly = std::max(1, std::min(g.ly[j], k.d.ny));
         d.nstrip = strips_of(k.d);
         d.nseg = (k.d.ny + d.ly - 1) / d.ly;
-        d.ngroup = (d.nstrip + g.waves - 1) / g.waves;
         d.tile0 = tile0;
-        const int nt = d.ngroup * d.nseg;
+        const int nt = d.nstrip * d.nseg;
         tile_case.insert(tile_case.end(), nt, j);
         tile0 += nt;
         g.cur0[j] = k.cur;
@@ -694,7 +692,6 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
     a.parity = parity;
     a.record = record;
     a.depth = g.depth;
-    a.waves = g.waves;
     const CaseState& k0 = c->cases[g.members[0]];   // a context has one storage form
     HIPCHK(c, k0.elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, st)
                            : launch_step_lp(k0.elem, k0.sym, g.kind, a, g.ntiles, st));
@@ -703,7 +700,7 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
 
 // a tile shape and its measured time per step
 struct Cand {
-    int ly, depth, waves;
+    int ly, depth;
     float ms;
 };
 
@@ -723,8 +720,6 @@ bool needs_tuning(lbm_ctx* c, Group& g)
     if (g.tuned || !c->autotune) return false;
     if (std::getenv("ITOLBM_LY")) {
         if (const char* e = std::getenv("ITOLBM_DEPTH")) g.depth = std::max(0, std::min(3, std::atoi(e)));
-        if (const char* e = std::getenv("ITOLBM_WAVES"))
-            if (c->cases[g.members[0]].elem != 8) g.waves = std::max(1, std::min(4, std::atoi(e)));
         g.tuned = true;
         return false;
     }
@@ -769,8 +764,8 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
 // favoured the wrong shapes), and all active families run together, as they do in a step, so a
 // family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
 // fixed: the pipeline depth (FP32/FP16 kernels) at the current rows per segment, then the rows per
-// segment (1 .. 64) at that depth, then (FP32/FP16) 2 and 4 wavefronts per workgroup at the best of
-// those, each candidate over W untimed + T timed steps; the two fastest are timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
+// segment (1 .. 64) at that depth, each candidate over W untimed + T timed steps; the two fastest are
+// timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
 // members changed).  The steps used are subtracted from `left`; nothing is done unless `left`
 // covers the whole search.
 int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
@@ -780,7 +775,7 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
     auto depths_of = [&](const Group& g) { return c->cases[g.members[0]].elem == 8 ? 1 : 3; };
     long long budget = 0;
     for (const Group* g : act)
-        if (!g->tuned) budget += (long long)(2 * depths_of(*g) + 5) * (W + T) + 2 * (W + 2 * T);
+        if (!g->tuned) budget += (long long)(depths_of(*g) + 6) * (W + T) + 2 * (W + 2 * T);
     if (budget > left) return LBM_OK;
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
@@ -802,7 +797,6 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         if (g->tuned) continue;   // a family whose members did not change keeps its shape
         auto apply = [&](const Cand& cd) -> int {
             g->depth = cd.depth;
-            g->waves = cd.waves;
             g->ly.assign(g->members.size(), cd.ly);
             g->written = false;
             return group_sync(c, *g);
@@ -821,25 +815,16 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         std::vector<Cand> tried;
         const bool lp = depths_of(*g) > 1;
         for (int d = 1; d <= depths_of(*g); ++d) {
-            Cand cd{start, lp ? d : 0, 1, 0.f};
+            Cand cd{start, lp ? d : 0, 0.f};
             if (int r = time_of(cd, T)) return r;
             tried.push_back(cd);
         }
         const int best_depth = std::min_element(tried.begin(), tried.end(), by_time)->depth;
         for (int v : LYS) {
             if (v == start) continue;
-            Cand cd{v, best_depth, 1, 0.f};
+            Cand cd{v, best_depth, 0.f};
             if (int r = time_of(cd, T)) return r;
             tried.push_back(cd);
-        }
-        // wavefronts per workgroup (adjacent strips marching together) at the best shape so far
-        if (lp) {
-            const Cand best = *std::min_element(tried.begin(), tried.end(), by_time);
-            for (int w : {2, 4}) {
-                Cand cd{best.ly, best.depth, w, 0.f};
-                if (int r = time_of(cd, T)) return r;
-                tried.push_back(cd);
-            }
         }
         std::sort(tried.begin(), tried.end(), by_time);
         for (int j = 0; j < 2; ++j)
@@ -848,11 +833,9 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         if (int r = apply(tried[0])) return r;
         g->tuned = true;
         if (std::getenv("ITOLBM_TUNE_LOG"))
-            std::fprintf(stderr,
-                         "itolbm step tuner: family %d, %zu cases, element %d B: %d rows per segment, depth %d, "
-                         "%d wavefronts per workgroup, step %.1f us\n",
+            std::fprintf(stderr, "itolbm step tuner: family %d, %zu cases, element %d B: %d rows per segment, depth %d, step %.1f us\n",
                          g->kind, g->members.size(), c->cases[g->members[0]].elem, tried[0].ly, tried[0].depth,
-                         tried[0].waves, 1000.0 * tried[0].ms);
+                         1000.0 * tried[0].ms);
     }
     return LBM_OK;
 }

@@ -17,8 +17,7 @@ struct CaseDev {
     double* aux;         // [6][S]: mediator momentum gm xyz, force F xyz (diagnostic steps), or nullptr
     long long S;         // population stride (elements)
     int ly;              // rows per wavefront segment
-    int nstrip, nseg;    // strips of 64 columns x segments of ly rows
-    int ngroup;          // tiles per segment: groups of StepLaunch::waves strips (FP64: nstrip)
+    int nstrip, nseg;    // tiles of the case: strips of 64 - 2H columns x segments of ly rows
     int tile0;           // first tile of the case in the launch
 };
 
@@ -28,7 +27,6 @@ struct StepLaunch {
     int parity;              // which of pop[0] / pop[1] holds the input state
     int record;              // write aux for cases that have it
     int depth;               // FP32/FP16 kernels: rows in flight ahead (0 = default)
-    int waves;               // FP32/FP16 kernels: wavefronts (adjacent strips) per workgroup, 1..4
 };
 
 int step_halo(int op);

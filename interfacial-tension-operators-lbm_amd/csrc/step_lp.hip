@@ -237,7 +237,7 @@ struct HaloLaneLP {
 __device__ __forceinline__ HaloLaneLP halo_lane_lp(int nx, int col_first, int col_second)
 {
     HaloLaneLP h;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x;
     const int site = lane >> 5, j = lane & 31;
     h.on = j < Q;
     h.i = h.on ? j : 0;
@@ -501,12 +501,9 @@ __device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtxLP& t)
     t.out = d.pop[L.parity ^ 1];
     t.S = d.S;
     t.aux = L.record ? (gdouble*)d.aux : nullptr;
-    // a tile is a group of L.waves adjacent strips of one segment, one per wavefront; they march
-    // the same rows together, so the cache lines on their common borders are read while shared
     const int local = tile - d.tile0;
-    const int group = local % d.ngroup, seg = local / d.ngroup;
-    t.strip = group * L.waves + (int)(threadIdx.x >> 6);
-    if (t.strip >= d.nstrip) return false;
+    t.strip = local % d.nstrip;
+    const int seg = local / d.nstrip;
     t.y0 = seg * d.ly;
     t.y1 = min(t.y0 + d.ly, d.cc.ny);
     t.rev = (seg & 1) != 0;
@@ -634,7 +631,7 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
     const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
-    const Lane L = make_lane_aligned(nx, t.strip, (int)(threadIdx.x & 63));
+    const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
     Halo2LP<ST, SYM> h;
@@ -757,7 +754,7 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
 }
 
 template <typename ST, int KIND, int DEPTH, bool SYM>
-__global__ void __launch_bounds__(256) lp_f12(StepLaunch launch)
+__global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
 {
     static_assert(KIND == KIND_F1 || KIND == KIND_F2, "two-row stencils only");
     static_assert(DEPTH >= 1 && DEPTH <= 3, "pipeline depth 1..3");
@@ -828,7 +825,7 @@ __device__ __forceinline__ void opaque(unsigned& v) { asm volatile("" : "+v"(v))
 // register copies).  DEPTH trades registers (occupancy) for bytes in flight per wavefront; the
 // library's tuner picks it with the row-segment length.
 template <typename ST, int KIND, int DEPTH, bool SYM>
-__global__ void __launch_bounds__(256) lp_pipe(StepLaunch launch)
+__global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
 {
     static_assert(KIND == KIND_P || KIND == KIND_F3, "one-row stencils only");
     static_assert(DEPTH >= 1 && DEPTH <= 3, "pipeline depth 1..3");
@@ -839,7 +836,7 @@ __global__ void __launch_bounds__(256) lp_pipe(StepLaunch launch)
     const int nx = t.cc.nx, ny = t.cc.ny;
     const ConstF k = to_float(t.cc);
     const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
-    const Lane L = make_lane_aligned(nx, t.strip, (int)(threadIdx.x & 63));
+    const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
     HaloLP<ST, SYM> h;
@@ -908,7 +905,7 @@ __global__ void __launch_bounds__(256) lp_pipe(StepLaunch launch)
 template <typename ST, int KIND, bool SYM>
 void launch_pipe(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
 {
-    const dim3 block(64 * a.waves);
+    const dim3 block(64);
     if (depth <= 1) hipLaunchKernelGGL((lp_pipe<ST, KIND, 1, SYM>), grid, block, 0, stream, a);
     else if (depth == 2) hipLaunchKernelGGL((lp_pipe<ST, KIND, 2, SYM>), grid, block, 0, stream, a);
     else hipLaunchKernelGGL((lp_pipe<ST, KIND, 3, SYM>), grid, block, 0, stream, a);
@@ -917,7 +914,7 @@ void launch_pipe(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
 template <typename ST, int KIND, bool SYM>
 void launch_f12(int depth, const StepLaunch& a, dim3 grid, hipStream_t stream)
 {
-    const dim3 block(64 * a.waves);
+    const dim3 block(64);
     if (depth <= 1) hipLaunchKernelGGL((lp_f12<ST, KIND, 1, SYM>), grid, block, 0, stream, a);
     else if (depth == 2) hipLaunchKernelGGL((lp_f12<ST, KIND, 2, SYM>), grid, block, 0, stream, a);
     else hipLaunchKernelGGL((lp_f12<ST, KIND, 3, SYM>), grid, block, 0, stream, a);

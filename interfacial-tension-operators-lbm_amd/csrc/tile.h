@@ -45,10 +45,10 @@ __device__ __forceinline__ Lane make_lane(int nx, int strip, int H)
 // Aligned strip (one-row stencils): the 64 lanes own the 64 columns strip*64 .. strip*64+63, so
 // every wavefront-wide load of a column row and every store starts on a 64-element boundary.  The
 // columns just outside the strip (its halo) are handled by Halo1.
-__device__ __forceinline__ Lane make_lane_aligned(int nx, int strip, int lane)
+__device__ __forceinline__ Lane make_lane_aligned(int nx, int strip)
 {
     Lane L;
-    L.lane = lane;
+    L.lane = threadIdx.x;
     const int xr = strip * 64 + L.lane;
     L.col = wrapi(xr, nx);
     L.xm = (L.col == 0) ? nx - 1 : L.col - 1;
@@ -56,8 +56,6 @@ __device__ __forceinline__ Lane make_lane_aligned(int nx, int strip, int lane)
     L.interior = xr < nx;
     return L;
 }
-// one wavefront per workgroup
-__device__ __forceinline__ Lane make_lane_aligned(int nx, int strip) { return make_lane_aligned(nx, strip, threadIdx.x); }
 
 // Bit masks of the velocities with c_x = +1, c_x = -1, c_y = +1, c_y = -1 (D3Q19 ordering).
 constexpr unsigned CXP_MASK = (1u << 1) | (1u << 7) | (1u << 9) | (1u << 11) | (1u << 13);

@@ -1,14 +1,12 @@
-"""GPU: a step's result does not depend on the tile shape, the pipeline depth or the wavefronts per
-workgroup.
+"""GPU: a step's result does not depend on the tile shape or the pipeline depth.
 
 The autotuner picks, per kernel family, the rows per wavefront segment and (FP32 / FP16 storage) the
-number of rows whose loads are in flight and the number of adjacent strips a workgroup marches
-together, by timing real steps of the caller's; that is sound only if every shape computes every
-site the same way.  Contexts with fixed shapes (ITOLBM_LY rows per segment, ITOLBM_DEPTH depth,
-ITOLBM_WAVES wavefronts) are compared bit for bit with one whose shapes the tuner chose.  The cases
-cover the four kernel families, widths that are not a multiple of the 64-column strip (lanes outside
-the case, wavefronts past the last strip) and heights that are not a multiple of the segment (short
-last segments, look-ahead rows past a segment's end), and the aux fields of a record step.
+number of rows whose loads are in flight, by timing real steps of the caller's; that is sound only if
+every shape computes every site the same way.  Contexts with fixed shapes (ITOLBM_LY rows per segment, ITOLBM_DEPTH depth) are
+compared bit for bit with one whose shapes the tuner chose.  The cases cover the four kernel families,
+widths that are not a multiple of the 64-column strip (lanes outside the case) and heights that are
+not a multiple of the segment (short last segments, look-ahead rows past a segment's end), and the
+aux fields of a record step.
 """
 import os
 import sys
@@ -22,8 +20,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [(96, 37, "P3 0.5", 1.0, 0.225), (70, 45, "F1", 0.8, 0.1), (64, 29, "F2", 1.0, 0.1),
          (33, 64, "F3 1.569988", 0.6, 0.1)]
-# (rows per segment, pipeline depth, wavefronts per workgroup)
-SHAPES = [(1, 1, 1), (2, 2, 4), (3, 3, 1), (5, 2, 2), (7, 1, 4), (64, 3, 1), (13, 3, 2)]
+SHAPES = [(1, 1), (2, 2), (3, 3), (5, 2), (7, 1), (64, 3), (13, 3)]
 
 
 def say(*a):
@@ -32,14 +29,13 @@ def say(*a):
 
 
 def run(m, prec, shape, steps=4):
-    keys = ("ITOLBM_LY", "ITOLBM_DEPTH", "ITOLBM_WAVES")
-    saved = {k: os.environ.get(k) for k in keys}
+    saved = {k: os.environ.get(k) for k in ("ITOLBM_LY", "ITOLBM_DEPTH")}
     try:
-        for k in keys:
-            os.environ.pop(k, None)
         if shape:
-            for k, v in zip(keys, shape):
-                os.environ[k] = str(v)
+            os.environ["ITOLBM_LY"], os.environ["ITOLBM_DEPTH"] = str(shape[0]), str(shape[1])
+        else:
+            os.environ.pop("ITOLBM_LY", None)
+            os.environ.pop("ITOLBM_DEPTH", None)
         descs = [m.case_desc(nx, ny, op, tau, A) for nx, ny, op, tau, A in CASES]
         ctx = m.Context(descs, precision=prec)
         try:
@@ -68,7 +64,7 @@ def test_result_independent_of_tile_shape(prec):
     say(f"tile shapes: precision {prec}, tuned")
     ref = run(m, prec, None)
     for shape in SHAPES:
-        say(f"tile shapes: precision {prec}, rows per segment {shape[0]}, depth {shape[1]}, wavefronts {shape[2]}")
+        say(f"tile shapes: precision {prec}, rows per segment {shape[0]}, depth {shape[1]}")
         got = run(m, prec, shape)
         for j, (x, y) in enumerate(zip(ref, got)):
             assert np.array_equal(x, y), f"shape {shape}, array {j}"
@@ -80,7 +76,7 @@ def test_online_tuning_keeps_results(prec):
     family's candidate shapes timed in turn) ends in the same state as a fixed shape."""
     m = _pkg.load()
     say(f"online tuning: precision {prec}, 420 steps")
-    ref = run(m, prec, (8, 2, 1), steps=420)
+    ref = run(m, prec, (8, 2), steps=420)
     got = run(m, prec, None, steps=420)
     for j, (x, y) in enumerate(zip(ref, got)):
         assert np.array_equal(x, y), f"array {j}"
