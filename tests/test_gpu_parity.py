@@ -231,3 +231,23 @@ def test_init_band_matches_oracle(nx, ny, p, q, width):
     assert_same(gR, sim.fR, "red")
     assert_same(gB, sim.fB, "blue")
     ctx.close()
+
+
+@pytest.mark.parametrize("op,A", [("P3 0.5", 0.225), ("P1", 0.0), ("F1", 0.1), ("F2", 0.1), ("F3 1.569988", 0.1)])
+def test_step_bitwise_edge_branches(op, A):
+    """The reference's edge branches on the GPU, bitwise: a 3 x 3 block of sites whose populations are
+    scaled to a density of ~1e-33 is left uncollided (rho <= 1e-30, Operadores_tensao.cpp:347) while
+    its neighbours pull from it, and A = 0 switches the P-class perturbation off
+    (Operadores_tensao.cpp:120).  Exactly empty sites are not used: there the reference's colour field
+    is 0 / 0."""
+    say(f"[gpu] step parity edge branches 48x40 {op} A={A}")
+    m, sim, ctx = pair(48, 40, op, A, tau=1.0)
+    sim.init_drop(10.0, width=3.45)
+    fR = sim.fR.reshape(40, 48, 19)
+    fB = sim.fB.reshape(40, 48, 19)
+    for y, x in [(20, 12), (5, 30)]:   # one block on the interface, one in the bulk
+        fR[y - 1:y + 2, x - 1:x + 2, :] *= 1e-33
+        fB[y - 1:y + 2, x - 1:x + 2, :] *= 1e-33
+    ctx.upload(sim.fR, sim.fB)
+    check_steps(m, sim, ctx, [1, 2, 5])
+    ctx.close()
