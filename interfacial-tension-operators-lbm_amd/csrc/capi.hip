@@ -157,9 +157,9 @@ CaseConst make_const(const lbm_case_desc& d)
     return c;
 }
 
-// Elements added to each population array's length.  Array i starts at i * S elements; with S a
-// power of two every one of the 38 load and 38 store streams of a wavefront would map to the same
-// HBM channel, so S is padded (ITOLBM_PAD overrides).
+// Elements added to the stride S of the [k][S] site arrays (the aux fields and the FP64 [19][S]
+// views of transfers and diagnostics).  0 by default: the step kernels use row blocks, whose
+// streams the row pitch already spreads over HBM channels.  ITOLBM_PAD sets it.
 long long stride_pad()
 {
     if (const char* e = std::getenv("ITOLBM_PAD")) return std::atoll(e);
