@@ -376,13 +376,14 @@ static long long gcdll(long long a, long long b)
 
 /* Imm_INCL.cpp:309-334: rest equilibrium rho = 1, red fraction logistic in the signed distance to
  * the nearest interface.  Generalised from N x N to nx x ny with s = (p x + q y) mod M,
- * M = gcd(p nx, q ny) (= N for a square box with gcd(p, q) = 1).  width <= 0 gives a sharp band
- * (cR = 1 / 0 off the interface, 1/2 on it: the limit of the logistic). */
+ * M = gcd(p nx, q ny) / gcd(p, q), which is N for every normal on an N x N box (the reference's
+ * "% N", Imm_INCL.cpp:314-315, also for a non-coprime normal such as (2, 2)).  width <= 0 gives a
+ * sharp band (cR = 1 / 0 off the interface, 1/2 on it: the limit of the logistic). */
 void orc_init_band(const orc_case *cs, int p, int q, double width, double *fR, double *fB)
 {
     ensure_lattice();
     const int nx = cs->nx, ny = cs->ny;
-    const long long M = gcdll((long long)p * nx, (long long)q * ny);
+    const long long M = gcdll((long long)p * nx, (long long)q * ny) / gcdll(p, q);
     const double m = sqrt((double)(p * p + q * q));
     const double k = (width > 0) ? 2.0 * log(9.0) / width : -1.0;
     for (int y = 0; y < ny; y++)

@@ -47,6 +47,10 @@ SMALL.append(("incl 30 P3 0.5 (2,1)", "incl",
 SMALL.append(("incl 30 F1 (1,1)", "incl",
               dict(nx=30, ny=30, p=1, q=1, width=3.45, op="F1", tau=1.0, A=0.1, beta=0.8, steps=10,
                    diag=3, dump="0,9")))
+# a normal that is not coprime: the reference's band period is N (Imm_INCL.cpp:314-315, "% N")
+SMALL.append(("incl 30 P3 0.5 (2,2)", "incl",
+              dict(nx=30, ny=30, p=2, q=2, width=3.45, op="P3 0.5", tau=1.0, A=0.225, beta=0.8, steps=10,
+                   diag=3, dump="0,9")))
 
 
 def sha(path):

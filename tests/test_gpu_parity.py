@@ -88,6 +88,17 @@ def test_step_bitwise_band(op, A):
     ctx.close()
 
 
+def test_band_non_coprime_normal():
+    """Normal (2, 2) on a 30^2 box: the band repeats with period N as the reference's "% N"
+    (Imm_INCL.cpp:314-315); the library's own initial band, stepped, equals the oracle's bitwise."""
+    say("[gpu] band 30x30 (2,2) P3 0.5: library init + 10 steps")
+    m, sim, ctx = pair(30, 30, "P3 0.5", 0.225, tau=1.0)
+    sim.init_band(2, 2)
+    ctx.init_band(2, 2)
+    check_steps(m, sim, ctx, [1, 10])
+    ctx.close()
+
+
 TINY = [(5, 3, "F2", 0.1), (7, 1, "F1", 0.1), (6, 2, "F2", 0.1), (9, 2, "P3 0.5", 0.225), (4, 1, "F3 1.569988", 0.1),
         (3, 4, "F1", 0.1)]
 
@@ -221,7 +232,8 @@ def test_init_drop_matches_oracle(width, v0):
     ctx.close()
 
 
-@pytest.mark.parametrize("nx,ny,p,q,width", [(60, 60, 2, 1, 3.45), (320, 160, 2, 1, -1.0), (90, 45, 1, 1, 3.45)])
+@pytest.mark.parametrize("nx,ny,p,q,width", [(60, 60, 2, 1, 3.45), (320, 160, 2, 1, -1.0), (90, 45, 1, 1, 3.45),
+                                              (30, 30, 2, 2, 3.45)])
 def test_init_band_matches_oracle(nx, ny, p, q, width):
     say(f"[gpu] init band {nx}x{ny} ({p},{q}) width={width}")
     m, sim, ctx = pair(nx, ny, "P3 0.5", 0.225)

@@ -74,7 +74,7 @@ def _val(d, name):
     return float(getattr(d, DROP_MAP.get(name, name)))
 
 
-@pytest.mark.parametrize("idx", range(11))
+@pytest.mark.parametrize("idx", range(12))
 def test_bitwise_vs_reference_build(golden, idx):
     ent = golden["ref"][idx]
     hashes, diags = _run_small(ent)
