@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -159,11 +160,39 @@ CaseConst make_const(const lbm_case_desc& d)
 
 // Elements added to the stride S of the [k][S] site arrays (the aux fields and the FP64 [19][S]
 // views of transfers and diagnostics).  0 by default: the step kernels use row blocks, whose
-// streams the row pitch already spreads over HBM channels.  ITOLBM_PAD sets it.
+// streams the row pitch already spreads over HBM channels.  ITOLBM_PAD sets it: a whole number of
+// elements in [0, 2^20]; anything else (negative, not a number, trailing text) is refused, and
+// lbm_create returns LBM_E_ARG.  Returns -1 for a refused value.
+constexpr long long MAX_STRIDE_PAD = 1LL << 20;
 long long stride_pad()
 {
-    if (const char* e = std::getenv("ITOLBM_PAD")) return std::atoll(e);
-    return 0;
+    const char* e = std::getenv("ITOLBM_PAD");
+    if (!e || !*e) return 0;
+    char* end = nullptr;
+    errno = 0;
+    const long long v = std::strtoll(e, &end, 10);
+    if (errno != 0 || end == e || *end != '\0' || v < 0 || v > MAX_STRIDE_PAD) {
+        std::fprintf(stderr, "itolbm: ITOLBM_PAD=\"%s\" refused (a whole number of elements in [0, %lld])\n", e,
+                     MAX_STRIDE_PAD);
+        return -1;
+    }
+    return v;
+}
+
+// Whether the context's stream is capturing a HIP graph (a caller's capture).  Work that allocates,
+// frees, copies from host memory or synchronises must not be recorded into a graph: a captured
+// host-to-device copy keeps the address of a host buffer that is gone when the graph replays, and a
+// captured free leaves the graph holding memory that is no longer the context's.  An error from the
+// query counts as capturing.
+bool capturing(lbm_ctx* c)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &st) != hipSuccess) return true;
+    return st != hipStreamCaptureStatusNone;
+}
+int refuse_capture(lbm_ctx* c, const char* what)
+{
+    return fail(c, LBM_E_STATE, std::string(what) + " is not allowed while the context's stream is capturing a graph");
 }
 
 // Strips per row: aligned 64-column strips (halo columns come from separate loads).
@@ -206,6 +235,7 @@ int check_case(lbm_ctx* c, int id)
 int ensure_aux(lbm_ctx* c, CaseState& k)
 {
     if (k.aux) return LBM_OK;
+    if (capturing(c)) return refuse_capture(c, "allocating the aux fields (the first LBM_STEP_RECORD_AUX step)");
     int r = dev_alloc(c, (void**)&k.aux, sizeof(double) * 6 * (size_t)k.S);
     if (r) return r;
     HIPCHK(c, hipMemsetAsync(k.aux, 0, sizeof(double) * 6 * (size_t)k.S, c->stream));
@@ -395,6 +425,11 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         return LBM_E_HIP;
     }
     c->stream = c->own_stream;
+    const long long pad = stride_pad();
+    if (pad < 0) {
+        lbm_destroy(c);
+        return LBM_E_ARG;
+    }
     c->cases.resize(n_cases);
     for (int i = 0; i < n_cases; ++i) {
         const lbm_case_desc& d = cases[i];
@@ -408,14 +443,16 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.d = d;
         k.cc = make_const(d);
         k.n = (long long)d.nx * d.ny;
-        k.S = (k.n + 63) / 64 * 64 + stride_pad();
+        k.S = (k.n + 63) / 64 * 64 + pad;
         k.kind = step_kind(d.op);
         k.elem = elem;
         k.sym = sym;
         k.nslot = n_slots(sym);
+        // state buffers hold 2 x nslot values per site in row blocks (the step kernels index them by
+        // nx and ny only), so their size follows from n and not from the padded stride S
+        const size_t pop_bytes = (size_t)elem * 2 * k.nslot * (size_t)k.n;
         for (int b = 0; b < 2; ++b) {
-            if (dev_alloc(c, (void**)&k.pop[b], (size_t)elem * 2 * k.nslot * (size_t)k.S) ||
-                hipMemsetAsync(k.pop[b], 0, (size_t)elem * 2 * k.nslot * (size_t)k.S, c->stream) != hipSuccess) {
+            if (dev_alloc(c, (void**)&k.pop[b], pop_bytes) || hipMemsetAsync(k.pop[b], 0, pop_bytes, c->stream) != hipSuccess) {
                 lbm_destroy(c);
                 return LBM_E_NOMEM;
             }
@@ -439,6 +476,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
 int lbm_destroy(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
+    if (c->stream && capturing(c)) return refuse_capture(c, "lbm_destroy");
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (CaseState& k : c->cases) {
         for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.ordered, k.partial, k.hist, k.tab, k.prof, k.f64})
@@ -468,6 +506,7 @@ int lbm_destroy(lbm_ctx* c)
 int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_upload_state");
     if (!fR || !fB) return fail(c, LBM_E_ARG, "null population array");
     CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
@@ -524,6 +563,7 @@ void host_feq(double* feq, double vx, double vy, double vz, double rho)
 int lbm_init_drop(lbm_ctx* c, int id, double R, double width, int x0, int y0, double v0)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_init_drop");
     if (!std::isfinite(R) || !std::isfinite(v0) || !std::isfinite(width)) return fail(c, LBM_E_ARG, "non-finite argument");
     CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
@@ -563,6 +603,7 @@ int lbm_init_drop(lbm_ctx* c, int id, double R, double width, int x0, int y0, do
 int lbm_init_band(lbm_ctx* c, int id, int p, int q, double width)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_init_band");
     if (p == 0 && q == 0) return fail(c, LBM_E_ARG, "p = q = 0");
     const CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
@@ -593,6 +634,7 @@ int lbm_init_band(lbm_ctx* c, int id, int p, int q, double width)
 int lbm_download_state(lbm_ctx* c, int id, int which, double* fR, double* fB)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_download_state");
     if (!fR || !fB) return fail(c, LBM_E_ARG, "null population array");
     if (which != LBM_STATE_POST_STREAM && which != LBM_STATE_POST_COLLISION)
         return fail(c, LBM_E_ARG, "bad state selector");
@@ -628,6 +670,7 @@ int group_sync(lbm_ctx* c, Group& g)
         stale = (k.cur != (g.cur0[j] ^ (g.k & 1))) || (k.aux != g.aux0[j]);
     }
     if (!stale) return LBM_OK;
+    if (capturing(c)) return refuse_capture(c, "writing a family's case table (first step after a change of cases, shape or state)");
     const int nm = (int)g.members.size();
     if (g.ly.empty()) {
         const int ly = group_rows(c->cases, g.members);
@@ -908,10 +951,12 @@ int lbm_step_case(lbm_ctx* c, int id, int n_steps, int flags)
     if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
     Group& g = c->single[id];
     if (n_steps == 0 || g.members.empty()) return LBM_OK;
+    const bool cap = capturing(c);
+    if (cap && (n_steps & 1)) return fail(c, LBM_E_ARG, "a captured step call must advance an even number of steps");
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     if (int r = group_prepare(c, g, rec)) return r;
     int left = n_steps;
-    if (needs_tuning(c, g)) {   // as lbm_step: the tile shape is tuned on the call's own steps
+    if (!cap && needs_tuning(c, g)) {   // as lbm_step: the tile shape is tuned on the call's own steps
         int avail = left - (rec ? 1 : 0);
         const int before = avail;
         std::vector<Group*> one{&g};
@@ -930,11 +975,16 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
     for (Group& g : c->family)
         if (!g.members.empty()) act.push_back(&g);
     if (act.empty()) return LBM_OK;
+    // Under a caller's graph capture only kernel launches (and the fork / join of the families'
+    // streams) are queued: no tuning, no graph of the library's own, no table written.  An even step
+    // count returns every family to the buffers the host bookkeeping expects after each replay.
+    const bool cap = capturing(c);
+    if (cap && (n_steps & 1)) return fail(c, LBM_E_ARG, "a captured step call must advance an even number of steps");
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     bool untuned = false;
     for (Group* g : act) {
         if (int r = group_prepare(c, *g, rec)) return r;
-        if (needs_tuning(c, *g)) untuned = true;
+        if (!cap && needs_tuning(c, *g)) untuned = true;
     }
     if (act.size() > 1 && !c->fork) {
         HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
@@ -953,7 +1003,7 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
         if (int r = online_tune(c, act, avail)) return r;
         left -= before - avail;
     }
-    const int G = c->graph_steps;
+    const int G = cap ? 0 : c->graph_steps;
     if (G > 0 && left - (rec ? 1 : 0) >= G) {
         const int reps = (left - (rec ? 1 : 0)) / G;
         if (int r = ensure_graph(c, act, G)) return r;
@@ -977,6 +1027,7 @@ int lbm_set_graph(lbm_ctx* c, int steps_per_graph)
 int lbm_sync(lbm_ctx* c)
 {
     if (!c) return LBM_E_ARG;
+    if (capturing(c)) return refuse_capture(c, "lbm_sync");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LBM_OK;
 }
@@ -984,6 +1035,7 @@ int lbm_sync(lbm_ctx* c)
 int lbm_set_stream(lbm_ctx* c, void* s)
 {
     if (!c) return LBM_E_ARG;
+    if (capturing(c)) return refuse_capture(c, "lbm_set_stream");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->stream = s ? (hipStream_t)s : c->own_stream;
     return LBM_OK;
@@ -1008,6 +1060,7 @@ int lbm_set_active(lbm_ctx* c, int id, int active)
 int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_set_reference");
     if (!(std::isfinite(rho_red) && std::isfinite(rho_blue) && rho_red > 0.0 && rho_blue > 0.0))
         return fail(c, LBM_E_ARG, "reference densities must be finite and positive");
     CaseState& k = c->cases[id];
@@ -1025,6 +1078,7 @@ int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
 int lbm_download_aux(lbm_ctx* c, int id, double* gm, double* F)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_download_aux");
     CaseState& k = c->cases[id];
     if (!k.aux_valid) return fail(c, LBM_E_STATE, "no recorded step (use LBM_STEP_RECORD_AUX)");
     std::vector<double> h(6 * (size_t)k.S);
@@ -1041,6 +1095,7 @@ int lbm_download_aux(lbm_ctx* c, int id, double* gm, double* F)
 int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double y_ini, lbm_drop_diag* o)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_diag_drop");
     if (!o) return fail(c, LBM_E_ARG, "null output");
     CaseState& k = c->cases[id];
     const bool forced = k.d.op >= OP_F1;
@@ -1250,6 +1305,7 @@ int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double 
 int lbm_diag_incl(lbm_ctx* c, int id, int p, int q, lbm_incl_diag* o, double* profile)
 {
     if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_diag_incl");
     if (!o) return fail(c, LBM_E_ARG, "null output");
     if (p == 0 && q == 0) return fail(c, LBM_E_ARG, "p = q = 0");
     CaseState& k = c->cases[id];

@@ -669,16 +669,6 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
 
 int step_halo(int op) { return (op == OP_F1 || op == OP_F2) ? 2 : 1; }
 
-int step_kind(int op)
-{
-    switch (op) {
-    case OP_P1:
-    case OP_P3: return KIND_P;
-    case OP_F3: return KIND_F3;
-    case OP_F1: return KIND_F1;
-    default: return KIND_F2;
-    }
-}
 
 namespace {
 // row blocks [y][colour][19][nx] -> [colour][19][S] (the diagnostics' and transfers' view)

@@ -30,7 +30,17 @@ struct StepLaunch {
 };
 
 int step_halo(int op);
-int step_kind(int op);
+// kernel family of an operator (OpId)
+inline int step_kind(int op)
+{
+    switch (op) {
+    case OP_P1:
+    case OP_P3: return KIND_P;
+    case OP_F3: return KIND_F3;
+    case OP_F1: return KIND_F1;
+    default: return KIND_F2;
+    }
+}
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
 // FP64 state in row blocks [y][colour][19][nx] -> [colour][19][S]
