@@ -144,6 +144,18 @@ int lbm_step(lbm_ctx* ctx, int n_steps, int flags);
 /* Advances one case only. */
 int lbm_step_case(lbm_ctx* ctx, int case_id, int n_steps, int flags);
 
+/* As lbm_step_case, allocating nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  The case's
+ * launch table and tile map go into a static device array of 4 MiB + 256 B that each GPU the library
+ * runs on holds until the process ends, instead of memory lbm_step_case allocates; the kernels, the
+ * tile shape (the one lbm_step_case tuned for the case, else the default) and the results are the
+ * same.  No tuning, no allocation, no copy from host memory, no synchronisation.  Limits: a case of
+ * more than 2^20 tiles (64-column strips x row segments) is refused with LBM_E_ARG;
+ * LBM_STEP_RECORD_AUX needs the aux fields that a first recording lbm_step / lbm_step_case allocates
+ * (LBM_E_STATE otherwise).  Every call on one GPU shares that GPU's array, so calls that overlap on
+ * one GPU (other streams, other host threads) race; calls queued on one stream do not. */
+int lbm_step_case_noalloc(lbm_ctx* ctx, int case_id, int n_steps, int flags);
+
 /* Waits for all queued work. */
 int lbm_sync(lbm_ctx* ctx);
 

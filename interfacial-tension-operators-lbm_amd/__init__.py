@@ -27,7 +27,8 @@ STEP_RECORD_AUX = 1
 
 # every symbol include/itolbm.h declares
 SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "lbm_upload_state",
-           "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
+           "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case",
+           "lbm_step_case_noalloc", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
            "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl"]
 
 
@@ -87,6 +88,7 @@ def lib():
         L.lbm_init_band.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_double]
         L.lbm_step.argtypes = [P, C.c_int, C.c_int]
         L.lbm_step_case.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        L.lbm_step_case_noalloc.argtypes = [P, C.c_int, C.c_int, C.c_int]
         L.lbm_sync.argtypes = [P]
         L.lbm_set_stream.argtypes = [P, C.c_void_p]
         L.lbm_set_reference.argtypes = [P, C.c_int, C.c_double, C.c_double]
@@ -189,6 +191,10 @@ class Context:
             self._chk(lib().lbm_step(self.h, n, flags))
         else:
             self._chk(lib().lbm_step_case(self.h, case, n, flags))
+
+    def step_noalloc(self, n=1, record=False, case=0):
+        """lbm_step_case_noalloc: one case, nothing allocated (static scratch on the device)."""
+        self._chk(lib().lbm_step_case_noalloc(self.h, case, n, STEP_RECORD_AUX if record else 0))
 
     def set_stream(self, hip_stream):
         """Queues later work on a caller's hipStream_t (an int handle, e.g. torch's cuda_stream); None = own."""

@@ -98,6 +98,8 @@ struct lbm_ctx {
     int graph_steps = 32;              // steps per captured graph in lbm_step (0: no graphs)
     StepGraph graph;
     long long bytes = 0;
+    int device = 0;                    // the device current at lbm_create
+    void* scratch = nullptr;           // that device's static scratch array (lbm_step_case_noalloc)
     Group family[N_KINDS];             // all cases of each kernel family (lbm_step)
     std::vector<Group> single;         // one group per case (lbm_step_case)
 };
@@ -420,11 +422,16 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         delete c;
         return LBM_E_HIP;
     }
-    if (hipStreamCreate(&c->own_stream) != hipSuccess) {
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->own_stream) != hipSuccess) {
         delete c;
         return LBM_E_HIP;
     }
     c->stream = c->own_stream;
+    // the allocation-free step's scratch array on this device (its address, looked up once per device)
+    if (scratch_setup(c->device, &c->scratch) != hipSuccess) {
+        lbm_destroy(c);
+        return LBM_E_HIP;
+    }
     const long long pad = stride_pad();
     if (pad < 0) {
         lbm_destroy(c);
@@ -964,6 +971,56 @@ int lbm_step_case(lbm_ctx* c, int id, int n_steps, int flags)
         left -= before - avail;
     }
     return group_step(c, g, left, flags, c->stream);
+}
+
+// lbm_step_case without allocation: the case's table entry and tile map go into the device's static
+// scratch array (static_step.hip), written by a kernel whose arguments carry the entry, instead of the
+// table lbm_step_case allocates.  Same kernels, same tile shape (the one lbm_step_case tuned for the
+// case, else the default), same results.  Nothing is allocated, freed, copied from host memory or
+// synchronised; no tuning.  Calls that overlap on one device share the scratch array and race.
+int lbm_step_case_noalloc(lbm_ctx* c, int id, int n_steps, int flags)
+{
+    if (int r = check_case(c, id)) return r;
+    if (n_steps < 0) return fail(c, LBM_E_ARG, "negative step count");
+    if (n_steps == 0) return LBM_OK;
+    if ((n_steps & 1) && capturing(c))
+        return fail(c, LBM_E_ARG, "a captured step call must advance an even number of steps");
+    CaseState& k = c->cases[id];
+    const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    if (rec && !k.aux)
+        return fail(c, LBM_E_STATE, "LBM_STEP_RECORD_AUX needs the aux fields, which a first lbm_step or lbm_step_case "
+                                    "with LBM_STEP_RECORD_AUX allocates");
+    if (!c->scratch) return fail(c, LBM_E_STATE, "no static scratch on this device");
+    const Group& g = c->single[id];
+    const int ly = g.ly.empty() ? group_rows(c->cases, g.members) : g.ly[0];
+    CaseDev d{};
+    d.cc = k.cc;
+    d.pop[0] = k.pop[k.cur];
+    d.pop[1] = k.pop[k.cur ^ 1];
+    d.aux = k.aux;
+    d.S = k.S;
+    d.ly = std::max(1, std::min(ly, k.d.ny));
+    d.nstrip = strips_of(k.d);
+    d.nseg = (k.d.ny + d.ly - 1) / d.ly;
+    d.tile0 = 0;
+    const long long ntiles = (long long)d.nstrip * d.nseg;
+    if (ntiles > SCRATCH_MAX_TILES)
+        return fail(c, LBM_E_ARG, "case needs " + std::to_string(ntiles) + " tiles; the static scratch holds " +
+                                      std::to_string(SCRATCH_MAX_TILES) + " (use lbm_step_case)");
+    HIPCHK(c, launch_fill_static_table(c->scratch, d, (int)ntiles, c->stream));
+    StepLaunch a{};
+    a.tab = scratch_table(c->scratch);
+    a.tile_case = scratch_tiles(c->scratch);
+    a.depth = g.depth;
+    for (int s = 0; s < n_steps; ++s) {
+        a.parity = s & 1;
+        a.record = (rec && s == n_steps - 1) ? 1 : 0;
+        HIPCHK(c, k.elem == 8 ? launch_step_f64(k.kind, a, (int)ntiles, c->stream)
+                              : launch_step_lp(k.elem, k.sym, k.kind, a, (int)ntiles, c->stream));
+        k.cur ^= 1;
+    }
+    k.aux_valid = rec;
+    return LBM_OK;
 }
 
 int lbm_step(lbm_ctx* c, int n_steps, int flags)

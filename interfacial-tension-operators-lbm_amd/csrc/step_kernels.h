@@ -42,6 +42,19 @@ inline int step_kind(int op)
     }
 }
 
+// Static scratch of the allocation-free step (static_step.hip): a table entry slot, then the
+// tile -> case map of at most SCRATCH_MAX_TILES tiles.
+constexpr int SCRATCH_MAX_TILES = 1 << 20;
+constexpr size_t SCRATCH_TABLE_BYTES = 256;
+constexpr size_t SCRATCH_BYTES = SCRATCH_TABLE_BYTES + sizeof(int) * (size_t)SCRATCH_MAX_TILES;
+inline CaseDev* scratch_table(void* base) { return (CaseDev*)base; }
+inline int* scratch_tiles(void* base) { return (int*)((char*)base + SCRATCH_TABLE_BYTES); }
+// device address of the scratch array on `device`, which must be current: looked up once per
+// device with hipGetSymbolAddress, then kept
+hipError_t scratch_setup(int device, void** out);
+// writes the table entry d and a tile map of ntiles tiles of case 0 into the scratch at `base`
+hipError_t launch_fill_static_table(void* base, const CaseDev& d, int ntiles, hipStream_t stream);
+
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
 // FP64 state in row blocks [y][colour][19][nx] -> [colour][19][S]
 hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream);

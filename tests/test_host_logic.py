@@ -153,3 +153,30 @@ def test_first_recording_step_under_capture_refused(mock):
     assert counts(L)["malloc"] == 0
     L.mock_set_capturing(0)
     assert L.lbm_destroy(h) == LBM_OK
+
+
+def test_noalloc_step_touches_no_memory_api(mock):
+    """lbm_step_case_noalloc: one table-fill launch plus one launch per step; no allocation, free,
+    copy, memset or synchronisation, also under a caller's capture (even step counts there)."""
+    L, m = mock
+    L.lbm_step_case_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    rc, h = create(L, m, [m.case_desc(100, 80, "F1", 1.0, 0.1), m.case_desc(40, 30, "P3 0.5")], prec=3)
+    assert rc == LBM_OK
+    L.mock_reset_counts()
+    assert L.lbm_step_case_noalloc(h, 0, 5, 0) == LBM_OK
+    assert L.lbm_step_case_noalloc(h, 1, 3, 1) == LBM_E_STATE      # no aux fields yet
+    c = counts(L)
+    assert c["launches"] == 1 + 5, c
+    for k in ("malloc", "free", "memcpy", "memset", "sync", "captures", "graph_launches"):
+        assert c[k] == 0, (k, c)
+    assert L.lbm_step_case(h, 1, 1, 1) == LBM_OK                     # allocates the aux fields
+    L.mock_set_capturing(1)
+    L.mock_reset_counts()
+    assert L.lbm_step_case_noalloc(h, 1, 4, 1) == LBM_OK
+    assert L.lbm_step_case_noalloc(h, 0, 3, 0) == LBM_E_ARG          # odd count under capture
+    c = counts(L)
+    assert c["launches"] == 1 + 4, c
+    for k in ("malloc", "free", "memcpy", "memset", "sync", "captures", "graph_launches"):
+        assert c[k] == 0, (k, c)
+    L.mock_set_capturing(0)
+    assert L.lbm_destroy(h) == LBM_OK
