@@ -8,8 +8,11 @@ F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (one laun
 family per step).  A bench step advances every case by one time step.  Inputs are synthetic
 (generated by the library's initial condition), resident in HBM before timing.
 
-The headline form (--prec, default f32z) is timed over --steps steps after --warmup; the other
-storage forms are timed the same way and reported beside it:
+Every form first runs --tune-steps untimed steps (default 400, whatever --warmup is): the library
+tunes its tile shapes on the first ~300-360 of them, and the shapes it chose are reported.  Then
+--warmup untimed steps, then --steps timed steps.  The headline (--prec, default f64) is the FP64
+drop-in form, whose arithmetic is the reference's; the other storage forms are timed the same way
+and reported under "forms", each with its delta_p difference from the FP64 form after the same steps:
 
   f64    FP64 compute and storage (the drop-in form, bit-equal to the reference's arithmetic)
   f32    FP32 compute, FP32 storage of all 19 populations per site
@@ -22,25 +25,25 @@ Also reported:
 
   roofline       the dominant kernel (the P1/P3 family launch, 8 of the 14 cases) timed alone with HIP
                  events on its stream: algorithmic bytes per launch (the form's stored bytes) / average
-                 launch duration; frac_at_304B = the kernel's lattice updates per second x 304 B (the
-                 full FP32 state's bytes per update) / peak, the MLUPS-based roofline fraction;
-                 traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-                 (profiles/traffic_<form>.json), when present; copy_GBs = a plain 1 GiB device copy
-                 (torch's copy kernel) timed the same way, the bandwidth a streaming kernel reaches
-                 on this GPU, and frac_of_copy = achieved / copy_GBs
-  large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in the headline form
+                 launch duration; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+                 runs (profiles/traffic_<form>.json), when present; stream_GBs = the library's own
+                 streaming kernel (read + write of 1 GiB, 16-byte loads, nontemporal stores) timed the
+                 same way, the bandwidth a plain streaming kernel reaches on this GPU
+  large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in FP32 (all 19
+                 populations, 304 B per update), FP32 z-symmetric and FP64 storage
   sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
                  numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
-                 batched reduced-precision form vs the plain FP64
-                 one-launch-per-case path, time per step and delta_p agreement after --sweep-steps
-  cpu_baseline   the reference program (built from the reference's sources by oracle/Makefile) on
-                 this host's cores, over a bounded sample of the same workload
+                 batched reduced-precision form vs the plain FP64 one-launch-per-case path over
+                 --sweep-steps steps in all (the first 400 untimed), delta_p agreement at the end
+  cpu_baseline   the reference program Imm_OPS (built from the reference's sources by oracle/Makefile)
+                 on this host's cores, over a bounded sample of the same workload
 
 MLUPS = sum over cases of sites x steps / wall / 1e6 (the reference's definition, Imm_OPS.cpp:377).
 """
 import argparse
 import json
 import os
+import platform
 import subprocess
 import sys
 import tempfile
@@ -51,23 +54,27 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 OPS = [("P1", "P"), ("P3 -0.4", "P"), ("P3 0.5", "P"), ("P3 2", "P"), ("F1", "F1"), ("F2", "F2"),
        ("F3 1.569988", "F3")]
+FAMILY_NAMES = ("P1/P3", "F3", "F1", "F2")
 TAUS = (0.6, 1.0)
 SIGMA = 0.1
 NX = NY = 512
 R = 100.0
 PEAK_HBM_GBS = 8000.0
 SWEEP_WARMUP = 400   # untimed sweep steps (the batched path tunes its tile shapes on them)
+LARGE_TUNE = 120     # untimed steps of the 8192^2 case (tuning)
 B_LU = {"f64": 608, "f32": 304, "f16": 152, "f32z": 224, "f16z": 112}
 PREC_ID = {"f64": 0, "f32": 1, "f16": 2, "f32z": 3, "f16z": 4}
 KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth,false> (P1/P3 family)",
           "f16": "lp_pipe<_Float16,P,depth,false> (P1/P3 family)",
           "f32z": "lp_pipe<float,P,depth,true> (P1/P3 family)",
           "f16z": "lp_pipe<_Float16,P,depth,true> (P1/P3 family)"}
-FORM_TEXT = {"f64": "f64 storage", "f32": "f32 storage of all 19 populations, fp32 compute",
+FORM_TEXT = {"f64": "f64 compute and storage (the drop-in form, the reference's arithmetic)",
+             "f32": "f32 storage of all 19 populations, fp32 compute",
              "f16": "f16 storage of all 19 populations, fp32 compute",
              "f32z": "f32 storage of the 14 distinct populations of the z-mirror-symmetric slab states "
                      "(bitwise equal to full f32 storage), fp32 compute",
              "f16z": "f16 storage of the 14 distinct populations of the z-mirror-symmetric slab states, fp32 compute"}
+DTYPE = {"f64": "f64", "f32": "f32", "f32z": "f32", "f16": "f32 compute, f16 storage", "f16z": "f32 compute, f16 storage"}
 
 
 def amplitude(kind, tau, sigma=SIGMA):
@@ -97,7 +104,18 @@ class Timer:
         return e0.elapsed_time(e1) * 1e-3, time.perf_counter() - w0
 
 
-def bench_form(m, timer, stream, form, steps, warmup):
+def shapes_of(ctx, kinds):
+    out = {}
+    for fam, name in enumerate(FAMILY_NAMES):
+        for i, k in enumerate(kinds):
+            if ("P1/P3" if k == "P" else k) == name:
+                rows, depth, tuned = ctx.tile_shape(case=i)
+                out[name] = {"rows_per_segment": rows, "depth": depth, "tuned": tuned}
+                break
+    return out
+
+
+def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     cases, kinds = [], []
     for tau in TAUS:
         for op, kind in OPS:
@@ -107,26 +125,44 @@ def bench_form(m, timer, stream, form, steps, warmup):
     ctx.set_stream(stream.cuda_stream)
     for i in range(len(cases)):
         ctx.init_drop(R, width=-1.0, case=i)
-    ctx.step(warmup)
+    ctx.step(tune_steps)   # untimed: tile shapes are tuned on these steps
+    ctx.sync()
+    shapes = shapes_of(ctx, kinds)
+    say(f"bench:   {form} tile shapes " + ", ".join(f"{k}: {v['rows_per_segment']} rows, depth {v['depth']}"
+                                                    + ("" if v["tuned"] else " (default)") for k, v in shapes.items()))
+    if warmup > 0:
+        ctx.step(warmup)
     gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
+    ctx.step(1, record=True)
+    total = tune_steps + warmup + steps + 1
+    dp = [ctx.diag_drop(0.0, total - 1, case=i).delta_p for i in range(len(cases))]
     n_sites = NX * NY * len(cases)
     # the dominant kernel alone: the P1/P3 family launch (8 cases), same launch shape
     for i, k in enumerate(kinds):
         if k != "P":
             ctx.set_active(False, case=i)
     nP = sum(1 for k in kinds if k == "P")
-    ctx.step(5)
-    kern_s, _ = timer.run(lambda: ctx.step(steps))
+    ksteps = max(steps, 200)
+    ctx.step(10)
+    kern_s, _ = timer.run(lambda: ctx.step(ksteps))
     ctx.close()
-    avg = kern_s / steps
+    avg = kern_s / ksteps
     bytes_launch = B_LU[form] * NX * NY * nP
     return {"MLUPS": n_sites * steps / wall_s / 1e6, "ms_per_step": wall_s / steps * 1e3,
-            "gpu_ms_per_step": gpu_s / steps * 1e3,
+            "gpu_ms_per_step": gpu_s / steps * 1e3, "tile_shapes": shapes,
             "dominant_kernel": KERNEL[form], "kernel_cases_per_launch": nP,
-            "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
+            "kernel_launches_timed": ksteps, "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
             "achieved_GBs": bytes_launch / avg / 1e9, "frac": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
-            "frac_at_304B": 304 * NX * NY * nP / avg / 1e9 / PEAK_HBM_GBS,
-            "bytes_per_lattice_update": B_LU[form]}
+            "bytes_per_lattice_update": B_LU[form], "steps_before_delta_p": total, "_dp": dp}
+
+
+def dp_agreement(dp, ref):
+    err = [abs(a - b) / abs(b) for a, b in zip(dp, ref) if b != 0 and a == a and b == b]
+    err.sort()
+    if not err:
+        return None
+    return {"cases": len(err), "median_rel": err[len(err) // 2], "max_rel": err[-1],
+            "within_1e-4": sum(1 for e in err if e <= 1e-4)}
 
 
 def bench_large(m, timer, stream, form, steps):
@@ -134,32 +170,37 @@ def bench_large(m, timer, stream, form, steps):
     ctx = m.Context(m.case_desc(n, n, "P3 0.5", 1.0, amplitude("P", 1.0), 0.8), precision=PREC_ID[form])
     ctx.set_stream(stream.cuda_stream)
     ctx.init_drop(1000.0, width=-1.0, v0=0.1)
-    ctx.step(100)   # untimed: the library tunes the tile shape on these steps
+    ctx.step(LARGE_TUNE)   # untimed: the library tunes the tile shape on these steps
+    ctx.sync()
+    rows, depth, tuned = ctx.tile_shape()
     gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
     ctx.close()
     avg = gpu_s / steps
     return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
-            "steps": steps, "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
-            "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS,
-            "frac_at_304B": 304 * n * n / avg / 1e9 / PEAK_HBM_GBS}
+            "steps": steps, "tile_shape": {"rows_per_segment": rows, "depth": depth, "tuned": tuned},
+            "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
+            "bytes_per_lattice_update": B_LU[form],
+            "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS}
 
 
-def copy_ceiling(torch, timer, stream, reps=10):
-    """Plain device copy of 1 GiB (torch's copy kernel) on the same stream: read + write bandwidth a
-    simple streaming kernel reaches on this GPU, beside the nominal peak."""
-    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+def stream_ceiling(m, timer, stream, reps=10):
+    """The library's own streaming kernel (lbm_stream_copy: 16-byte loads, nontemporal 16-byte stores)
+    copying 1 GiB on the same stream: read + write bandwidth a plain streaming kernel reaches here."""
+    import torch
+    n = 1 << 30
+    x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
-    with torch.cuda.stream(stream):
-        for _ in range(3):
-            y.copy_(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        m.stream_copy(y.data_ptr(), x.data_ptr(), n, stream.cuda_stream)
 
-        def run():
-            for _ in range(reps):
-                y.copy_(x)
-        gpu_s, _ = timer.run(run)
-    nbytes = 2 * x.numel() * x.element_size()
+    def run():
+        for _ in range(reps):
+            m.stream_copy(y.data_ptr(), x.data_ptr(), n, stream.cuda_stream)
+    gpu_s, _ = timer.run(run)
+    ok = bool(torch.equal(x, y))
     del x, y
-    return {"bytes_per_copy": nbytes, "GBs": nbytes * reps / gpu_s / 1e9}
+    return {"bytes_per_copy": 2 * n, "GBs": 2 * n * reps / gpu_s / 1e9, "copy_correct": ok}
 
 
 def sweep_cases(m):
@@ -178,18 +219,20 @@ def sweep_cases(m):
 
 
 def bench_sweep(m, timer, stream, form, steps):
-    """168 cases: plain FP64 one launch per case and step vs the batched form, `steps` steps each."""
+    """168 cases: plain FP64 one launch per case and step vs the batched form, `steps` steps each in
+    all; the first SWEEP_WARMUP (tile tuning, first launches) are not timed."""
     import numpy as np
     cases = sweep_cases(m)
     descs = [c for c, _, _ in cases]
     sites = sum(d.nx * d.ny for d in descs)
+    timed = steps - SWEEP_WARMUP
     res = {}
     for label, prec in (("f64_per_case", 0), (f"{form}_batched", PREC_ID[form])):
+        say(f"bench:   sweep {label}")
         ctx = m.Context(descs, precision=prec)
         ctx.set_stream(stream.cuda_stream)
         for i, (d, r, _) in enumerate(cases):
             ctx.init_drop(r, width=3.45, case=i)
-        # untimed warm-up (first launches, tile-shape tuning), the same steps on both paths
         if prec == 0:
             for i in range(len(descs)):
                 ctx.step(SWEEP_WARMUP, case=i)
@@ -200,53 +243,79 @@ def bench_sweep(m, timer, stream, form, steps):
             # one launch per case and step (cases are independent, so case by case is the same work)
             def run():
                 for i in range(len(descs)):
-                    ctx.step(steps - 1, case=i)
+                    ctx.step(timed - 1, case=i)
                     ctx.step(1, record=True, case=i)
         else:
             def run():
-                ctx.step(steps - 1)
+                ctx.step(timed - 1)
                 ctx.step(1, record=True)
         gpu_s, wall_s = timer.run(run)
         diags = [ctx.diag_drop(0.0, steps - 1, case=i) for i in range(len(descs))]
         dp = np.array([g.delta_p for g in diags])
         ok = np.array([g.ok for g in diags])
         ctx.close()
-        res[label] = {"wall_s": wall_s, "ms_per_step": wall_s / steps * 1e3, "dp": dp, "ok": ok}
+        res[label] = {"wall_s": wall_s, "ms_per_step": wall_s / timed * 1e3, "dp": dp, "ok": ok}
     a, b = res["f64_per_case"], res[f"{form}_batched"]
     good = (a["ok"] == 1) & (b["ok"] == 1) & np.isfinite(a["dp"]) & np.isfinite(b["dp"]) & (np.abs(a["dp"]) > 1e-9)
     err = np.abs(b["dp"][good] - a["dp"][good]) / np.abs(a["dp"][good])
     labels = [lab for (_, _, lab), g in zip(cases, good) if g]
-    per_case_20k = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
-    return {"cases": len(descs), "sites": sites, "steps": steps, "warmup": SWEEP_WARMUP, "form": form,
+    per_hour = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
+    over = [[labels[j], float(err[j])] for j in np.argsort(err)[::-1] if err[j] > 1e-4]
+    return {"cases": len(descs), "sites": sites, "steps": steps, "untimed_steps": SWEEP_WARMUP, "form": form,
             "f64_per_case_ms_per_step": a["ms_per_step"], "batched_ms_per_step": b["ms_per_step"],
             "speedup": a["ms_per_step"] / b["ms_per_step"],
-            "cases_per_hour_20k_steps": {"f64_per_case": per_case_20k(a["ms_per_step"]),
-                                         f"{form}_batched": per_case_20k(b["ms_per_step"])},
+            "cases_per_hour_20k_steps": {"f64_per_case": per_hour(a["ms_per_step"]),
+                                         f"{form}_batched": per_hour(b["ms_per_step"])},
+            "unstable_f64": int((a["ok"] != 1).sum()), "unstable_batched": int((b["ok"] != 1).sum()),
             "dp_compared_cases": int(good.sum()),
             "dp_rel_err_median": float(np.median(err)) if err.size else None,
             "dp_rel_err_max": float(err.max()) if err.size else None,
             "dp_within_1e-4": int((err <= 1e-4).sum()),
-            "dp_worst_cases": [[labels[j], float(err[j])] for j in np.argsort(err)[::-1][:3]] if err.size else []}
+            "dp_beyond_1e-4": over}
+
+
+def host_cpu():
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
 
 
 def cpu_baseline(max_threads):
-    """The reference program (oracle/_ref/harness_drop_fma, compiled from the reference sources with its
-    own optimisation flags) on this host's cores: 512^2 P3 chi = 0.5 drop, bounded number of steps."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "harness_drop_fma")
-    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    """The reference program itself (oracle/_ref/Imm_OPS, compiled from the reference's sources by
+    oracle/Makefile with -O3 -march=x86-64-v3 -fopenmp) on this host's cores: a 512^2 P3 chi = 0.5
+    case directory, bounded number of steps; MLUPS as the program reports it in resumo.dat."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    model, nproc = host_cpu()
+    exe = os.path.join(ROOT, "oracle", "_ref", "Imm_OPS")
+    threads = max(1, min(max_threads, nproc))
     if os.path.exists(exe):
-        steps = 200
+        import _cases as K
+        steps = 1001
         with tempfile.TemporaryDirectory() as d:
+            K.make_case(d, NX, "P3 0.5", 1.0, amplitude("P", 1.0), steps)
+            txt = open(os.path.join(d, "data_in.txt")).read().replace("n_threads:  1", f"n_threads:  {threads}")
+            open(os.path.join(d, "data_in.txt"), "w").write(txt)
             t0 = time.perf_counter()
-            r = subprocess.run([exe, "mode=drop", f"nx={NX}", f"ny={NY}", f"R={R}", "width=-1", "op=P3 0.5",
-                                "tau=1.0", f"A={amplitude('P', 1.0)}", "beta=0.8", f"steps={steps}",
-                                f"diag={10 * steps}", f"threads={threads}", f"out={d}"],
-                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            r = subprocess.run([exe], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
             dt = time.perf_counter() - t0
-        if r.returncode == 0:
-            return {"value": NX * NY * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "reference",
-                    "sample": f"reference program (OpenMP, {threads} threads): 512x512 sharp drop R=100, P3 chi=0.5, "
-                              f"tau=1, {steps} steps incl. 2 diagnostics, wall {dt:.1f} s"}
+            mlups = None
+            if r.returncode == 0 and os.path.exists(os.path.join(d, "resumo.dat")):
+                mlups = float(open(os.path.join(d, "resumo.dat")).read().splitlines()[1].split()[-1])
+        if mlups:
+            return {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": "reference",
+                    "cpu_model": model, "nproc": nproc,
+                    "sample": f"the reference program Imm_OPS (OpenMP, {threads} threads, built from its sources with "
+                              f"-O3 -march=x86-64-v3) on a 512x512 P3 chi=0.5 tau=1 case directory (its own drop: "
+                              f"R = nx/4, logistic profile; per-step work as the bench drop), {steps} steps with its "
+                              f"diagnostics every 500; MLUPS as the program writes it to resumo.dat "
+                              f"(process wall {dt:.1f} s)"}
     # the reference build is absent: the oracle (scalar C restatement), one core
     import _oracle as O
     cs = O.make_case(NX, NY, "P3 0.5", 1.0, amplitude("P", 1.0))
@@ -258,6 +327,7 @@ def cpu_baseline(max_threads):
         s.step()
     dt = time.perf_counter() - t0
     return {"value": NX * NY * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "cpu_model": model, "nproc": nproc,
             "sample": f"oracle C restatement, 1 thread: 512x512 P3 chi=0.5, {steps} steps, wall {dt:.1f} s"}
 
 
@@ -265,21 +335,25 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=450,
-                    help="untimed steps first (the library tunes its tile shapes on the first ~360 of them)")
-    ap.add_argument("--prec", default="f32z", choices=list(B_LU), help="headline storage form")
-    ap.add_argument("--forms", default="f64,f32,f16,f16z", help="forms measured beside the headline")
-    ap.add_argument("--sweep-steps", type=int, default=1000, help="0 skips the 168-case sweep")
-    ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 case")
+    ap.add_argument("--warmup", type=int, default=50, help="untimed steps after the tuning steps")
+    ap.add_argument("--tune-steps", type=int, default=400,
+                    help="untimed steps first, on which the library tunes its tile shapes (needs ~360)")
+    ap.add_argument("--prec", default="f64", choices=list(B_LU), help="headline storage form")
+    ap.add_argument("--forms", default="f32,f32z,f16,f16z", help="forms measured beside the headline")
+    ap.add_argument("--sweep-steps", type=int, default=20000, help="0 skips the 168-case sweep")
+    ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 cases")
+    ap.add_argument("--large-forms", default="f32z,f32,f64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-only", action="store_true",
-                    help="only the 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+                    help="only the 1 GiB streaming copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if a.sweep_steps and a.sweep_steps <= SWEEP_WARMUP + 1:
+        ap.error(f"--sweep-steps must exceed {SWEEP_WARMUP + 1} (or be 0)")
 
     import torch
     import _pkg
@@ -287,13 +361,18 @@ def main():
     stream = torch.cuda.Stream()
     timer = Timer(torch, stream)
     if a.copy_only:
-        print(json.dumps({"copy": copy_ceiling(torch, timer, stream, reps=3)}), flush=True)
+        print(json.dumps({"copy": stream_ceiling(m, timer, stream, reps=3)}), flush=True)
         return
 
     forms = {}
     for form in [a.prec] + [f for f in a.forms.split(",") if f and f != a.prec]:
-        say(f"bench: form {form}, 14 cases x 512^2, {a.steps} steps")
-        forms[form] = bench_form(m, timer, stream, form, a.steps, a.warmup)
+        say(f"bench: form {form}, 14 cases x 512^2, {a.tune_steps} tuning + {a.warmup} warm-up + {a.steps} timed steps")
+        forms[form] = bench_form(m, timer, stream, form, a.steps, a.warmup, a.tune_steps)
+    ref_dp = forms["f64"]["_dp"] if "f64" in forms else None
+    for form, r in forms.items():
+        dp = r.pop("_dp")
+        if ref_dp is not None and form != "f64":
+            r["delta_p_vs_f64"] = dp_agreement(dp, ref_dp)
     head = forms[a.prec]
 
     traffic = None
@@ -302,8 +381,8 @@ def main():
         with open(tpath) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
-    say("bench: 1 GiB device copy")
-    copy = copy_ceiling(torch, timer, stream)
+    say("bench: 1 GiB streaming copy")
+    copy = stream_ceiling(m, timer, stream)
     out = {
         "metric": "MLUPS + % HBM roofline, 1 GPU; sweep cases/hour vs plain FP64 at same Δp error",
         "value": head["MLUPS"],
@@ -311,11 +390,12 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
+        "tune_steps": a.tune_steps,
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.prec == "f64" else "f32",
+        "dtype": DTYPE[a.prec],
         "data": "synthetic",
         "config": {"workload": "512x512x1 static sharp drop R=100; tau {0.6, 1.0} x 7 operators "
                                "(P1, P3 chi -0.4/0.5/2, F1, F2, F3 xi 1.569988), sigma 0.1, beta 0.8; 14 cases "
@@ -324,15 +404,16 @@ def main():
                    "bytes_per_lattice_update": B_LU[a.prec]},
         "roofline": {"bound": "hbm", "kernel": head["dominant_kernel"], "achieved": head["achieved_GBs"],
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": head["frac"], "traffic": traffic,
-                     "frac_at_304B": head["frac_at_304B"],
                      "algorithmic_bytes_per_launch": head["bytes_per_launch"],
                      "avg_launch_us": head["kernel_avg_launch_us"],
-                     "copy_GBs": copy["GBs"], "frac_of_copy": head["achieved_GBs"] / copy["GBs"]},
+                     "stream_GBs": copy["GBs"], "frac_of_stream": head["achieved_GBs"] / copy["GBs"]},
         "forms": forms,
     }
     if a.large_steps > 0:
-        say(f"bench: 8192^2 {a.prec}, {a.large_steps} steps")
-        out["large"] = bench_large(m, timer, stream, a.prec if a.prec != "f64" else "f32z", a.large_steps)
+        out["large"] = {}
+        for form in [f for f in a.large_forms.split(",") if f]:
+            say(f"bench: 8192^2 {form}, {LARGE_TUNE} tuning + {a.large_steps} timed steps")
+            out["large"][form] = bench_large(m, timer, stream, form, a.large_steps)
     if a.sweep_steps > 0:
         sform = a.prec if a.prec != "f64" else "f32z"
         say(f"bench: 168-case sweep, {a.sweep_steps} steps, f64 per case vs {sform} batched")

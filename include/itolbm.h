@@ -171,6 +171,17 @@ int lbm_set_stream(lbm_ctx* ctx, void* hip_stream);
  * the per-step launch cost of small cases). */
 int lbm_set_graph(lbm_ctx* ctx, int steps_per_graph);
 
+/* Tile shape lbm_step uses for the kernel family of case case_id: rows per wavefront segment, the
+ * FP32/FP16 pipeline depth (0 = the kernel's default) and whether the shape was tuned (1) or is the
+ * default (0).  No reference counterpart (the reference has no tiles); for reports and benchmarks. */
+int lbm_tile_shape(const lbm_ctx* ctx, int case_id, int* rows, int* depth, int* tuned);
+
+/* Copies `bytes` bytes of device memory from src to dst (both 16-byte aligned) on the HIP stream
+ * hip_stream (NULL: the null stream) with the library's plain streaming kernel (16-byte loads,
+ * nontemporal stores).  No reference counterpart: it measures the bandwidth a read-once / write-once
+ * kernel reaches on the GPU, the ceiling the step kernels' rates are compared with. */
+int lbm_stream_copy(void* dst, const void* src, unsigned long long bytes, void* hip_stream);
+
 /* Includes (active != 0, the initial state) or leaves out (0) a case from lbm_step, e.g. once a batch
  * member has finished or become unstable; lbm_step_case still advances it.  No reference
  * counterpart (the reference runs one case per process). */

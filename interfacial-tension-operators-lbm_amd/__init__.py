@@ -28,7 +28,7 @@ STEP_RECORD_AUX = 1
 # every symbol include/itolbm.h declares
 SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "lbm_upload_state",
            "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case",
-           "lbm_step_case_noalloc", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
+           "lbm_step_case_noalloc", "lbm_tile_shape", "lbm_stream_copy", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
            "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl"]
 
 
@@ -89,6 +89,9 @@ def lib():
         L.lbm_step.argtypes = [P, C.c_int, C.c_int]
         L.lbm_step_case.argtypes = [P, C.c_int, C.c_int, C.c_int]
         L.lbm_step_case_noalloc.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        ip = C.POINTER(C.c_int)
+        L.lbm_tile_shape.argtypes = [P, C.c_int, ip, ip, ip]
+        L.lbm_stream_copy.argtypes = [P, P, C.c_ulonglong, P]
         L.lbm_sync.argtypes = [P]
         L.lbm_set_stream.argtypes = [P, C.c_void_p]
         L.lbm_set_reference.argtypes = [P, C.c_int, C.c_double, C.c_double]
@@ -124,6 +127,14 @@ def case_desc(nx, ny, op="P3 0.5", tau=1.0, A=0.225, beta=0.8):
 
 class LBMError(RuntimeError):
     pass
+
+
+def stream_copy(dst_ptr, src_ptr, nbytes, hip_stream=None):
+    """lbm_stream_copy: the library's streaming copy kernel between two device addresses."""
+    rc = lib().lbm_stream_copy(C.c_void_p(int(dst_ptr)), C.c_void_p(int(src_ptr)), int(nbytes),
+                               None if hip_stream is None else C.c_void_p(int(hip_stream)))
+    if rc != 0:
+        raise LBMError(f"lbm_stream_copy failed ({rc})")
 
 
 class Context:
@@ -195,6 +206,12 @@ class Context:
     def step_noalloc(self, n=1, record=False, case=0):
         """lbm_step_case_noalloc: one case, nothing allocated (static scratch on the device)."""
         self._chk(lib().lbm_step_case_noalloc(self.h, case, n, STEP_RECORD_AUX if record else 0))
+
+    def tile_shape(self, case=0):
+        """(rows per segment, pipeline depth, tuned) lbm_step uses for the case's kernel family."""
+        r, d, t = C.c_int(), C.c_int(), C.c_int()
+        self._chk(lib().lbm_tile_shape(self.h, case, C.byref(r), C.byref(d), C.byref(t)))
+        return r.value, d.value, bool(t.value)
 
     def set_stream(self, hip_stream):
         """Queues later work on a caller's hipStream_t (an int handle, e.g. torch's cuda_stream); None = own."""

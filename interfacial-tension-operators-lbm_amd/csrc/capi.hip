@@ -1098,6 +1098,24 @@ int lbm_set_stream(lbm_ctx* c, void* s)
     return LBM_OK;
 }
 
+int lbm_stream_copy(void* dst, const void* src, unsigned long long bytes, void* s)
+{
+    if (!dst || !src) return LBM_E_ARG;
+    const hipError_t e = launch_stream_copy(dst, src, (size_t)bytes, (hipStream_t)s);
+    if (e == hipErrorInvalidValue) return LBM_E_ARG;
+    return e == hipSuccess ? LBM_OK : LBM_E_HIP;
+}
+
+int lbm_tile_shape(const lbm_ctx* c, int id, int* rows, int* depth, int* tuned)
+{
+    if (!c || id < 0 || id >= (int)c->cases.size()) return LBM_E_ARG;
+    const Group& g = c->family[c->cases[id].kind];
+    if (rows) *rows = g.ly.empty() ? group_rows(c->cases, g.members) : g.ly[0];
+    if (depth) *depth = g.depth;
+    if (tuned) *tuned = g.tuned ? 1 : 0;
+    return LBM_OK;
+}
+
 int lbm_set_active(lbm_ctx* c, int id, int active)
 {
     if (int r = check_case(c, id)) return r;

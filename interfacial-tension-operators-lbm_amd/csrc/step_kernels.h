@@ -55,6 +55,9 @@ hipError_t scratch_setup(int device, void** out);
 // writes the table entry d and a tile map of ntiles tiles of case 0 into the scratch at `base`
 hipError_t launch_fill_static_table(void* base, const CaseDev& d, int ntiles, hipStream_t stream);
 
+// plain streaming copy of `bytes` bytes (16-byte aligned pointers), stream_copy.hip
+hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
+
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
 // FP64 state in row blocks [y][colour][19][nx] -> [colour][19][S]
 hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream);

@@ -170,6 +170,7 @@ hipError_t launch_drop_ordered(const OrderedArgs&, hipStream_t) MOCK_LAUNCH
 hipError_t launch_drop_modes(const ModesArgs&, hipStream_t) MOCK_LAUNCH
 hipError_t launch_incl_pass(const InclArgs&, int, hipStream_t) MOCK_LAUNCH
 hipError_t launch_fill_static_table(void*, const CaseDev&, int, hipStream_t) MOCK_LAUNCH
+hipError_t launch_stream_copy(void*, const void*, size_t, hipStream_t) MOCK_LAUNCH
 
 // the static scratch array of one device, as host memory
 hipError_t scratch_setup(int, void** out)
