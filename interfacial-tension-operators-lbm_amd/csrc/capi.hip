@@ -244,10 +244,11 @@ int ensure_aux(lbm_ctx* c, CaseState& k)
     return LBM_OK;
 }
 
-// Up to this many sites the drop's area and centroid sums run in the reference's site order on one
-// wavefront (bitwise equal sums, hence the same centre, radius and region masks; ~0.1 ms per 10^5
-// sites); larger boxes use the block-tree sums (equal to ~1e-16 relative).
-constexpr long long ORDERED_MAX_SITES = 1LL << 24;
+// The drop's area and centroid sums run in the reference's site order on one wavefront, whatever the
+// box size: bitwise equal sums, hence the same centre, radius and in / out region masks as the
+// reference (a block-tree order moves them by ~1e-16 relative, enough to move a site that lies on a
+// region boundary to the other side and flip delta_p).  Cost ~1 ns per site (67 ms at 8192^2, once
+// per diagnostic).
 
 int ensure_diag(lbm_ctx* c, CaseState& k)
 {
@@ -1190,8 +1191,7 @@ int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double 
     a1.gal = k.gal; a1.dsum = k.dsum; a1.aphi = k.ordered; a1.partial = k.partial;
     HIPCHK(c, launch_drop_pass1(a1, k.nblk, c->stream));
     // the sums that place the drop (area, circular centroid) in the reference's site order
-    const bool ordered = k.n <= ORDERED_MAX_SITES;
-    if (ordered) {
+    {
         OrderedArgs ao{};
         ao.aphi = k.ordered; ao.n = k.n; ao.nx = nx;
         ao.cosx = a1.cosx; ao.sinx = a1.sinx; ao.cosy = a1.cosy; ao.siny = a1.siny;
@@ -1200,7 +1200,7 @@ int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double 
     }
     std::vector<double> r1;
     if (int r = reduce_partials<P1_NP>(c, k, r1, P1_NS, P1_NX, P1_NN)) return r;
-    if (ordered) {
+    {
         double h[5];
         HIPCHK(c, hipMemcpyAsync(h, k.ordered + k.S, sizeof(h), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
