@@ -156,6 +156,33 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
             "bytes_per_lattice_update": B_LU[form], "steps_before_delta_p": total, "_dp": dp}
 
 
+FAMILY_OPS = {"P": "P3 0.5", "F3": "F3 1.569988", "F1": "F1", "F2": "F2"}
+
+
+def bench_family(m, timer, stream, family, form, steps, tune_steps):
+    """One kernel family alone: eight 512^2 sharp drops R = 100 of that family's operator (tau
+    alternating 0.6 / 1.0), so each launch is that family's kernel over 8 cases; tuned tile shape (or
+    the one ITOLBM_LY / ITOLBM_DEPTH fix)."""
+    op = FAMILY_OPS[family]
+    kind = "P" if family == "P" else "F"
+    descs = [m.case_desc(NX, NY, op, tau, amplitude(kind, tau), 0.8) for tau in TAUS * 4]
+    ctx = m.Context(descs, precision=PREC_ID[form])
+    ctx.set_stream(stream.cuda_stream)
+    for i in range(len(descs)):
+        ctx.init_drop(R, width=-1.0, case=i)
+    ctx.step(tune_steps)
+    ctx.sync()
+    rows, depth, tuned = ctx.tile_shape()
+    gpu_s, _ = timer.run(lambda: ctx.step(steps))
+    ctx.close()
+    avg = gpu_s / steps
+    nbytes = B_LU[form] * NX * NY * len(descs)
+    return {"family": family, "operator": op, "form": form, "cases": len(descs), "steps": steps,
+            "tile_shape": {"rows_per_segment": rows, "depth": depth, "tuned": tuned},
+            "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": nbytes, "achieved_GBs": nbytes / avg / 1e9,
+            "frac": nbytes / avg / 1e9 / PEAK_HBM_GBS}
+
+
 def dp_agreement(dp, ref):
     err = [abs(a - b) / abs(b) for a, b in zip(dp, ref) if b != 0 and a == a and b == b]
     err.sort()
@@ -347,6 +374,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-only", action="store_true",
                     help="only the 1 GiB streaming copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--family", choices=list(FAMILY_OPS),
+                    help="only one kernel family, timed alone over eight 512^2 cases in form --prec")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
@@ -362,6 +391,9 @@ def main():
     timer = Timer(torch, stream)
     if a.copy_only:
         print(json.dumps({"copy": stream_ceiling(m, timer, stream, reps=3)}), flush=True)
+        return
+    if a.family:
+        print(json.dumps(bench_family(m, timer, stream, a.family, a.prec, a.steps, a.tune_steps)), flush=True)
         return
 
     forms = {}

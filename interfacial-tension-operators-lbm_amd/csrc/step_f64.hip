@@ -100,6 +100,24 @@ __device__ __forceinline__ double cdot(double a0, double a1, double a2)
     return s;
 }
 
+// c_i . a for a direction i known only at run time (uniform across the wavefront): the same operations
+// as cdot<i>, the zero components left out by uniform branches
+constexpr unsigned CZP_MASK = (1u << 5) | (1u << 11) | (1u << 14) | (1u << 16) | (1u << 17);
+constexpr unsigned CZN_MASK = (1u << 6) | (1u << 12) | (1u << 13) | (1u << 15) | (1u << 18);
+__device__ __forceinline__ int cx_of(int i) { return (int)((CXP_MASK >> i) & 1u) - (int)((CXN_MASK >> i) & 1u); }
+__device__ __forceinline__ int cy_of(int i) { return (int)((CYP_MASK >> i) & 1u) - (int)((CYN_MASK >> i) & 1u); }
+__device__ __forceinline__ int cz_of(int i) { return (int)((CZP_MASK >> i) & 1u) - (int)((CZN_MASK >> i) & 1u); }
+__device__ __forceinline__ double cdot_rt(int i, double a0, double a1, double a2)
+{
+    const int c0 = cx_of(i), c1 = cy_of(i), c2 = cz_of(i);
+    double s = 0.0;
+    bool first = true;
+    if (c0 != 0) { s = (c0 > 0) ? a0 : -a0; first = false; }
+    if (c1 != 0) { const double t = (c1 > 0) ? a1 : -a1; s = first ? t : s + t; first = false; }
+    if (c2 != 0) { const double t = (c2 > 0) ? a2 : -a2; s = first ? t : s + t; first = false; }
+    return s;
+}
+
 // Nine values of a field around a lane: v[dy + 1][dx + 1] = field(x + dx, y + dy).
 struct Nine {
     double v[3][3];
@@ -401,9 +419,32 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
         vr = t.in[e];
         vb = t.in[e + (unsigned)(Q * nx)];
     };
+    // colour field of the two halo columns: their 38 pulled values go through LDS to lane 0 (left
+    // site) and lane 63 (right site), the only lanes that use them, each summing in index order as
+    // dens() (summing on the wavefront's uniform values took 152 lane reads per row)
+    __shared__ double hbuf[2][2 * Q];
     auto halo_phase = [&](double vr, double vb, double& l, double& r) {
-        l = phase(lane_dens(vr, 0), lane_dens(vb, 0));
-        r = phase(lane_dens(vr, Q), lane_dens(vb, Q));
+        const int lane = L.lane;
+        if (lane < 2 * Q) {
+            hbuf[0][lane] = vr;
+            hbuf[1][lane] = vb;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        double v = 0.0;
+        if (lane == 0 || lane == 63) {
+            const double* pr = hbuf[0] + (lane ? Q : 0);
+            const double* pb = hbuf[1] + (lane ? Q : 0);
+            double rR = pr[0], rB = pb[0];
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                rR = rR + pr[i];
+                rB = rB + pb[i];
+            }
+            v = phase(rR, rB);
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // hbuf is written again by the next row
+        l = v;
+        r = v;
     };
 
     double cR_[Q], cB_[Q], nR[Q], nB[Q];
@@ -506,139 +547,169 @@ __device__ __forceinline__ Aux6 aux_of(const CaseConst& cc, double g0, double g1
 }
 
 // Aligned 64-column strips for F1 / F2: the colour field is needed two columns beyond the strip on
-// each side; those four halo columns come from two extra loads per colour and row (Halo2) and their
-// gradient and n / T fields are evaluated on the wavefront's uniform values with the same functions
-// as every lane's, so the results are identical to the owning lanes'.
+// each side; those four halo columns come from two extra loads per colour and row (Halo2), summed in
+// the same order as every other site's, so the halo values are identical to the owning lanes'.
+//
+// The fields derived from the populations live in LDS rings, not in registers:
+//   phi  colour field, 5 rows (y - 2 .. y + 2) x 68 columns (x0 - 2 .. x0 + 65);
+//   n/T  F1's unit normal n or F2's tensor T (NA doubles), 3 rows (y - 1 .. y + 1) x 66 columns
+//        (x0 - 1 .. x0 + 64).
+// Each row iteration forms the colour field of row y + 2 from its pulled populations, n / T of row
+// y + 1 from the colour-field ring, and the force and mediator gradient of row y from the rings;
+// then it collides row y, whose populations are read again (read two iterations ago for the colour
+// field).  Holding the 3 x 3 neighbourhoods of every component and the five colour-field rows in
+// registers took 256 VGPRs plus accumulation registers (one wavefront per SIMD); with the rings it
+// takes 169 (F1) / 209 (F2), two wavefronts per SIMD, and summing the halo sites' densities through
+// LDS instead of 304 lane reads per row halved the instructions of the row loop.  Eight 512^2 cases:
+// F1 347 -> 309 us, F2 376 -> 318 us per launch.  What bounds them now is the re-read of row y: the
+// rows between a strip's colour-field read and its re-read (3 x 19.5 KB per wavefront) exceed an
+// XCD's 4 MB L2 at two wavefronts per SIMD, so the re-read comes from the last-level cache (L2 fetch
+// 2.2x the algorithmic reads).  Every value is computed with the same expressions as before, so the
+// results are unchanged bit for bit.
 template <int OP>
 __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
 {
     constexpr int NA = (OP == OP_F1) ? 3 : 6;
+    constexpr int NP = 68;   // colour-field ring columns: x0 - 2 (0) .. x0 + 65 (67)
+    constexpr int NC = 66;   // n / T ring columns: x0 - 1 (0) .. x0 + 64 (65)
+    __shared__ double phir[5][NP];
+    __shared__ double ring[3][NA][NC];
+    __shared__ double hbuf[2][4 * Q];   // pulled populations of the four halo sites, by colour
     TileCtx t;
     if (!tile_ctx(launch, t)) return;
     const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
     const Lane L = make_lane_aligned(nx, t.strip);
     const Halo2 h = make_halo2(nx, t.strip);
+    const int lane = L.lane;
+    // ring slots: colour field of row yy (yy >= y0 - 2), n / T of row yy (yy >= y0 - 1)
+    auto pslot = [&](int yy) { return (yy - t.y0 + 2) % 5; };
+    auto nslot = [&](int yy) { return (yy - t.y0 + 1) % 3; };
+    // LDS written by some lanes is read by others below: the wavefront's LDS operations run in order,
+    // and this keeps the compiler from moving reads above writes (it waits on LDS only)
+    auto lds_order = [] { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); };
 
-    double fR[Q], fB[Q];
-    // colour field of a row: own lanes, and the four halo columns (c0-2, c0-1, c0+64, c0+65)
-    auto phi_row = [&](int y, double hw[4]) {
-        load_row(t.in, nx, ny, y, L, fR, fB);
-        const unsigned long long e0 = elem_index(nx, wrap_row3(y, h.dy[0], ny), 0, h.i[0], h.x[0]);
-        const unsigned long long e1 = elem_index(nx, wrap_row3(y, h.dy[1], ny), 0, h.i[1], h.x[1]);
+    // colour field of row yy into its ring slot: own lanes, and the four halo columns.  The halo sites'
+    // 76 pulled values (two loads per colour, value g = 19 s + i in lane g of load A or g - 64 of load B)
+    // go through LDS to lanes 0 .. 3, each of which sums one site's values in index order, as dens().
+    auto phi_row = [&](int yy) {
+        double fR[Q], fB[Q];
+        load_row(t.in, nx, ny, yy, L, fR, fB);
+        const unsigned long long e0 = elem_index(nx, wrap_row3(yy, h.dy[0], ny), 0, h.i[0], h.x[0]);
+        const unsigned long long e1 = elem_index(nx, wrap_row3(yy, h.dy[1], ny), 0, h.i[1], h.x[1]);
         const unsigned cb = (unsigned)(Q * nx);
         const double ra = t.in[e0], ba = t.in[e0 + cb], rb = t.in[e1], bb = t.in[e1 + cb];
+        double* row = phir[pslot(yy)];
+        row[lane + 2] = phase(dens(fR), dens(fB));
+        hbuf[0][lane] = ra;
+        hbuf[1][lane] = ba;
+        if (lane < 4 * Q - 64) {
+            hbuf[0][64 + lane] = rb;
+            hbuf[1][64 + lane] = bb;
+        }
+        lds_order();
+        if (lane < 4) {
+            const double* vr = hbuf[0] + Q * lane;
+            const double* vb = hbuf[1] + Q * lane;
+            double rR = vr[0], rB = vb[0];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) hw[q] = phase(halo2_dens(ra, rb, q), halo2_dens(ba, bb, q));
-        return phase(dens(fR), dens(fB));
+            for (int i = 1; i < Q; ++i) {
+                rR = rR + vr[i];
+                rB = rB + vb[i];
+            }
+            row[lane < 2 ? lane : NP - 4 + lane] = phase(rR, rB);
+        }
+        lds_order();   // hbuf is written again by the next row
     };
-    // rows y-2 .. y+2 of the colour field: own lanes P, halo columns HW
-    double P[5], HW[5][4];
-    P[0] = phi_row(t.y0 - 2, HW[0]);
-    P[1] = phi_row(t.y0 - 1, HW[1]);
-    P[2] = phi_row(t.y0, HW[2]);
-    P[3] = phi_row(t.y0 + 1, HW[3]);
-    // gradient of the colour field at the centre row of three (k .. k+2 of the window), own lanes
-    auto grad_lanes = [&](int k, double& g0, double& g1, double& g2) {
-        const double r[3] = {P[k], P[k + 1], P[k + 2]};
-        const double hl[3] = {HW[k][1], HW[k + 1][1], HW[k + 2][1]};
-        const double hr[3] = {HW[k][2], HW[k + 1][2], HW[k + 2][2]};
-        phase_gradient(gather9h(r, hl, hr, L.lane), g0, g1, g2);
-    };
-    // n / T at the halo columns c0-1 (side 0) and c0+64 (side 1), centre row k+1 of the window
-    auto aux_halo = [&](int k, int side) {
+    // 3 x 3 colour field around ring column c (colour-field ring coordinates) of row yy
+    auto phi9 = [&](int yy, int c) {
         Nine n;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            if (side == 0) {
-                n.v[j][0] = HW[k + j][0];
-                n.v[j][1] = HW[k + j][1];
-                n.v[j][2] = lane_value(P[k + j], 0);
-            } else {
-                n.v[j][0] = lane_value(P[k + j], 63);
-                n.v[j][1] = HW[k + j][2];
-                n.v[j][2] = HW[k + j][3];
-            }
+            const double* row = phir[pslot(yy - 1 + j)];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) n.v[j][i] = row[c - 1 + i];
         }
-        double g0, g1, g2;
-        phase_gradient(n, g0, g1, g2);
-        return aux_of<OP>(cc, g0, g1, g2);
+        return n;
     };
-    double gm0, gm1, gm2, g00, g01, g02;
-    grad_lanes(0, gm0, gm1, gm2);   // row y0-1
-    grad_lanes(1, g00, g01, g02);   // row y0
-    Aux6 am = aux_of<OP>(cc, gm0, gm1, gm2), a0 = aux_of<OP>(cc, g00, g01, g02);
+    // n / T of row yy into its ring slot (columns x0 - 1 .. x0 + 64); g receives the own lanes'
+    // colour-field gradient at row yy
+    auto nt_row = [&](int yy, double& g0, double& g1, double& g2) {
+        const int s = nslot(yy);
+        phase_gradient(phi9(yy, lane + 2), g0, g1, g2);
+        const Aux6 a = aux_of<OP>(cc, g0, g1, g2);
+#pragma unroll
+        for (int q = 0; q < NA; ++q) ring[s][q][lane + 1] = a.a[q];
+        // the two halo columns, on lanes 0 (x0 - 1) and 1 (x0 + 64)
+        if (lane < 2) {
+            double h0, h1, h2;
+            phase_gradient(phi9(yy, lane == 0 ? 1 : NP - 2), h0, h1, h2);
+            const Aux6 b = aux_of<OP>(cc, h0, h1, h2);
+#pragma unroll
+            for (int q = 0; q < NA; ++q) ring[s][q][lane == 0 ? 0 : NC - 1] = b.a[q];
+        }
+    };
+
+    // one row of populations in flight at a time (the scheduler would otherwise issue all four rows'
+    // loads together and size the kernel's registers for them)
+    for (int k = -2; k <= 1; ++k) {
+        phi_row(t.y0 + k);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    lds_order();
+    double g00, g01, g02;   // colour-field gradient at row y (F1 scales it by the curvature)
+    nt_row(t.y0 - 1, g00, g01, g02);
+    nt_row(t.y0, g00, g01, g02);
 
     for (int y = t.y0; y < t.y1; ++y) {
-        P[4] = phi_row(y + 2, HW[4]);
-        // populations of row y again (read two rows ago for the colour field, now in L2); issued
-        // before the force evaluation so that they arrive while it runs
-        load_row(t.in, nx, ny, y, L, fR, fB);
+        phi_row(y + 2);
+        lds_order();
         double gp0, gp1, gp2;
-        grad_lanes(2, gp0, gp1, gp2);   // row y+1
-        const Aux6 ap = aux_of<OP>(cc, gp0, gp1, gp2);
-        // halo n / T for rows y-1, y, y+1
-        Aux6 hl[3], hr[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            hl[j] = aux_halo(j, 0);
-            hr[j] = aux_halo(j, 1);
-        }
-
-        // neighbour aux values: A[k][dy+1][dx+1]
-        double A[NA][3][3];
-#pragma unroll
-        for (int k = 0; k < NA; ++k) {
-            const double r[3] = {am.a[k], a0.a[k], ap.a[k]};
-            const double l3[3] = {hl[0].a[k], hl[1].a[k], hl[2].a[k]};
-            const double r3[3] = {hr[0].a[k], hr[1].a[k], hr[2].a[k]};
-            const Nine n = gather9h(r, l3, r3, L.lane);
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) A[k][rr][c] = n.v[rr][c];
-        }
+        nt_row(y + 1, gp0, gp1, gp2);
+        lds_order();
+        const int s0 = nslot(y);
+        // neighbour value (dx, dy) of component q from the n / T ring (the slot by arithmetic: a choice
+        // among three slot variables became a private-memory lookup table)
+        auto A = [&](int q, int dx, int dy) { return ring[(s0 + dy + 3) % 3][q][lane + 1 + dx]; };
         double Fx, Fy, Fz;
         if constexpr (OP == OP_F1) {
             // kappa = -div_iso(n) (Operadores_tensao.cpp:187-200, 300-304)
+            // (a loop over the directions kept rolled needs 25 (F1) / 71 (F2) fewer VGPRs, three waves per
+            // SIMD instead of two, but waits on each direction's ring reads in turn: 10-20 % slower)
             double d = 0.0;
-#define ITOLBM_DIV(i)                                                                               \
-    d += (i < 7 ? W1 : W2) * cdot<i>(A[0][CY[i] + 1][CX[i] + 1], A[1][CY[i] + 1][CX[i] + 1],       \
-                                     A[2][CY[i] + 1][CX[i] + 1]);
-            ITOLBM_DIV(1) ITOLBM_DIV(2) ITOLBM_DIV(3) ITOLBM_DIV(4) ITOLBM_DIV(5) ITOLBM_DIV(6)
-            ITOLBM_DIV(7) ITOLBM_DIV(8) ITOLBM_DIV(9) ITOLBM_DIV(10) ITOLBM_DIV(11) ITOLBM_DIV(12)
-            ITOLBM_DIV(13) ITOLBM_DIV(14) ITOLBM_DIV(15) ITOLBM_DIV(16) ITOLBM_DIV(17) ITOLBM_DIV(18)
-#undef ITOLBM_DIV
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                const int dx = cx_of(i), dy = cy_of(i);
+                d += (i < 7 ? W1 : W2) * cdot_rt(i, A(0, dx, dy), A(1, dx, dy), A(2, dx, dy));
+            }
             const double kappa = -(3.0 * d);
             Fx = cc.hs * kappa * g00; Fy = cc.hs * kappa * g01; Fz = cc.hs * kappa * g02;
         } else {
-            // F_a = d_b T_ab (Operadores_tensao.cpp:306-324); T = xx yy zz xy xz yz
+            // F_a = d_b T_ab (Operadores_tensao.cpp:306-324); T = xx yy zz xy xz yz.  The three sums are
+            // independent chains over the directions; each is formed in turn, reading only the three
+            // components it needs (the order within each sum is the reference's)
             double sx = 0.0, sy = 0.0, sz = 0.0;
-#define ITOLBM_DIVT(i)                                                                              \
-    {                                                                                               \
-        const double T[6] = {A[0][CY[i] + 1][CX[i] + 1], A[1][CY[i] + 1][CX[i] + 1],                \
-                             A[2][CY[i] + 1][CX[i] + 1], A[3][CY[i] + 1][CX[i] + 1],                \
-                             A[4][CY[i] + 1][CX[i] + 1], A[5][CY[i] + 1][CX[i] + 1]};               \
-        constexpr double w = (i < 7 ? W1 : W2);                                                     \
-        sx += w * cdot<i>(T[0], T[3], T[4]);                                                        \
-        sy += w * cdot<i>(T[3], T[1], T[5]);                                                        \
-        sz += w * cdot<i>(T[4], T[5], T[2]);                                                        \
-    }
-            ITOLBM_DIVT(1) ITOLBM_DIVT(2) ITOLBM_DIVT(3) ITOLBM_DIVT(4) ITOLBM_DIVT(5) ITOLBM_DIVT(6)
-            ITOLBM_DIVT(7) ITOLBM_DIVT(8) ITOLBM_DIVT(9) ITOLBM_DIVT(10) ITOLBM_DIVT(11) ITOLBM_DIVT(12)
-            ITOLBM_DIVT(13) ITOLBM_DIVT(14) ITOLBM_DIVT(15) ITOLBM_DIVT(16) ITOLBM_DIVT(17) ITOLBM_DIVT(18)
-#undef ITOLBM_DIVT
+            auto chain = [&](double& acc, int a, int b, int c) {
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+                    const int dx = cx_of(i), dy = cy_of(i);
+                    acc += (i < 7 ? W1 : W2) * cdot_rt(i, A(a, dx, dy), A(b, dx, dy), A(c, dx, dy));
+                }
+            };
+            chain(sx, 0, 3, 4);
+            chain(sy, 3, 1, 5);
+            chain(sz, 4, 5, 2);
             Fx = 3.0 * sx; Fy = 3.0 * sy; Fz = 3.0 * sz;
         }
-
         double gx, gy, gz;
-        {
-            const double r[3] = {P[1], P[2], P[3]};
-            const double hl1[3] = {HW[1][1], HW[2][1], HW[3][1]};
-            const double hr0[3] = {HW[1][2], HW[2][2], HW[3][2]};
-            mediator_gradient(gather9h(r, hl1, hr0, L.lane), gx, gy, gz);
-        }
+        mediator_gradient(phi9(y, lane + 2), gx, gy, gz);
         const double mod = sqrt(gx * gx + gy * gy + gz * gz);
+        // populations of row y again (read two iterations ago for the colour field, now in L2), issued
+        // once the force is formed: the scheduler would otherwise hoist them above the force's ring
+        // reads and hold both at once
+        __builtin_amdgcn_sched_barrier(0);
+        double fR[Q], fB[Q];
+        load_row(t.in, nx, ny, y, L, fR, fB);
 
         if (L.interior) {
             const unsigned site = (unsigned)(y * nx + L.col);
@@ -649,19 +720,14 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
             if (!s.skip) {
                 bgk<true>(cc, s, Fx, Fy, Fz);
                 recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) { store_pop(ro, nx, xoff, i, r, b); });
-            } else {
-                store_row(ro, nx, xoff, fR, fB);
+            } else {   // left untouched (rare): the raw populations again
+                double uR[Q], uB[Q];
+                load_row(t.in, nx, ny, y, L, uR, uB);
+                store_row(ro, nx, xoff, uR, uB);
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            P[k] = P[k + 1];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) HW[k][q] = HW[k + 1][q];
-        }
         g00 = gp0; g01 = gp1; g02 = gp2;
-        am = a0; a0 = ap;
     }
 }
 
