@@ -237,6 +237,10 @@ int main(int argc, char** argv)
     std::vector<std::string> dirs;
     for (int i = 1; i < argc; ++i) dirs.emplace_back(argv[i]);
     const bool batch = dirs.size() > 1;
+    // a case directory given as an argument gets its console output in <dir>/execucao.log, as the
+    // reference's batch script keeps it (rodar_lote.py:234-235); with no argument the case is the
+    // working directory and the output goes to the console
+    const bool to_log = !dirs.empty();
     if (dirs.empty()) dirs.emplace_back(".");
     std::vector<std::unique_ptr<DropCase>> cases;
     std::map<int, std::vector<Member*>> by_prec;
@@ -244,14 +248,14 @@ int main(int argc, char** argv)
     for (const auto& d : dirs) {
         auto c = std::make_unique<DropCase>();
         c->dir = d;
-        if (batch) {
+        if (to_log) {
             c->log = std::make_unique<std::ofstream>(join(d, "execucao.log"));
             c->con = c->log.get();
         } else {
             c->con = &std::cout;
         }
         if (!c->load()) {
-            if (batch) std::fprintf(stderr, "%s: invalid case, skipped\n", d.c_str());
+            if (to_log) std::fprintf(stderr, "%s: invalid case, skipped\n", d.c_str());
             rc = 1;
             continue;
         }

@@ -78,3 +78,14 @@ def test_small_batch_runs_batched(tmp_path):
     r = subprocess.run([sys.executable, str(PKG / "lote.py"), "E1", "--filtro", "R=10", "--passos", "1000",
                         "--saida", str(tmp_path)], capture_output=True, text=True, timeout=600)
     assert "nothing to do" in r.stdout
+
+
+def test_single_case_directory_gets_its_log(tmp_path):
+    """A program given one case directory (a lote.py chunk of one case) writes that case's console output
+    to <dir>/execucao.log, as the reference's batch script keeps it (rodar_lote.py:234-235).  Without a
+    device the program stops at lbm_create, after the log is open, which is all this checks."""
+    d = tmp_path / "um"
+    K.make_case(str(d), 16, "P3 0.5", 1.0, 0.225, 10)
+    subprocess.run([K.program("Imm_OPS"), str(d)], capture_output=True, text=True, timeout=60)
+    assert (d / "execucao.log").exists()
+    assert "dat_out.txt" in (d / "execucao.log").read_text()
