@@ -753,6 +753,125 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     }
 }
 
+// Depth-1 form of f12_rows without the re-read: the raw row loaded for a row's colour field is kept in
+// registers until that row is collided two iterations later, in three rotating buffers (the row loop
+// unrolled three times, so every buffer role is a compile-time constant).  The re-read of f12_rows
+// mostly missed L2 (fetch 1.8-2.8x the algorithmic reads over eight 512^2 cases): the rows between a
+// strip's two reads of a row exceed an XCD's 4 MB L2 at the wavefronts in flight.  Two more raw rows
+// cost 28 (FP32, z-symmetric) to 76 (FP32) VGPRs.  Same arithmetic per site as f12_rows.
+template <typename ST, int KIND, bool SYM, bool REV>
+__device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
+{
+    using R = typename PairOf<ST>::raw;
+    const int nx = t.cc.nx, ny = t.cc.ny;
+    const ConstF k = to_float(t.cc);
+    const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
+    const Lane L = make_lane_aligned(nx, t.strip);
+    gcst<R>* in = (gcst<R>*)t.in;
+    gst<R>* out = (gst<R>*)t.out;
+    Halo2LP<ST, SYM> h;
+    h.init(nx, t.strip);
+    const int y0 = t.y0, y1 = t.y1, n = y1 - y0;
+    auto row = [&](int kk) { return REV ? y1 - 1 - kk : y0 + kk; };
+    auto G = [](int r) { return REV ? 2 - r : r; };
+
+    R buf[3][Q];            // raw row row(kk) in buffer kk mod 3, from its colour field to its collision
+    float P[5], HW[5][4];   // rows row(k - 2) .. row(k + 2)
+    double pd[3];           // FP64 colour field of rows row(k), row(k + 1), row(k + 2) (for their collision)
+    auto phi_row = [&](int y, R (&dst)[Q], float (&hw)[4]) {
+        load_raw<ST, SYM>(in, nx, ny, y, L, dst);
+        R va, vb;
+        h.load(in, nx, ny, y, va, vb);
+        halo2_phase<ST>(va, vb, rf, hw);
+        return raw_phase<ST>(dst, rf);
+    };
+    auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
+        const float hl[3] = {HW[j + G(0)][1], HW[j + 1][1], HW[j + G(2)][1]};
+        const float hr[3] = {HW[j + G(0)][2], HW[j + 1][2], HW[j + G(2)][2]};
+        return aux_of<KIND>(k, gather9h(P[j + G(0)], P[j + 1], P[j + G(2)], hl, hr, L.lane));
+    };
+    auto aux_halo = [&](int j, int side) {   // n / T at c0-1 (side 0) or c0+64 (side 1)
+        Nine nn;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int w = j + G(r);
+            if (side == 0) {
+                nn.v[r][0] = HW[w][0];
+                nn.v[r][1] = HW[w][1];
+                nn.v[r][2] = lane_value(P[w], 0);
+            } else {
+                nn.v[r][0] = lane_value(P[w], 63);
+                nn.v[r][1] = HW[w][2];
+                nn.v[r][2] = HW[w][3];
+            }
+        }
+        return aux_of<KIND>(k, nn);
+    };
+    {
+        R tmp[Q];
+        P[0] = (float)phi_row(row(-2), tmp, HW[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        P[1] = (float)phi_row(row(-1), tmp, HW[1]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    pd[1] = phi_row(row(0), buf[0], HW[2]);
+    P[2] = (float)pd[1];
+    pd[2] = phi_row(row(1), buf[1], HW[3]);
+    P[3] = (float)pd[2];
+    AuxF am = aux_lanes(0), a0 = aux_lanes(1);
+    // n / T at the halo columns of rows row(k-1) and row(k) (carried along; row(k+1)'s is new each row)
+    AuxF hl[3], hr[3];
+    hl[0] = aux_halo(0, 0); hr[0] = aux_halo(0, 1);
+    hl[1] = aux_halo(1, 0); hr[1] = aux_halo(1, 1);
+
+    for (int k0 = 0; k0 < n; k0 += 3) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int kk = k0 + j;
+            if (kk >= n) break;
+            const int yy = row(kk);
+            pd[0] = pd[1];
+            pd[1] = pd[2];
+            pd[2] = phi_row(row(kk + 2), buf[(j + 2) % 3], HW[4]);
+            P[4] = (float)pd[2];
+            const AuxF ap = aux_lanes(2);
+            hl[2] = aux_halo(2, 0);
+            hr[2] = aux_halo(2, 1);
+            auto field = [&](float AuxF::*m) {
+                const float l3[3] = {hl[G(0)].*m, hl[1].*m, hl[G(2)].*m};
+                const float r3[3] = {hr[G(0)].*m, hr[1].*m, hr[G(2)].*m};
+                return REV ? gather9h(ap.*m, a0.*m, am.*m, l3, r3, L.lane)
+                          : gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
+            };
+            float Fx, Fy;
+            if constexpr (KIND == KIND_F1) {
+                const float divn = dx_iso(field(&AuxF::a0)) + dy_iso(field(&AuxF::a1));
+                Fx = -k.hs * divn * a0.g0;
+                Fy = -k.hs * divn * a0.g1;
+            } else {
+                const Nine txy = field(&AuxF::a2);
+                Fx = dx_iso(field(&AuxF::a0)) + dy_iso(txy);
+                Fy = dx_iso(txy) + dy_iso(field(&AuxF::a1));
+            }
+            const float hl1[3] = {HW[1 + G(0)][1], HW[2][1], HW[1 + G(2)][1]};
+            const float hr0[3] = {HW[1 + G(0)][2], HW[2][2], HW[1 + G(2)][2]};
+            const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0, L.lane);
+            const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
+            collide_store<ST, true, SYM>(t, k, rf, L, out, yy, buf[j], pd[0], gx, gy, Fx, Fy);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                P[r] = P[r + 1];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) HW[r][q] = HW[r + 1][q];
+            }
+            am = a0;
+            a0 = ap;
+            hl[0] = hl[1]; hl[1] = hl[2];
+            hr[0] = hr[1]; hr[1] = hr[2];
+        }
+    }
+}
+
 template <typename ST, int KIND, int DEPTH, bool SYM>
 __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
 {
@@ -762,8 +881,13 @@ __global__ void __launch_bounds__(64) lp_f12(StepLaunch launch)
     if (!tile_ctx(launch, t)) return;
     // FP16: the second body's registers cost more than the shared rows save (measured), so FP16
     // segments all march forward; the sites' results are the same either way
-    if (std::is_same<ST, float>::value && t.rev) f12_rows<ST, KIND, DEPTH, SYM, std::is_same<ST, float>::value>(t);
-    else f12_rows<ST, KIND, DEPTH, SYM, false>(t);
+    if constexpr (DEPTH == 1) {
+        if (std::is_same<ST, float>::value && t.rev) f12_rows_cached<ST, KIND, SYM, std::is_same<ST, float>::value>(t);
+        else f12_rows_cached<ST, KIND, SYM, false>(t);
+    } else {
+        if (std::is_same<ST, float>::value && t.rev) f12_rows<ST, KIND, DEPTH, SYM, std::is_same<ST, float>::value>(t);
+        else f12_rows<ST, KIND, DEPTH, SYM, false>(t);
+    }
 }
 
 // Halo of an aligned strip for the pair layout: one wavefront load per row (halo_lane_lp).  The address is
