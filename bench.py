@@ -8,8 +8,8 @@ F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (one laun
 family per step).  A bench step advances every case by one time step.  Inputs are synthetic
 (generated by the library's initial condition), resident in HBM before timing.
 
-Every form first runs --tune-steps untimed steps (default 400, whatever --warmup is): the library
-tunes its tile shapes on the first ~300-360 of them, and the shapes it chose are reported.  Then
+Every form first runs --tune-steps untimed steps (default 500, whatever --warmup is): the library
+tunes its tile shapes on the first ~340-400 of them, and the shapes it chose are reported.  Then
 --warmup untimed steps, then --steps timed steps.  The headline (--prec, default f64) is the FP64
 drop-in form, whose arithmetic is the reference's; the other storage forms are timed the same way
 and reported under "forms", each with its delta_p difference from the FP64 form after the same steps:
@@ -24,8 +24,9 @@ and reported under "forms", each with its delta_p difference from the FP64 form 
 Also reported:
 
   roofline       the dominant kernel (the P1/P3 family launch, 8 of the 14 cases) timed alone with HIP
-                 events on its stream: algorithmic bytes per launch (the form's stored bytes) / average
-                 launch duration; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+                 events on its stream, at the tile shape the library tunes for it alone (after
+                 --tune-steps more untimed steps): algorithmic bytes per launch (the form's stored
+                 bytes) / average launch duration; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
                  runs (profiles/traffic_<form>.json), when present; stream_GBs = the library's own
                  streaming kernel (read + write of 1 GiB, 16-byte loads, nontemporal stores) timed the
                  same way, the bandwidth a plain streaming kernel reaches on this GPU
@@ -137,13 +138,19 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     total = tune_steps + warmup + steps + 1
     dp = [ctx.diag_drop(0.0, total - 1, case=i).delta_p for i in range(len(cases))]
     n_sites = NX * NY * len(cases)
-    # the dominant kernel alone: the P1/P3 family launch (8 cases), same launch shape
+    # the dominant kernel alone: the P1/P3 family launch (8 cases), at the tile shape the library tunes
+    # for it running alone (leaving a member out and back in re-tunes the family)
     for i, k in enumerate(kinds):
         if k != "P":
             ctx.set_active(False, case=i)
+    p0 = kinds.index("P")
+    ctx.set_active(False, case=p0)
+    ctx.set_active(True, case=p0)
     nP = sum(1 for k in kinds if k == "P")
     ksteps = max(steps, 200)
-    ctx.step(10)
+    ctx.step(tune_steps)
+    ctx.sync()
+    kshape = ctx.tile_shape(case=p0)
     kern_s, _ = timer.run(lambda: ctx.step(ksteps))
     ctx.close()
     avg = kern_s / ksteps
@@ -151,6 +158,7 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     return {"MLUPS": n_sites * steps / wall_s / 1e6, "ms_per_step": wall_s / steps * 1e3,
             "gpu_ms_per_step": gpu_s / steps * 1e3, "tile_shapes": shapes,
             "dominant_kernel": KERNEL[form], "kernel_cases_per_launch": nP,
+            "kernel_tile_shape": {"rows_per_segment": kshape[0], "depth": kshape[1], "tuned": kshape[2]},
             "kernel_launches_timed": ksteps, "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
             "achieved_GBs": bytes_launch / avg / 1e9, "frac": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
             "bytes_per_lattice_update": B_LU[form], "steps_before_delta_p": total, "_dp": dp}
@@ -363,8 +371,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50, help="untimed steps after the tuning steps")
-    ap.add_argument("--tune-steps", type=int, default=400,
-                    help="untimed steps first, on which the library tunes its tile shapes (needs ~360)")
+    ap.add_argument("--tune-steps", type=int, default=500,
+                    help="untimed steps first, on which the library tunes its tile shapes (needs ~400)")
     ap.add_argument("--prec", default="f64", choices=list(B_LU), help="headline storage form")
     ap.add_argument("--forms", default="f32,f32z,f16,f16z", help="forms measured beside the headline")
     ap.add_argument("--sweep-steps", type=int, default=20000, help="0 skips the 168-case sweep")

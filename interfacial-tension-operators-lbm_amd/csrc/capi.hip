@@ -824,9 +824,15 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
     constexpr int W = 1, T = 6;
     constexpr int LYS[] = {1, 2, 4, 8, 16, 32, 64};   // short segments suit small cases (more wavefronts)
     auto depths_of = [&](const Group& g) { return c->cases[g.members[0]].elem == 8 ? 1 : 3; };
+    constexpr int COMMON[] = {4, 8, 16, 32, 64};
     long long budget = 0;
+    int untuned = 0;
     for (const Group* g : act)
-        if (!g->tuned) budget += (long long)(depths_of(*g) + 6) * (W + T) + 2 * (W + 2 * T);
+        if (!g->tuned) {
+            budget += (long long)(depths_of(*g) + 6) * (W + T) + 2 * (W + 2 * T);
+            ++untuned;
+        }
+    if (untuned > 1) budget += (long long)(sizeof(COMMON) / sizeof(COMMON[0])) * (W + T);
     if (budget > left) return LBM_OK;
     if (!c->ev[0]) {
         HIPCHK(c, hipEventCreate(&c->ev[0]));
@@ -844,6 +850,36 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         }
         return LBM_OK;
     };
+    // First one rows-per-segment for every untuned family together: the families share the GPU in a
+    // step, and each family's search below starts from the others' shapes.  Starting from the
+    // families' separate defaults, the coordinate search settled on mixes up to 3 % slower than the
+    // best common shape (eight 512^2 cases of four families, FP64).
+    if (untuned > 1) {
+        float best = 0.f;
+        int best_ly = 16;
+        for (int v : COMMON) {
+            for (Group* g : act)
+                if (!g->tuned) {
+                    g->ly.assign(g->members.size(), v);
+                    g->depth = 0;
+                    g->written = false;
+                    if (int r = group_sync(c, *g)) return r;
+                }
+            float ms = 0.f;
+            if (int r = run(W, nullptr)) return r;
+            if (int r = run(T, &ms)) return r;
+            if (v == COMMON[0] || ms < best) {
+                best = ms;
+                best_ly = v;
+            }
+        }
+        for (Group* g : act)
+            if (!g->tuned) {
+                g->ly.assign(g->members.size(), best_ly);
+                g->written = false;
+                if (int r = group_sync(c, *g)) return r;
+            }
+    }
     for (Group* g : act) {
         if (g->tuned) continue;   // a family whose members did not change keeps its shape
         auto apply = [&](const Cand& cd) -> int {
@@ -881,6 +917,8 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         for (int j = 0; j < 2; ++j)
             if (int r = time_of(tried[j], 2 * T)) return r;
         std::sort(tried.begin(), tried.begin() + 2, by_time);
+        // the starting segment length stays unless another is faster by more than 1 % (run-to-run noise)
+        if (tried[0].ly != start && tried[1].ly == start && tried[1].ms <= 1.01f * tried[0].ms) std::swap(tried[0], tried[1]);
         if (int r = apply(tried[0])) return r;
         g->tuned = true;
         if (std::getenv("ITOLBM_TUNE_LOG"))
