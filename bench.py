@@ -8,6 +8,7 @@ F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (one laun
 family per step).  A bench step advances every case by one time step.  Inputs are synthetic
 (generated by the library's initial condition), resident in HBM before timing.
 
+Before anything is timed, 2 s of untimed streaming copies bring the GPU to its sustained clocks.
 Every form first runs --tune-steps untimed steps (default 500, whatever --warmup is): the library
 tunes its tile shapes on the first ~340-400 of them, and the shapes it chose are reported.  Then
 --warmup untimed steps, then --steps timed steps.  The headline (--prec, default f64) is the FP64
@@ -26,8 +27,9 @@ Also reported:
   roofline       the dominant kernel (the P1/P3 family launch, 8 of the 14 cases) timed alone with HIP
                  events on its stream, at the tile shape the library tunes for it alone (after
                  --tune-steps more untimed steps): algorithmic bytes per launch (the form's stored
-                 bytes) / average launch duration; traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
-                 runs (profiles/traffic_<form>.json), when present; stream_GBs = the library's own
+                 bytes) / average launch duration; traffic = HBM bytes per launch from rocprofv3
+                 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_<form>.json), when present;
+                 stream_GBs = the library's own
                  streaming kernel (read + write of 1 GiB, 16-byte loads, nontemporal stores) timed the
                  same way, the bandwidth a plain streaming kernel reaches on this GPU
   large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in FP32 (all 19
@@ -238,6 +240,23 @@ def stream_ceiling(m, timer, stream, reps=10):
     return {"bytes_per_copy": 2 * n, "GBs": 2 * n * reps / gpu_s / 1e9, "copy_correct": ok}
 
 
+def device_warmup(m, stream, seconds=2.0):
+    """Untimed streaming copies for `seconds` before anything is timed.  The GPU raises its clocks
+    over the first ~0.1-0.2 s of sustained load: eight 512^2 P-family cases timed after 20 steps of
+    a fresh process ran at 113 us per launch, the same launches after 2 s of load at 97.5 us."""
+    import time
+    import torch
+    n = 1 << 30
+    x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    t0 = time.time()
+    while time.time() - t0 < seconds:
+        for _ in range(10):
+            m.stream_copy(y.data_ptr(), x.data_ptr(), n, stream.cuda_stream)
+        stream.synchronize()
+    del x, y
+
+
 def sweep_cases(m):
     import numpy as np
     rng = np.random.default_rng(2026)
@@ -400,6 +419,7 @@ def main():
     if a.copy_only:
         print(json.dumps({"copy": stream_ceiling(m, timer, stream, reps=3)}), flush=True)
         return
+    device_warmup(m, stream)
     if a.family:
         print(json.dumps(bench_family(m, timer, stream, a.family, a.prec, a.steps, a.tune_steps)), flush=True)
         return

@@ -888,10 +888,15 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
             g->written = false;
             return group_sync(c, *g);
         };
+        const char* log = std::getenv("ITOLBM_TUNE_LOG");
         auto time_of = [&](Cand& cd, int t) -> int {
             if (int r = apply(cd)) return r;
             if (int r = run(W, nullptr)) return r;
-            return run(t, &cd.ms);
+            const int r = run(t, &cd.ms);
+            if (log && std::atoi(log) >= 2)
+                std::fprintf(stderr, "itolbm step tuner: family %d trial %d rows, depth %d: %.1f us over %d steps\n",
+                             g->kind, cd.ly, cd.depth, 1000.0 * cd.ms, t);
+            return r;
         };
         auto by_time = [](const Cand& x, const Cand& y) { return x.ms < y.ms; };
         // starting rows per segment: the current one, rounded to a candidate
@@ -921,7 +926,7 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
         if (tried[0].ly != start && tried[1].ly == start && tried[1].ms <= 1.01f * tried[0].ms) std::swap(tried[0], tried[1]);
         if (int r = apply(tried[0])) return r;
         g->tuned = true;
-        if (std::getenv("ITOLBM_TUNE_LOG"))
+        if (log)
             std::fprintf(stderr, "itolbm step tuner: family %d, %zu cases, element %d B: %d rows per segment, depth %d, step %.1f us\n",
                          g->kind, g->members.size(), c->cases[g->members[0]].elem, tried[0].ly, tried[0].depth,
                          1000.0 * tried[0].ms);
