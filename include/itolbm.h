@@ -207,6 +207,43 @@ int lbm_download_aux(lbm_ctx* ctx, int case_id, double* gm, double* F);
 int lbm_diag_drop(lbm_ctx* ctx, int case_id, double v0, int step, double x_ini, double y_ini,
                   lbm_drop_diag* out);
 
+/* Geometry file (meio.vtk), as read_geo reads it (LBM_functions.cpp:280-354): the box size from the
+ * DIMENSIONS line (the 5th), 5 more header lines, then nx * ny * nz values in site order
+ * (x + nx * (y + ny * z)), 0 = solid; pts_in / pts_out columns are added before / after the file's
+ * columns, fluid except the outermost column on each side.  nx / ny / nz (may be NULL) receive the box
+ * size (with the added columns); meio (cap >= nx * ny * nz entries) receives read_geo's numbering: 0 for
+ * a solid site, 1, 2, ... for the fluid sites in site order.  meio NULL only queries the size.  Returns
+ * the number of fluid sites, or LBM_E_ARG (unreadable or malformed file, cap too small).  Host only. */
+long long lbm_read_geometry(const char* path, int pts_in, int pts_out, int* nx, int* ny, int* nz, int* meio,
+                            long long cap);
+
+/* Solid sites of a case and the wall's wettability, for the FP64 form: meio holds nx * ny entries as
+ * lbm_read_geometry gives them (0 = solid, > 0 = fluid); NULL (or no 0 entry) makes the box all fluid
+ * again.  On the next steps, a population streaming into a solid site comes back in the opposite
+ * direction at its own site (def_dir_prop LBM_functions.cpp:509-853, propag_site :1183-1253), a mediator
+ * that would come from a solid site carries the wall's phase wett_R (propag_site_med :1309-1337), and
+ * the force operators read a solid neighbour's field at the site itself (aloca_campos_F,
+ * Operadores_tensao.cpp:149-182).  The reference's drop and band programs require an all-fluid box
+ * (Imm_OPS.cpp:123-127); its library supports solids, and so does this step.  The post-stream state
+ * is kept across the call; solid sites hold no populations (0 in downloads).  Such a case steps with
+ * site-parallel kernels (lbm_step, lbm_step_case), not in its family's launch; lbm_diag_drop /
+ * lbm_diag_incl refuse it (LBM_E_STATE), lbm_step_case_noalloc too (LBM_E_ARG), and other forms than
+ * LBM_PREC_F64 (LBM_E_ARG). */
+int lbm_set_geometry(lbm_ctx* ctx, int case_id, const int* meio, double wett_R);
+
+/* VTK field files of the final state: rec_density / rec_velocity (LBM_functions.cpp:1347-1510) write
+ * legacy structured-points files of one colour's density (field LBM_VTK_DENSITY, populations fR) or the
+ * colour-weighted velocity with Guo's half-force correction (LBM_VTK_VELOCITY: fR, fB and the force
+ * field [site][3], which may be NULL), 0 at solid sites (meio as above, may be NULL), ASCII with the
+ * stream's default 6 digits.  flags LBM_VTK_BINARY writes the binary form the reference describes
+ * (LBM_functions.cpp:1515-1570: same header with BINARY, raw big-endian values, float unless
+ * LBM_VTK_DOUBLE).  Host arrays [site][19]; no context needed.  LBM_E_ARG for bad arguments or a file
+ * that cannot be written. */
+enum { LBM_VTK_DENSITY = 0, LBM_VTK_VELOCITY = 1 };
+enum { LBM_VTK_BINARY = 1, LBM_VTK_DOUBLE = 2 };
+int lbm_write_vtk(const char* path, int field, int nx, int ny, const double* fR, const double* fB,
+                  const double* force, const int* meio, int flags);
+
 /* Inclined-interface diagnostics (mede_inclinada) for normal (p, q); needs LBM_STEP_RECORD_AUX.
  * profile (optional, NULL to skip) receives nx*ny rows of (d, c_R, rho, u_x, u_y) in site order,
  * the values perfil.dat is written from (Imm_INCL.cpp:101,162). */

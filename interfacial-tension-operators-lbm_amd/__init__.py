@@ -24,12 +24,17 @@ PREC_F16_ZSYM = 4
 STATE_POST_STREAM = 0
 STATE_POST_COLLISION = 1
 STEP_RECORD_AUX = 1
+VTK_DENSITY = 0
+VTK_VELOCITY = 1
+VTK_BINARY = 1
+VTK_DOUBLE = 2
 
 # every symbol include/itolbm.h declares
 SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "lbm_upload_state",
            "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case",
            "lbm_step_case_noalloc", "lbm_tile_shape", "lbm_stream_copy", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
-           "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl"]
+           "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl", "lbm_read_geometry", "lbm_set_geometry",
+           "lbm_write_vtk"]
 
 
 class CaseDesc(C.Structure):
@@ -100,8 +105,13 @@ def lib():
         L.lbm_download_aux.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
         L.lbm_diag_drop.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(DropDiag)]
         L.lbm_diag_incl.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(InclDiag), C.c_void_p]
+        L.lbm_read_geometry.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, ip, ip, C.c_void_p, C.c_longlong]
+        L.lbm_read_geometry.restype = C.c_longlong
+        L.lbm_set_geometry.argtypes = [P, C.c_int, C.c_void_p, C.c_double]
+        L.lbm_write_vtk.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int]
         for s in SYMBOLS:
-            if s not in ("lbm_last_error", "lbm_device_bytes"):
+            if s not in ("lbm_last_error", "lbm_device_bytes", "lbm_read_geometry"):
                 getattr(L, s).restype = C.c_int
         _lib = L
     return _lib
@@ -135,6 +145,33 @@ def stream_copy(dst_ptr, src_ptr, nbytes, hip_stream=None):
                                None if hip_stream is None else C.c_void_p(int(hip_stream)))
     if rc != 0:
         raise LBMError(f"lbm_stream_copy failed ({rc})")
+
+
+def read_geometry(path, pts_in=0, pts_out=0):
+    """lbm_read_geometry (read_geo): returns (nx, ny, nz, meio), meio an int32 array of nx*ny*nz entries,
+    0 for a solid site and 1, 2, ... for the fluid sites in site order."""
+    nx, ny, nz = C.c_int(), C.c_int(), C.c_int()
+    n = lib().lbm_read_geometry(os.fsencode(path), pts_in, pts_out, C.byref(nx), C.byref(ny), C.byref(nz), None, 0)
+    if n < 0:
+        raise LBMError(f"lbm_read_geometry({path!r}) failed ({n})")
+    meio = np.zeros(nx.value * ny.value * nz.value, dtype=np.int32)
+    n = lib().lbm_read_geometry(os.fsencode(path), pts_in, pts_out, None, None, None,
+                                meio.ctypes.data_as(C.c_void_p), meio.size)
+    if n < 0:
+        raise LBMError(f"lbm_read_geometry({path!r}) failed ({n})")
+    return nx.value, ny.value, nz.value, meio
+
+
+def write_vtk(path, field, nx, ny, fR, fB=None, force=None, meio=None, binary=False, double=False):
+    """lbm_write_vtk: rho (field VTK_DENSITY, of fR) or velocity (VTK_VELOCITY) file of host populations."""
+    fR = np.ascontiguousarray(fR, dtype=np.float64).ravel()
+    keep = [np.ascontiguousarray(a, dtype=dt).ravel() if a is not None else None
+            for a, dt in ((fB, np.float64), (force, np.float64), (meio, np.int32))]
+    ptr = [None if a is None else a.ctypes.data_as(C.c_void_p) for a in keep]
+    flags = (VTK_BINARY if binary else 0) | (VTK_DOUBLE if double else 0)
+    rc = lib().lbm_write_vtk(os.fsencode(path), field, nx, ny, fR, ptr[0], ptr[1], ptr[2], flags)
+    if rc != 0:
+        raise LBMError(f"lbm_write_vtk({path!r}) failed ({rc})")
 
 
 class Context:
@@ -227,6 +264,16 @@ class Context:
     def set_active(self, active, case=0):
         """Includes or leaves out a case from step() over all cases."""
         self._chk(lib().lbm_set_active(self.h, case, 1 if active else 0))
+
+    def set_geometry(self, meio, wett=0.0, case=0):
+        """lbm_set_geometry: meio (nx*ny, 0 = solid) as read_geometry gives it, or None for all fluid."""
+        if meio is None:
+            self._chk(lib().lbm_set_geometry(self.h, case, None, float(wett)))
+            return
+        m = np.ascontiguousarray(meio, dtype=np.int32).ravel()
+        if m.size != self.n_sites(case):
+            raise ValueError(f"meio has {m.size} entries, the case {self.n_sites(case)} sites")
+        self._chk(lib().lbm_set_geometry(self.h, case, m.ctypes.data_as(C.c_void_p), float(wett)))
 
     def set_reference(self, rho_red, rho_blue, case=0):
         """Reference phase densities of the FP32/FP16 deviation storage (no effect on FP64)."""

@@ -10,7 +10,9 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <new>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -52,6 +54,12 @@ struct CaseState {
     double* prof = nullptr;
     int n_casca = 0;
     int nblk = 1;
+    // solid sites (lbm_set_geometry): host and device masks (1 = solid), the wall's wettability and
+    // the site-parallel kernels' field arrays phi [S], g [3][S], nt [6][S]
+    std::vector<unsigned char> solid_h;
+    unsigned char* solid = nullptr;
+    double wett = 0.0;
+    double* geo_f = nullptr;
 };
 
 // A set of cases advanced by one launch per step (cases of one kernel family).
@@ -218,6 +226,8 @@ int group_rows(const std::vector<CaseState>& cs, const std::vector<int>& members
 
 const int CXh[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
 const int CYh[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1};
+const int CZh[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1};
+const int OPPh[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};   // c_opp = -c_i
 
 inline long long nbr(int x, int y, int dx, int dy, int nx, int ny)
 {
@@ -487,9 +497,11 @@ int lbm_destroy(lbm_ctx* c)
     if (c->stream && capturing(c)) return refuse_capture(c, "lbm_destroy");
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (CaseState& k : c->cases) {
-        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.ordered, k.partial, k.hist, k.tab, k.prof, k.f64})
+        for (double* p : {k.pop[0], k.pop[1], k.aux, k.gal, k.dsum, k.ordered, k.partial, k.hist, k.tab, k.prof, k.f64,
+                          k.geo_f})
             if (p) (void)hipFree(p);
         if (k.histn) (void)hipFree(k.histn);
+        if (k.solid) (void)hipFree(k.solid);
     }
     auto free_group = [](Group& g) {
         if (g.dtab) (void)hipFree(g.dtab);
@@ -518,15 +530,24 @@ int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
     if (!fR || !fB) return fail(c, LBM_E_ARG, "null population array");
     CaseState& k = c->cases[id];
     const int nx = k.d.nx, ny = k.d.ny;
-    // device holds "post-collision at own site": f*_i(x) = f_i(x + c_i) for a post-stream state f
+    // device holds "post-collision at own site": f*_i(x) = f_i(x + c_i) for a post-stream state f;
+    // with solid sites, f*_i(x) = f_opp(i)(x) where x + c_i is solid (it bounced back), and solid sites
+    // hold nothing (0)
     std::vector<double> h(2 * Q * (size_t)k.S, 0.0);
+    const unsigned char* solid = k.solid_h.empty() ? nullptr : k.solid_h.data();
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             const long long site = (long long)x + (long long)nx * y;
+            if (solid && solid[site]) continue;
             for (int i = 0; i < Q; ++i) {
-                const long long src = nbr(x, y, CXh[i], CYh[i], nx, ny);
-                h[(size_t)i * k.S + site] = fR[(size_t)src * Q + i];
-                h[(size_t)(Q + i) * k.S + site] = fB[(size_t)src * Q + i];
+                long long src = nbr(x, y, CXh[i], CYh[i], nx, ny);
+                int j = i;
+                if (solid && solid[src]) {
+                    src = site;
+                    j = OPPh[i];
+                }
+                h[(size_t)i * k.S + site] = fR[(size_t)src * Q + j];
+                h[(size_t)(Q + i) * k.S + site] = fB[(size_t)src * Q + j];
             }
         }
     if (k.elem == 8) {
@@ -563,6 +584,28 @@ void host_feq(double* feq, double vx, double vy, double vz, double rho)
         for (int a = 0; a < 3; a++) cv = cv + Ch[i][a] * v[a];
         const double w = (i == 0) ? W0 : (i < 7 ? W1 : W2);
         feq[i] = w * rho * (1.0 + cv * o + 0.5 * cv * cv * o * o - 0.5 * vquad * o);
+    }
+}
+
+// calcula (LBM_functions.cpp:1099-1171) of one colour's populations: density and velocity, summed in
+// index order (rec_velocity's per-colour moments, :1386-1404)
+void host_moments(const double* f, double v[3], double& rho)
+{
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    rho = 0.0;
+    for (int i = 0; i < Q; ++i) {
+        rho = rho + f[i];
+        mx = mx + CXh[i] * f[i];
+        my = my + CYh[i] * f[i];
+        mz = mz + CZh[i] * f[i];
+    }
+    if (rho != 0.0) {
+        const double one_over_rho = 1.0 / rho;
+        v[0] = mx * one_over_rho;
+        v[1] = my * one_over_rho;
+        v[2] = mz * one_over_rho;
+    } else {
+        v[0] = v[1] = v[2] = 0.0;
     }
 }
 
@@ -653,13 +696,25 @@ int lbm_download_state(lbm_ctx* c, int id, int which, double* fR, double* fB)
     if (int r = state_f64(c, k, &src64)) return r;
     HIPCHK(c, hipMemcpyAsync(h.data(), src64, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // post-stream: f_i(x) = f*_i(x - c_i), or f*_opp(i)(x) where x - c_i is solid (bounce-back); a
+    // solid site's populations are 0
+    const unsigned char* solid = k.solid_h.empty() ? nullptr : k.solid_h.data();
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             const long long site = (long long)x + (long long)nx * y;
+            if (solid && solid[site]) {
+                for (int i = 0; i < Q; ++i) fR[(size_t)site * Q + i] = fB[(size_t)site * Q + i] = 0.0;
+                continue;
+            }
             for (int i = 0; i < Q; ++i) {
-                const long long src = (which == LBM_STATE_POST_STREAM) ? nbr(x, y, -CXh[i], -CYh[i], nx, ny) : site;
-                fR[(size_t)site * Q + i] = h[(size_t)i * k.S + src];
-                fB[(size_t)site * Q + i] = h[(size_t)(Q + i) * k.S + src];
+                long long src = (which == LBM_STATE_POST_STREAM) ? nbr(x, y, -CXh[i], -CYh[i], nx, ny) : site;
+                int j = i;
+                if (solid && which == LBM_STATE_POST_STREAM && solid[src]) {
+                    src = site;
+                    j = OPPh[i];
+                }
+                fR[(size_t)site * Q + i] = h[(size_t)j * k.S + src];
+                fB[(size_t)site * Q + i] = h[(size_t)(Q + j) * k.S + src];
             }
         }
     return LBM_OK;
@@ -680,6 +735,21 @@ int group_sync(lbm_ctx* c, Group& g)
     if (!stale) return LBM_OK;
     if (capturing(c)) return refuse_capture(c, "writing a family's case table (first step after a change of cases, shape or state)");
     const int nm = (int)g.members.size();
+    if (g.kind == KIND_GEO) {
+        // no table: each step's launches take the members' buffers from the host bookkeeping; the
+        // version still changes, so a captured graph of the old buffers is not replayed
+        g.cur0.assign(nm, 0);
+        g.aux0.assign(nm, nullptr);
+        for (int j = 0; j < nm; ++j) {
+            g.cur0[j] = c->cases[g.members[j]].cur;
+            g.aux0[j] = c->cases[g.members[j]].aux;
+        }
+        g.ntiles = 0;
+        g.k = 0;
+        g.written = true;
+        g.version++;
+        return LBM_OK;
+    }
     if (g.ly.empty()) {
         const int ly = group_rows(c->cases, g.members);
         g.ly.assign(nm, ly);
@@ -737,6 +807,24 @@ int group_sync(lbm_ctx* c, Group& g)
 
 int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
 {
+    if (g.kind == KIND_GEO) {   // cases with solid sites: the site-parallel kernels, case by case
+        for (int id : g.members) {
+            const CaseState& k = c->cases[id];
+            GeoLaunch a{};
+            a.cc = k.cc;
+            a.in = k.pop[k.cur];
+            a.out = k.pop[k.cur ^ 1];
+            a.solid = k.solid;
+            a.wett = k.wett;
+            a.phi = k.geo_f;
+            a.g = k.geo_f + k.S;
+            a.nt = k.geo_f + 4 * k.S;
+            a.aux = record ? k.aux : nullptr;
+            a.S = k.S;
+            HIPCHK(c, launch_step_geo(step_kind(k.d.op), a, st));
+        }
+        return LBM_OK;
+    }
     StepLaunch a{};
     a.tab = g.dtab;
     a.tile_case = g.dtile;
@@ -768,6 +856,7 @@ int group_prepare(lbm_ctx* c, Group& g, bool rec)
 // default) fix the shape instead.
 bool needs_tuning(lbm_ctx* c, Group& g)
 {
+    if (g.kind == KIND_GEO) g.tuned = true;   // site-parallel kernels: no tile shape
     if (g.tuned || !c->autotune) return false;
     if (std::getenv("ITOLBM_LY")) {
         if (const char* e = std::getenv("ITOLBM_DEPTH")) g.depth = std::max(0, std::min(3, std::atoi(e)));
@@ -1030,6 +1119,7 @@ int lbm_step_case_noalloc(lbm_ctx* c, int id, int n_steps, int flags)
     if ((n_steps & 1) && capturing(c))
         return fail(c, LBM_E_ARG, "a captured step call must advance an even number of steps");
     CaseState& k = c->cases[id];
+    if (k.solid) return fail(c, LBM_E_ARG, "a case with solid sites steps with lbm_step / lbm_step_case only");
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     if (rec && !k.aux)
         return fail(c, LBM_E_STATE, "LBM_STEP_RECORD_AUX needs the aux fields, which a first lbm_step or lbm_step_case "
@@ -1176,6 +1266,197 @@ int lbm_set_active(lbm_ctx* c, int id, int active)
     return LBM_OK;
 }
 
+int lbm_set_geometry(lbm_ctx* c, int id, const int* meio, double wett_R)
+{
+    if (int r = check_case(c, id)) return r;
+    if (capturing(c)) return refuse_capture(c, "lbm_set_geometry");
+    CaseState& k = c->cases[id];
+    if (k.elem != 8) return fail(c, LBM_E_ARG, "solid sites need the FP64 form (LBM_PREC_F64)");
+    if (!std::isfinite(wett_R)) return fail(c, LBM_E_ARG, "non-finite wettability");
+    std::vector<unsigned char> m;
+    long long nsolid = 0;
+    if (meio) {
+        m.assign((size_t)k.n, 0);
+        for (long long p = 0; p < k.n; ++p) {
+            if (meio[p] < 0) return fail(c, LBM_E_ARG, "negative entry in the geometry (0 = solid, > 0 = fluid)");
+            m[(size_t)p] = meio[p] == 0;
+            nsolid += m[(size_t)p];
+        }
+        if (nsolid == k.n) return fail(c, LBM_E_ARG, "the geometry has no fluid site");
+    }
+    // the post-stream state is kept across the change: downloaded with the old geometry, uploaded
+    // with the new one (at a site that turns solid it is dropped; one that turns fluid starts at 0)
+    std::vector<double> fR((size_t)k.n * Q), fB((size_t)k.n * Q);
+    if (int r = lbm_download_state(c, id, LBM_STATE_POST_STREAM, fR.data(), fB.data())) return r;
+    const int kind = nsolid ? KIND_GEO : step_kind(k.d.op);
+    if (nsolid) {
+        if (!k.solid && dev_alloc(c, (void**)&k.solid, (size_t)k.n)) return LBM_E_NOMEM;
+        if (!k.geo_f && dev_alloc(c, (void**)&k.geo_f, sizeof(double) * 10 * (size_t)k.S)) return LBM_E_NOMEM;
+        HIPCHK(c, hipMemcpyAsync(k.solid, m.data(), (size_t)k.n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        k.solid_h = std::move(m);
+    } else {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (k.solid) {
+            (void)hipFree(k.solid);
+            c->bytes -= k.n;
+            k.solid = nullptr;
+        }
+        if (k.geo_f) {
+            (void)hipFree(k.geo_f);
+            c->bytes -= (long long)sizeof(double) * 10 * k.S;
+            k.geo_f = nullptr;
+        }
+        k.solid_h.clear();
+    }
+    k.wett = wett_R;
+    if (kind != k.kind) {
+        // the case changes kernel family: both families' members and tables are redone
+        const int old = k.kind;
+        k.kind = kind;
+        for (int kd : {old, kind}) {
+            Group& g = c->family[kd];
+            g.members.clear();
+            for (int j = 0; j < (int)c->cases.size(); ++j)
+                if (c->cases[j].active && c->cases[j].kind == kd) g.members.push_back(j);
+            g.ly.clear();
+            g.written = false;
+            g.tuned = false;
+        }
+        Group& one = c->single[id];
+        one.kind = kind;
+        one.ly.clear();
+        one.written = false;
+        one.tuned = false;
+    }
+    drop_graph(c);   // a captured graph holds the old wettability and family launches
+    return lbm_upload_state(c, id, fR.data(), fB.data());
+}
+
+long long lbm_read_geometry(const char* path, int pts_in, int pts_out, int* nx, int* ny, int* nz, int* meio,
+                            long long cap)
+{
+    if (!path || pts_in < 0 || pts_out < 0) return LBM_E_ARG;
+    std::ifstream in(path);
+    if (!in) return LBM_E_ARG;
+    std::string line;
+    for (int i = 0; i < 4; ++i)
+        if (!std::getline(in, line)) return LBM_E_ARG;
+    if (!std::getline(in, line)) return LBM_E_ARG;
+    std::istringstream dims(line);
+    std::string word;
+    long long X = 0, Y = 0, Z = 0;
+    if (!(dims >> word >> X >> Y >> Z) || X < 1 || Y < 1 || Z < 1) return LBM_E_ARG;
+    for (int i = 0; i < 5; ++i)
+        if (!std::getline(in, line)) return LBM_E_ARG;
+    const long long NX = X + pts_in + pts_out;
+    if (NX * Y * Z > (1LL << 31)) return LBM_E_ARG;
+    if (nx) *nx = (int)NX;
+    if (ny) *ny = (int)Y;
+    if (nz) *nz = (int)Z;
+    const long long total = NX * Y * Z;
+    if (!meio) {   // size query: the number of fluid points still needs the whole file
+        std::vector<int> tmp((size_t)total);
+        return lbm_read_geometry(path, pts_in, pts_out, nullptr, nullptr, nullptr, tmp.data(), total);
+    }
+    if (cap < total) return LBM_E_ARG;
+    long long fluid = 0;
+    for (long long z = 0; z < Z; ++z)
+        for (long long y = 0; y < Y; ++y)
+            for (long long x = 0; x < NX; ++x) {
+                const long long pos = x + y * NX + z * NX * Y;
+                int v = 0;
+                if (x >= pts_in && x < NX - pts_out) {
+                    double d = 0.0;   // the reference reads an int; files written as floats ("1.0") read the same
+                    if (!(in >> d)) return LBM_E_ARG;
+                    v = d != 0.0;
+                } else {
+                    v = (x != 0 && x != NX - 1);   // added layers: fluid except the outermost columns
+                }
+                meio[pos] = v ? (int)++fluid : 0;
+            }
+    return fluid;
+}
+
+int lbm_write_vtk(const char* path, int field, int nx, int ny, const double* fR, const double* fB, const double* force,
+                  const int* meio, int flags)
+{
+    if (!path || nx < 1 || ny < 1 || !fR || (field == LBM_VTK_VELOCITY && !fB)) return LBM_E_ARG;
+    if (field != LBM_VTK_DENSITY && field != LBM_VTK_VELOCITY) return LBM_E_ARG;
+    if (flags & ~(LBM_VTK_BINARY | LBM_VTK_DOUBLE)) return LBM_E_ARG;
+    const bool binary = (flags & LBM_VTK_BINARY) != 0, dbl = (flags & LBM_VTK_DOUBLE) != 0;
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return LBM_E_ARG;
+    const long long n = (long long)nx * ny;
+    const bool vel = field == LBM_VTK_VELOCITY;
+    out << "# vtk DataFile Version 2.0\n" << (vel ? "Velocidade" : "Densidade") << '\n'
+        << (binary ? "BINARY" : "ASCII") << "\nDATASET STRUCTURED_POINTS\n";
+    out << "DIMENSIONS " << nx << ' ' << ny << ' ' << 1 << "\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\n";
+    out << "POINT_DATA " << n << '\n';
+    const char* type = (binary && !dbl) ? "float" : "double";
+    if (vel) out << "VECTORS velocidade " << type << '\n';
+    else out << "SCALARS densidade " << type << "\nLOOKUP_TABLE default\n";
+    // the value(s) of site s: density of fR (rec_density), or the colour-weighted velocity with Guo's
+    // half-force correction (rec_velocity); 0 at a solid site
+    auto site_values = [&](long long s, double v[3]) {
+        if (meio && meio[s] == 0) {
+            v[0] = v[1] = v[2] = 0.0;
+            return;
+        }
+        if (!vel) {
+            double rho = 0.0;
+            for (int i = 0; i < Q; ++i) rho += fR[s * Q + i];
+            v[0] = rho;
+            return;
+        }
+        double vR[3], vB[3], rR, rB;
+        host_moments(fR + s * Q, vR, rR);
+        host_moments(fB + s * Q, vB, rB);
+        const double cR = rR / (rR + rB), cB = 1. - cR;
+        for (int a = 0; a < 3; ++a) v[a] = cR * vR[a] + cB * vB[a];
+        if (force) {
+            const double rho = rR + rB;
+            if (rho > 0.0)
+                for (int a = 0; a < 3; ++a) v[a] = v[a] - 0.5 * force[3 * s + a] / rho;
+        }
+    };
+    const int nv = vel ? 3 : 1;
+    if (!binary) {
+        for (int y = 0; y < ny; ++y) {
+            for (int x = 0; x < nx; ++x) {
+                double v[3];
+                site_values((long long)y * nx + x, v);
+                for (int a = 0; a < nv; ++a) out << v[a] << ' ';
+            }
+            out << '\n';
+        }
+    } else {
+        // legacy binary VTK: raw big-endian values after the header line, then a newline
+        // (format rules, LBM_functions.cpp:1515-1570)
+        std::vector<char> buf;
+        buf.reserve((size_t)n * nv * (dbl ? 8 : 4));
+        auto push = [&](const void* p, int bytes) {
+            const unsigned char* b = (const unsigned char*)p;
+            for (int i = bytes - 1; i >= 0; --i) buf.push_back((char)b[i]);   // the host is little-endian
+        };
+        for (long long s = 0; s < n; ++s) {
+            double v[3];
+            site_values(s, v);
+            for (int a = 0; a < nv; ++a) {
+                if (dbl) {
+                    push(&v[a], 8);
+                } else {
+                    const float f = (float)v[a];
+                    push(&f, 4);
+                }
+            }
+        }
+        out.write(buf.data(), (std::streamsize)buf.size());
+        out << '\n';
+    }
+    return out.good() ? LBM_OK : LBM_E_ARG;
+}
+
 int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
 {
     if (int r = check_case(c, id)) return r;
@@ -1217,6 +1498,7 @@ int lbm_diag_drop(lbm_ctx* c, int id, double v0, int step, double x_ini, double 
     if (capturing(c)) return refuse_capture(c, "lbm_diag_drop");
     if (!o) return fail(c, LBM_E_ARG, "null output");
     CaseState& k = c->cases[id];
+    if (k.solid) return fail(c, LBM_E_STATE, "the drop diagnostics measure an all-fluid box (Imm_OPS.cpp:123-127)");
     const bool forced = k.d.op >= OP_F1;
     if (forced && !k.aux_valid) return fail(c, LBM_E_STATE, "force operators need a step with LBM_STEP_RECORD_AUX");
     if (int r = ensure_diag(c, k)) return r;
@@ -1427,6 +1709,7 @@ int lbm_diag_incl(lbm_ctx* c, int id, int p, int q, lbm_incl_diag* o, double* pr
     if (!o) return fail(c, LBM_E_ARG, "null output");
     if (p == 0 && q == 0) return fail(c, LBM_E_ARG, "p = q = 0");
     CaseState& k = c->cases[id];
+    if (k.solid) return fail(c, LBM_E_STATE, "the band diagnostics measure an all-fluid box (Imm_INCL.cpp:241-246)");
     if (!k.aux_valid) return fail(c, LBM_E_STATE, "needs a step with LBM_STEP_RECORD_AUX");
     if (int r = ensure_diag(c, k)) return r;
     if (profile && !k.prof) {

@@ -7,8 +7,8 @@
 // (LBM_functions.cpp:1099-1171, 889-945, 1602-1673, 1712-1746), perturbacao / coef_B
 // (Operadores_tensao.cpp:94-133) and calcula_forca (Operadores_tensao.cpp:202-332).
 //
-// Layout: populations of each colour are structure-of-arrays, f[i * S + x + nx * y], and hold the
-// POST-COLLISION values of the previous step at their own site.  The reference's push
+// Layout: row blocks [y][colour][19][nx] (below), holding the POST-COLLISION values of the previous
+// step at their own site.  The reference's push
 // (f_new[x + c_i] = f*(x)) is done here as a pull: site x reads f*_i(x - c_i).
 //
 // Work decomposition: one 64-lane wavefront owns a vertical strip of 64 consecutive columns, of
@@ -731,9 +731,232 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
     }
 }
 
+// ------------------------------------------------------------------ boxes with solid sites
+// A case with solid sites (the reference library's general geometry: read_geo's 0 sites) steps with
+// three site-parallel kernels instead of a strip kernel; one thread per site, x fastest, so every
+// load of a velocity row is coalesced.  The fields the strip kernels keep in registers and LDS
+// (colour field, its gradient, n / T / mu) go through per-case arrays of the padded stride S.
+//   geo_phase   colour field phi of every fluid site from its pulled populations;
+//   geo_fields  (F class) the isotropic gradient of phi and n (F1), T (F2) or mu (F3);
+//   geo_collide pull, mediator gradient, force, collision and recolouring; the site's post-collision
+//               populations are stored at the site, as in the strip kernels.
+// Bounce-back (def_dir_prop, LBM_functions.cpp:509-853: a population pushed towards a solid site
+// comes back in the opposite direction at its own site) becomes, in the pull: f_i(x) = f*_opp(i)(x)
+// when x - c_i is solid.  A mediator that would come from a solid site is the wall's, w_i wett 3
+// (propag_site_med :1331).  The force stencils read the site itself where a neighbour is solid
+// (aloca_campos_F, Operadores_tensao.cpp:170-178).  Every value is formed with the same expressions
+// as in the strip kernels, so an all-fluid neighbourhood gives their bits.
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ __forceinline__ void geo_pull(const GeoLaunch& a, int x, int y, double fR[Q], double fB[Q])
+{
+    const int nx = a.cc.nx, ny = a.cc.ny;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int sx = wrap1(x - CX[i], nx), sy = wrap1(y - CY[i], ny);
+        const bool back = (i != 0) && a.solid[(long long)sy * nx + sx];
+        const int yy = back ? y : sy, xx = back ? x : sx, ii = back ? OPP[i] : i;
+        fR[i] = a.in[elem_index(nx, yy, 0, ii, (unsigned)xx)];
+        fB[i] = a.in[elem_index(nx, yy, 1, ii, (unsigned)xx)];
+    }
+}
+
+// 3 x 3 values of a site field around (x, y); a solid neighbour reads the site's own value
+__device__ __forceinline__ Nine geo_nine(const GeoLaunch& a, const double* __restrict__ f, int x, int y)
+{
+    const int nx = a.cc.nx, ny = a.cc.ny;
+    const long long p = (long long)y * nx + x;
+    Nine n;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const long long q = (long long)wrap1(y + dy, ny) * nx + wrap1(x + dx, nx);
+            n.v[dy + 1][dx + 1] = a.solid[q] ? f[p] : f[q];
+        }
+    return n;
+}
+
+__global__ void __launch_bounds__(256) geo_phase(GeoLaunch a)
+{
+    const long long n = (long long)a.cc.nx * a.cc.ny;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (a.solid[p]) {
+        a.phi[p] = 0.0;
+        return;
+    }
+    const int y = (int)(p / a.cc.nx), x = (int)(p - (long long)y * a.cc.nx);
+    double fR[Q], fB[Q];
+    geo_pull(a, x, y, fR, fB);
+    a.phi[p] = phase(dens(fR), dens(fB));
+}
+
+// gradient of phi (calcula_forca :219-235) into g[3][S]; n (F1) / T (F2) into nt[q][S] (aux_of), or
+// mu (F3, :273-287) into nt[0][S]
+template <int OP>
+__global__ void __launch_bounds__(256) geo_fields(GeoLaunch a)
+{
+    const long long n = (long long)a.cc.nx * a.cc.ny;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || a.solid[p]) return;
+    const int y = (int)(p / a.cc.nx), x = (int)(p - (long long)y * a.cc.nx);
+    const CaseConst& cc = a.cc;
+    const Nine p9 = geo_nine(a, a.phi, x, y);
+    double g0, g1, g2;
+    phase_gradient(p9, g0, g1, g2);
+    a.g[p] = g0; a.g[a.S + p] = g1; a.g[2 * a.S + p] = g2;
+    if constexpr (OP == OP_F3) {
+        const double phc = p9.at(0, 0);
+        double lap = 0.0;
+#define ITOLBM_LAP(i) lap += (i < 7 ? W1 : W2) * (p9.at(CX[i], CY[i]) - phc);
+        ITOLBM_LAP(1) ITOLBM_LAP(2) ITOLBM_LAP(3) ITOLBM_LAP(4) ITOLBM_LAP(5) ITOLBM_LAP(6)
+        ITOLBM_LAP(7) ITOLBM_LAP(8) ITOLBM_LAP(9) ITOLBM_LAP(10) ITOLBM_LAP(11) ITOLBM_LAP(12)
+        ITOLBM_LAP(13) ITOLBM_LAP(14) ITOLBM_LAP(15) ITOLBM_LAP(16) ITOLBM_LAP(17) ITOLBM_LAP(18)
+#undef ITOLBM_LAP
+        lap *= 6.0;
+        a.nt[p] = 4.0 * cc.bf * phc * (phc * phc - 1.0) - cc.kf * lap;
+    } else {
+        const Aux6 r = aux_of<OP>(cc, g0, g1, g2);
+        constexpr int NA = (OP == OP_F1) ? 3 : 6;
+#pragma unroll
+        for (int q = 0; q < NA; ++q) a.nt[q * a.S + p] = r.a[q];
+    }
+}
+
+// mediator momentum (as mediator_gradient) with the wall's mediators where the source site is solid
+__device__ __forceinline__ void geo_mediator_gradient(const GeoLaunch& a, int x, int y, double& gx, double& gy,
+                                                      double& gz)
+{
+    const int nx = a.cc.nx, ny = a.cc.ny;
+    auto M = [&](double w, int dx, int dy) {
+        const long long q = (long long)wrap1(y + dy, ny) * nx + wrap1(x + dx, nx);
+        return a.solid[q] ? w * a.wett * 3.0 : w * (a.phi[q] * 3.0);
+    };
+    const double M1 = M(W1, -1, 0), M2 = M(W1, 1, 0), M3 = M(W1, 0, -1), M4 = M(W1, 0, 1);
+    const double M5 = M(W1, 0, 0);
+    const double M7 = M(W2, -1, -1), M8 = M(W2, 1, 1), M9 = M(W2, -1, 1), M10 = M(W2, 1, -1);
+    const double M11 = M(W2, -1, 0), M12 = M(W2, 1, 0), M15 = M(W2, 0, 1), M16 = M(W2, 0, -1);
+    gx = M1 - M2;
+    gx = gx + M7; gx = gx - M8; gx = gx + M9; gx = gx - M10;
+    gx = gx + M11; gx = gx - M12; gx = gx + M11; gx = gx - M12;
+    gy = M3 - M4;
+    gy = gy + M7; gy = gy - M8; gy = gy - M9; gy = gy + M10;
+    gy = gy - M15; gy = gy + M16; gy = gy - M15; gy = gy + M16;
+    gz = M5 - M5;
+    gz = gz + M11; gz = gz - M12; gz = gz - M11; gz = gz + M12;
+    gz = gz - M15; gz = gz + M16; gz = gz + M15; gz = gz - M16;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256) geo_collide(GeoLaunch a)
+{
+    const int nx = a.cc.nx;
+    const long long n = (long long)nx * a.cc.ny;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int y = (int)(p / nx), x = (int)(p - (long long)y * nx);
+    double* out = a.out;
+    auto put = [&](int i, double r, double b) {
+        out[elem_index(nx, y, 0, i, (unsigned)x)] = r;
+        out[elem_index(nx, y, 1, i, (unsigned)x)] = b;
+    };
+    if (a.solid[p]) {   // a solid site holds no populations
+#pragma unroll
+        for (int i = 0; i < Q; ++i) put(i, 0.0, 0.0);
+        if (a.aux)
+            for (int q = 0; q < 6; ++q) a.aux[q * a.S + p] = 0.0;
+        return;
+    }
+    const CaseConst& cc = a.cc;
+    double fR[Q], fB[Q];
+    geo_pull(a, x, y, fR, fB);
+    Site s;
+    site_sums(s, fR, fB, dens(fR), dens(fB));
+    double gx, gy, gz;
+    geo_mediator_gradient(a, x, y, gx, gy, gz);
+    const double mod2 = gx * gx + gy * gy + gz * gz;
+    const double mod = sqrt(mod2);
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    if constexpr (KIND == KIND_P) {
+        bgk<false>(cc, s, 0.0, 0.0, 0.0);
+        perturb(cc, s.f, gx, gy, gz, mod2, mod);
+    } else {
+        const double g0 = a.g[p], g1 = a.g[a.S + p], g2 = a.g[2 * a.S + p];
+        if constexpr (KIND == KIND_F3) {
+            const double mu = a.nt[p];
+            Fx = mu * g0; Fy = mu * g1; Fz = mu * g2;
+        } else {
+            // the neighbour (dx, dy) of component q of n / T, the site itself where it is solid
+            auto A = [&](int q, int dx, int dy) {
+                const long long r = (long long)wrap1(y + dy, a.cc.ny) * nx + wrap1(x + dx, nx);
+                return a.nt[q * a.S + (a.solid[r] ? p : r)];
+            };
+            if constexpr (KIND == KIND_F1) {
+                double d = 0.0;
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+                    const int dx = cx_of(i), dy = cy_of(i);
+                    d += (i < 7 ? W1 : W2) * cdot_rt(i, A(0, dx, dy), A(1, dx, dy), A(2, dx, dy));
+                }
+                const double kappa = -(3.0 * d);
+                Fx = cc.hs * kappa * g0; Fy = cc.hs * kappa * g1; Fz = cc.hs * kappa * g2;
+            } else {
+                double sx = 0.0, sy = 0.0, sz = 0.0;
+                auto chain = [&](double& acc, int qa, int qb, int qc) {
+#pragma unroll
+                    for (int i = 1; i < Q; ++i) {
+                        const int dx = cx_of(i), dy = cy_of(i);
+                        acc += (i < 7 ? W1 : W2) * cdot_rt(i, A(qa, dx, dy), A(qb, dx, dy), A(qc, dx, dy));
+                    }
+                };
+                chain(sx, 0, 3, 4);
+                chain(sy, 3, 1, 5);
+                chain(sz, 4, 5, 2);
+                Fx = 3.0 * sx; Fy = 3.0 * sy; Fz = 3.0 * sz;
+            }
+        }
+        bgk<true>(cc, s, Fx, Fy, Fz);
+    }
+    if (!s.skip) {
+        recolour(cc, s, gx, gy, gz, mod, put);
+    } else {   // left untouched (rare): the pulled populations
+#pragma unroll
+        for (int i = 0; i < Q; ++i) put(i, fR[i], fB[i]);
+    }
+    if (a.aux) {
+        a.aux[p] = gx; a.aux[a.S + p] = gy; a.aux[2 * a.S + p] = gz;
+        a.aux[3 * a.S + p] = Fx; a.aux[4 * a.S + p] = Fy; a.aux[5 * a.S + p] = Fz;
+    }
+}
+
 }  // namespace
 
 int step_halo(int op) { return (op == OP_F1 || op == OP_F2) ? 2 : 1; }
+
+hipError_t launch_step_geo(int kind, const GeoLaunch& a, hipStream_t stream)
+{
+    const long long n = (long long)a.cc.nx * a.cc.ny;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    hipLaunchKernelGGL(geo_phase, grid, block, 0, stream, a);
+    switch (kind) {
+    case KIND_P: break;
+    case KIND_F3: hipLaunchKernelGGL(geo_fields<OP_F3>, grid, block, 0, stream, a); break;
+    case KIND_F1: hipLaunchKernelGGL(geo_fields<OP_F1>, grid, block, 0, stream, a); break;
+    case KIND_F2: hipLaunchKernelGGL(geo_fields<OP_F2>, grid, block, 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    switch (kind) {
+    case KIND_P: hipLaunchKernelGGL(geo_collide<KIND_P>, grid, block, 0, stream, a); break;
+    case KIND_F3: hipLaunchKernelGGL(geo_collide<KIND_F3>, grid, block, 0, stream, a); break;
+    case KIND_F1: hipLaunchKernelGGL(geo_collide<KIND_F1>, grid, block, 0, stream, a); break;
+    default: hipLaunchKernelGGL(geo_collide<KIND_F2>, grid, block, 0, stream, a); break;
+    }
+    return hipGetLastError();
+}
+
 
 
 namespace {

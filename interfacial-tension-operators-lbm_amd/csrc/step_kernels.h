@@ -8,7 +8,8 @@
 namespace itolbm {
 
 // kernel families: one launch per family advances every case of that family by one step
-enum { KIND_P = 0, KIND_F3 = 1, KIND_F1 = 2, KIND_F2 = 3, N_KINDS = 4 };
+// KIND_GEO: cases with solid sites (any operator), stepped by the site-parallel kernels
+enum { KIND_P = 0, KIND_F3 = 1, KIND_F1 = 2, KIND_F2 = 3, KIND_GEO = 4, N_KINDS = 5 };
 
 // Device-resident entry of one case in a launch's case table.
 struct CaseDev {
@@ -59,6 +60,24 @@ hipError_t launch_fill_static_table(void* base, const CaseDev& d, int ntiles, hi
 hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
+
+// One FP64 step of one case with solid sites (step_f64.hip): `in` holds the post-collision state at
+// its own site, `out` receives the next one; phi [S], g [3][S] and nt [6][S] are the case's field
+// arrays; aux [6][S] (gm, F) is written when not null.
+struct GeoLaunch {
+    CaseConst cc;
+    const double* in;
+    double* out;
+    const unsigned char* solid;   // nx * ny sites, 1 = solid
+    double wett;                  // the wall's phase for the mediators (wett_R)
+    double* phi;
+    double* g;
+    double* nt;
+    double* aux;
+    long long S;
+};
+hipError_t launch_step_geo(int kind, const GeoLaunch& a, hipStream_t stream);
+
 // FP64 state in row blocks [y][colour][19][nx] -> [colour][19][S]
 hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream);
 

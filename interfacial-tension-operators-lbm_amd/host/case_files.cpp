@@ -192,6 +192,11 @@ bool read_options(const std::string& dir, Options& o, std::string& err)
                 if (o.digits < 1 || o.digits > 17) throw std::invalid_argument(val);
             } else if (key == "referencia") {
                 o.recentre = std::stoi(val) != 0;
+            } else if (key == "vtk") {
+                if (val == "ascii") o.vtk_flags = 0;
+                else if (val == "binario") o.vtk_flags = LBM_VTK_BINARY;
+                else if (val == "binario_double") o.vtk_flags = LBM_VTK_BINARY | LBM_VTK_DOUBLE;
+                else throw std::invalid_argument(val);
             } else {
                 err = "itolbm.txt: unknown key " + key;
                 return false;
@@ -275,89 +280,5 @@ void inclinada_row(std::ostream& out, int step, const lbm_incl_diag& r)
 }
 
 void perfil_header(std::ostream& out) { out << "# d  c=rho_R/rho  rho  ux  uy\n"; }
-
-// ---------------------------------------------------------------- VTK fields
-
-namespace {
-
-constexpr int Q = 19;
-constexpr int CXI[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
-constexpr int CYI[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, -1, 1, -1, 1};
-constexpr int CZI[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1};
-
-void vtk_head(std::ostream& out, const char* title, int nx, int ny)
-{
-    out << "# vtk DataFile Version 2.0\n" << title << "\nASCII\nDATASET STRUCTURED_POINTS\n";
-    out << "DIMENSIONS " << nx << ' ' << ny << ' ' << 1 << "\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\n";
-    out << "POINT_DATA " << (long long)nx * ny << '\n';
-}
-
-// density and velocity of one colour's populations, summed in index order (calcula,
-// LBM_functions.cpp:1099-1171)
-void moments(const double* f, double& vx, double& vy, double& vz, double& rho)
-{
-    double mx = 0.0, my = 0.0, mz = 0.0;
-    rho = 0.0;
-    for (int i = 0; i < Q; ++i) {
-        rho = rho + f[i];
-        mx = mx + CXI[i] * f[i];
-        my = my + CYI[i] * f[i];
-        mz = mz + CZI[i] * f[i];
-    }
-    if (rho != 0.0) {
-        const double one_over_rho = 1.0 / rho;
-        vx = mx * one_over_rho;
-        vy = my * one_over_rho;
-        vz = mz * one_over_rho;
-    } else {
-        vx = vy = vz = 0.0;
-    }
-}
-
-}  // namespace
-
-void write_density_vtk(const std::string& path, int nx, int ny, const double* f)
-{
-    std::ofstream out(path);
-    vtk_head(out, "Densidade", nx, ny);
-    out << "SCALARS densidade double\nLOOKUP_TABLE default\n";
-    for (int y = 0; y < ny; ++y) {
-        for (int x = 0; x < nx; ++x) {
-            const double* p = f + ((size_t)y * nx + x) * Q;
-            double rho = 0.0;
-            for (int i = 0; i < Q; ++i) rho += p[i];
-            out << rho << ' ';
-        }
-        out << '\n';
-    }
-}
-
-void write_velocity_vtk(const std::string& path, int nx, int ny, const double* fR, const double* fB,
-                        const double* force)
-{
-    std::ofstream out(path);
-    vtk_head(out, "Velocidade", nx, ny);
-    out << "VECTORS velocidade double\n";
-    for (int y = 0; y < ny; ++y) {
-        for (int x = 0; x < nx; ++x) {
-            const size_t s = (size_t)y * nx + x;
-            double vxR, vyR, vzR, rR, vxB, vyB, vzB, rB;
-            moments(fR + s * Q, vxR, vyR, vzR, rR);
-            moments(fB + s * Q, vxB, vyB, vzB, rB);
-            const double cR = rR / (rR + rB), cB = 1. - cR;
-            double vx = cR * vxR + cB * vxB, vy = cR * vyR + cB * vyB, vz = cR * vzR + cB * vzB;
-            if (force) {
-                const double rho = rR + rB;
-                if (rho > 0.0) {
-                    vx = vx - 0.5 * force[3 * s] / rho;
-                    vy = vy - 0.5 * force[3 * s + 1] / rho;
-                    vz = vz - 0.5 * force[3 * s + 2] / rho;
-                }
-            }
-            out << vx << ' ' << vy << ' ' << vz << ' ';
-        }
-        out << '\n';
-    }
-}
 
 }  // namespace itolbm_host

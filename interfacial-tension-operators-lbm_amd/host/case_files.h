@@ -53,6 +53,8 @@ bool read_box(const std::string& path, int& nx, int& ny, int& nz, long long& n_f
 //   digitos         digits of laplace.dat / iso.dat / inclinada.dat / perfil.dat / resumo.dat (default 10)
 //   referencia      1 | 0: re-centre the reduced-precision storage on the measured phase
 //                   densities after each diagnostic (default 1)
+//   vtk             ascii | binario | binario_double: form of the final rho_R / rho_B / vel files
+//                   (default ascii, as the reference writes them; binario: big-endian float)
 struct Options {
     int precision = LBM_PREC_F64;
     double radius = 0.0;
@@ -60,6 +62,7 @@ struct Options {
     int diag_every = 500;
     int digits = 10;
     bool recentre = true;
+    int vtk_flags = 0;   // LBM_VTK_BINARY | LBM_VTK_DOUBLE
 };
 bool read_options(const std::string& dir, Options& o, std::string& err);
 // the itolbm.txt name of a storage form
@@ -75,13 +78,6 @@ void galilean_row(std::ostream& out, int step, double v0, double cs2, const lbm_
 void inclinada_header(std::ostream& out);
 void inclinada_row(std::ostream& out, int step, const lbm_incl_diag& r);
 void perfil_header(std::ostream& out);
-
-// VTK legacy ASCII fields of a post-collision FP64 state ([site][19], site = x + nx y): colour
-// densities (rec_density) and the mixture velocity with the Guo half-force removed when a force
-// field is given (rec_velocity), LBM_functions.cpp:1347-1497.
-void write_density_vtk(const std::string& path, int nx, int ny, const double* f);
-void write_velocity_vtk(const std::string& path, int nx, int ny, const double* fR, const double* fB,
-                        const double* force);
 
 std::string join(const std::string& dir, const std::string& name);
 

@@ -199,9 +199,14 @@ struct DropCase : Member {
             if (lbm_download_aux(ctx, id, nullptr, F.data()) != LBM_OK) F.clear();
         }
         const std::string ts = std::to_string((unsigned)t);
-        write_density_vtk(join(dir, "rho_R" + ts + ".vtk"), nx, ny, fR.data());
-        write_density_vtk(join(dir, "rho_B" + ts + ".vtk"), nx, ny, fB.data());
-        write_velocity_vtk(join(dir, "vel_" + ts + ".vtk"), nx, ny, fR.data(), fB.data(), F.empty() ? nullptr : F.data());
+        const int fl = opt.vtk_flags;
+        const bool ok = lbm_write_vtk(join(dir, "rho_R" + ts + ".vtk").c_str(), LBM_VTK_DENSITY, nx, ny, fR.data(), nullptr,
+                                      nullptr, nullptr, fl) == LBM_OK &&
+                        lbm_write_vtk(join(dir, "rho_B" + ts + ".vtk").c_str(), LBM_VTK_DENSITY, nx, ny, fB.data(), nullptr,
+                                      nullptr, nullptr, fl) == LBM_OK &&
+                        lbm_write_vtk(join(dir, "vel_" + ts + ".vtk").c_str(), LBM_VTK_VELOCITY, nx, ny, fR.data(), fB.data(),
+                                      F.empty() ? nullptr : F.data(), nullptr, fl) == LBM_OK;
+        if (!ok) *con << "\ncampos: nao foi possivel gravar os arquivos vtk\n";
     }
 
     void finish(int step, double elapsed) override

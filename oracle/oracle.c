@@ -207,26 +207,42 @@ static inline int nb(int x, int y, int dx, int dy, int nx, int ny)
     return xx + nx * yy;
 }
 
-/* calcula_forca, Operadores_tensao.cpp:202-332 (viz = x + c_i, aloca_campos_F :149-182) */
-void orc_force_field(const orc_case *cs, const double *fR, const double *fB, double *F)
+/* direction opposite to i (c_opp = -c_i), found as def_dir_prop does (LBM_functions.cpp:713-733) */
+static int opposite(int i)
+{
+    for (int j = 0; j < ORC_Q; j++)
+        if (C[3 * j] == -C[3 * i] && C[3 * j + 1] == -C[3 * i + 1] && C[3 * j + 2] == -C[3 * i + 2]) return j;
+    return i;
+}
+
+/* calcula_forca, Operadores_tensao.cpp:202-332 (viz = x + c_i, aloca_campos_F :149-182: a solid
+ * neighbour reflects onto the site itself).  solid (n sites, 1 = solid) may be NULL: all fluid.
+ * Solid sites hold no fields; their F is set to 0. */
+static void force_field_impl(const orc_case *cs, const unsigned char *solid, const double *fR, const double *fB,
+                             double *F)
 {
     ensure_lattice();
     const int nx = cs->nx, ny = cs->ny, n = nx * ny;
     const double sigma = cs->A;
-    double *fase = (double *)malloc(sizeof(double) * n);
-    double *g = (double *)malloc(sizeof(double) * 3 * n);
-    double *aux = (double *)malloc(sizeof(double) * 6 * n);
+    double *fase = (double *)calloc((size_t)n, sizeof(double));
+    double *g = (double *)calloc((size_t)3 * n, sizeof(double));
+    double *aux = (double *)calloc((size_t)6 * n, sizeof(double));
     int *viz = (int *)malloc(sizeof(int) * ORC_Q * n);
     for (int y = 0; y < ny; y++)
         for (int x = 0; x < nx; x++)
-            for (int i = 0; i < ORC_Q; i++)
-                viz[(x + nx * y) * ORC_Q + i] = nb(x, y, (int)C[3 * i], (int)C[3 * i + 1], nx, ny);
+            for (int i = 0; i < ORC_Q; i++) {
+                const int v = nb(x, y, (int)C[3 * i], (int)C[3 * i + 1], nx, ny);
+                viz[(x + nx * y) * ORC_Q + i] = (solid && solid[v]) ? x + nx * y : v;
+            }
+#define ORC_FLUID(p) (!solid || !solid[p])
     for (int p = 0; p < n; p++) {
+        if (!ORC_FLUID(p)) continue;
         const double rR = density(fR + (size_t)p * ORC_Q);
         const double rB = density(fB + (size_t)p * ORC_Q);
         fase[p] = (rR - rB) / (rR + rB);
     }
     for (int p = 0; p < n; p++) {
+        if (!ORC_FLUID(p)) continue;
         double gx = 0.0, gy = 0.0, gz = 0.0;
         for (int i = 1; i < ORC_Q; i++) {
             const double ph = W[i] * fase[viz[p * ORC_Q + i]];
@@ -242,6 +258,7 @@ void orc_force_field(const orc_case *cs, const double *fR, const double *fB, dou
     const double kf = 0.75 * sigma * xi;
     const double bf = 3.0 * sigma / (8.0 * xi);
     for (int p = 0; p < n; p++) {
+        if (!ORC_FLUID(p)) continue;
         const double *gp = g + 3 * p;
         double *a = aux + 6 * p;
         const double m = sqrt(gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2]);
@@ -268,6 +285,10 @@ void orc_force_field(const orc_case *cs, const double *fR, const double *fB, dou
     for (int p = 0; p < n; p++) {
         const double *gp = g + 3 * p;
         double *Fp = F + 3 * p;
+        if (!ORC_FLUID(p)) {
+            Fp[0] = Fp[1] = Fp[2] = 0.0;
+            continue;
+        }
         if (cs->op == ORC_OP_F1) {
             double d = 0.0;
             for (int i = 1; i < ORC_Q; i++) {
@@ -297,43 +318,84 @@ void orc_force_field(const orc_case *cs, const double *fR, const double *fB, dou
     }
     free(fase); free(g); free(aux); free(viz);
 }
+#undef ORC_FLUID
+
+void orc_force_field(const orc_case *cs, const double *fR, const double *fB, double *F)
+{
+    force_field_impl(cs, NULL, fR, fB, F);
+}
 
 /* One step: Imm_OPS.cpp:246-372 (mediators :252-267, emite_mediadores LBM_functions.cpp:1286-1292,
- * propag_site_med :1309-1337, force :271, collide + propag_site :275-287, LBM_functions.cpp:1183-1253) */
-void orc_step(const orc_case *cs, double *fR, double *fB, double *fR_new, double *fB_new,
-              double *F, double *gm)
+ * propag_site_med :1309-1337, force :271, collide + propag_site :275-287, LBM_functions.cpp:1183-1253).
+ * With solid sites (solid != NULL, 1 = solid): only fluid sites are collided; a population pushed
+ * towards a solid site lands in the opposite direction at its own site (def_dir_prop
+ * LBM_functions.cpp:509-853, odd number of inversions), and the mediator that would come from a solid
+ * site is the wall's, w_i wett 3 (propag_site_med :1331).  Solid sites hold no populations: fR_new /
+ * fB_new are 0 there. */
+static void step_impl(const orc_case *cs, const unsigned char *solid, double wett, double *fR, double *fB,
+                      double *fR_new, double *fB_new, double *F, double *gm)
 {
     ensure_lattice();
     const int nx = cs->nx, ny = cs->ny, n = nx * ny;
-    double *phi = (double *)malloc(sizeof(double) * n);
+    double *phi = (double *)calloc((size_t)n, sizeof(double));
     for (int p = 0; p < n; p++) {
+        if (solid && solid[p]) continue;
         const double rho_R = density(fR + (size_t)p * ORC_Q);
         const double rho_B = density(fB + (size_t)p * ORC_Q);
         phi[p] = (rho_R - rho_B) / (rho_R + rho_B);
     }
     const int forced = cs->op >= ORC_OP_F1;
-    if (forced) orc_force_field(cs, fR, fB, F);
+    if (forced) force_field_impl(cs, solid, fR, fB, F);
     static const double zero[3] = {0.0, 0.0, 0.0};
+    if (solid)
+        for (int p = 0; p < n; p++)
+            if (solid[p])
+                for (int i = 0; i < ORC_Q; i++) {
+                    fR_new[(size_t)p * ORC_Q + i] = 0.0;
+                    fB_new[(size_t)p * ORC_Q + i] = 0.0;
+                }
     for (int y = 0; y < ny; y++)
         for (int x = 0; x < nx; x++) {
             const int p = x + nx * y;
-            /* mediator populations after propagation: M_i(x) = w_i * (3 phi(x - c_i)) */
+            if (solid && solid[p]) {
+                if (gm) gm[3 * p] = gm[3 * p + 1] = gm[3 * p + 2] = 0.0;
+                continue;
+            }
+            /* mediator populations after propagation: M_i(x) = w_i * (3 phi(x - c_i)), or the wall's
+             * (w_i wett) 3 when x - c_i is solid (w of the emitted direction -c_i equals w_i) */
             double fm[ORC_Q];
             for (int i = 0; i < ORC_Q; i++) {
                 const int src = nb(x, y, -(int)C[3 * i], -(int)C[3 * i + 1], nx, ny);
-                const double fat = phi[src] * 3.0;
-                fm[i] = W[i] * fat;
+                if (solid && solid[src]) {
+                    fm[i] = W[i] * wett * 3.0;
+                } else {
+                    const double fat = phi[src] * 3.0;
+                    fm[i] = W[i] * fat;
+                }
             }
             if (gm) momentum(fm, &gm[3 * p], &gm[3 * p + 1], &gm[3 * p + 2]);
             orc_collide_site(fR + (size_t)p * ORC_Q, fB + (size_t)p * ORC_Q, fm,
                              forced ? F + 3 * p : zero, cs);
             for (int i = 0; i < ORC_Q; i++) {
                 const int dst = nb(x, y, (int)C[3 * i], (int)C[3 * i + 1], nx, ny);
-                fR_new[(size_t)dst * ORC_Q + i] = fR[(size_t)p * ORC_Q + i];
-                fB_new[(size_t)dst * ORC_Q + i] = fB[(size_t)p * ORC_Q + i];
+                const size_t to = (solid && solid[dst]) ? (size_t)p * ORC_Q + opposite(i) : (size_t)dst * ORC_Q + i;
+                fR_new[to] = fR[(size_t)p * ORC_Q + i];
+                fB_new[to] = fB[(size_t)p * ORC_Q + i];
             }
         }
     free(phi);
+}
+
+void orc_step(const orc_case *cs, double *fR, double *fB, double *fR_new, double *fB_new,
+              double *F, double *gm)
+{
+    step_impl(cs, NULL, 0.0, fR, fB, fR_new, fB_new, F, gm);
+}
+
+void orc_step_geo(const orc_case *cs, const unsigned char *solid, double wett, double *fR, double *fB,
+                  double *fR_new, double *fB_new, double *F, double *gm)
+{
+    step_impl(cs, solid, wett, fR, fB, fR_new, fB_new, F, gm);
 }
 
 /* initial_conditions_drop, Initial_conditions.cpp:37-169, EQ_CLASSICO branch :141-145,

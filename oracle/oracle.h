@@ -15,6 +15,8 @@
  * Parity pins (what checks each function, see tests/test_oracle_*.py):
  *   orc_step / orc_collide_site / orc_force_field : bitwise vs reference nofma build,
  *       E0 closed forms (E0_resultado.txt), Lotes/ fixtures.
+ *   orc_step_geo : bitwise vs the reference build on boxes with solid sites read by the reference's
+ *       own read_geo (harness geo=), all operators.
  *   orc_init_drop / orc_init_band : bitwise vs reference build.
  *   orc_drop_diag / orc_incl_diag : reference build (%.17g) and Lotes/ galilean.dat.
  */
@@ -63,6 +65,16 @@ void orc_force_field(const orc_case *cs, const double *fR, const double *fB, dou
  */
 void orc_step(const orc_case *cs, double *fR, double *fB, double *fR_new, double *fB_new,
               double *F, double *gm);
+
+/*
+ * The same step on a box with solid sites (solid: n bytes, 1 = solid, x + nx*y): bounce-back of the
+ * populations pushed towards a solid site (def_dir_prop LBM_functions.cpp:509-853, propag_site
+ * :1183-1253), wall mediators of phase wett (propag_site_med :1309-1337), solid neighbours of the force
+ * stencils reflected onto the site (aloca_campos_F Operadores_tensao.cpp:149-182).  Solid sites are
+ * not collided; their populations in fR_new / fB_new are set to 0.
+ */
+void orc_step_geo(const orc_case *cs, const unsigned char *solid, double wett, double *fR, double *fB,
+                  double *fR_new, double *fB_new, double *F, double *gm);
 
 /* initial conditions */
 void orc_init_drop(const orc_case *cs, double R, double k_perfil, int x0, int y0, double v0,

@@ -11,7 +11,10 @@
 //   mode=drop|incl  nx ny  R (drop; default nx/4)  width (initial 10-90 width; <=0 = sharp)
 //   op="P3 0.5"  tau  A  beta  v0 (drop)  p q (incl)  steps  diag (interval)
 //   dump=s1,s2,...  (post-collision populations at these steps -> out/pop_<step>.bin)
+//   geo=<meio.vtk> wett=<wett_R>  (drop mode: solid sites from the file, read by read_geo; the
+//   populations are then dumped for the fluid sites only, in site order; diag=0 skips diagnostics)
 //   out=<dir>
+//   mode=readgeo geo=<file> pts_in pts_out: only read_geo's numbering of the file's sites -> out/meio.bin
 // Writes out/diag.txt: one line per diagnostic step, %.17g, fields named in the header.
 
 #define nvel 19
@@ -68,6 +71,32 @@ int main(int argc, char **argv) {
     }
     omp_set_num_threads(argi("threads", 1));
     const std::string mode = args_s("mode", "drop");
+    if (mode == "readgeo") {
+        // the reference's read_geo on geo=<file> with pts_in / pts_out added columns: the numbering it
+        // gives every site (0 solid, 1.. fluid) -> out/meio.bin (int32), the fluid count -> out/n.txt
+        const std::string geo = args_s("geo", "");
+        int gx = 0, gy = 0, gz = 0;
+        {
+            std::ifstream f(geo);
+            std::string line, dump;
+            for (int i = 0; i < 4; i++) std::getline(f, dump);
+            std::getline(f, line);
+            std::stringstream d(line);
+            d >> dump >> gx >> gy >> gz;
+        }
+        const int pin = argi("pts_in", 0), pout = argi("pts_out", 0);
+        const size_t total = (size_t)(gx + pin + pout) * gy * gz;
+        int *meio = new int[total];
+        const int n = read_geo(geo, meio, pin, pout);
+        const std::string out = args_s("out", ".");
+        FILE *f = fopen((out + "/meio.bin").c_str(), "wb");
+        fwrite(meio, sizeof(int), total, f);
+        fclose(f);
+        f = fopen((out + "/n.txt").c_str(), "w");
+        fprintf(f, "%d %d %d %d\n", n, gx + pin + pout, gy, gz);
+        fclose(f);
+        return 0;
+    }
     const int nx = argi("nx", 100), ny = argi("ny", nx);
     const double tau = argd("tau", 1.0), A_in = argd("A", 0.225), beta = argd("beta", 0.8);
     const double width = argd("width", 3.45);
@@ -93,16 +122,24 @@ int main(int argc, char **argv) {
 
     GEOMETRY geometry;
     geometry.nx = nx; geometry.ny = ny; geometry.nz = 1;
-    geometry.fluid = nx * ny;
-    geometry.ini = new int[nx * ny];
-    for (int p = 0; p < nx * ny; p++) geometry.ini[p] = p + 1;
-    const int n_fluid = nx * ny;
+    const std::string geo = args_s("geo", "");
+    if (geo.empty()) {
+        geometry.fluid = nx * ny;
+        geometry.ini = new int[nx * ny];
+        for (int p = 0; p < nx * ny; p++) geometry.ini[p] = p + 1;
+    } else {
+        // a geometry file with solid sites, read by the reference's own read_geo (its box must be
+        // nx x ny x 1, the size given on the command line)
+        geometry.ini = new int[nx * ny];
+        geometry.fluid = read_geo(geo, geometry.ini, 0, 0);
+    }
+    const int n_fluid = geometry.fluid;
 
     PARAMETERS parameters;
     parameters.tau = tau; parameters.tau_R = tau; parameters.tau_B = tau; parameters.tau_m = 1.0;
     parameters.rho_ini = 1.0; parameters.rho_ini_R = 1.0; parameters.rho_ini_B = 1.0;
     parameters.A_fact = A_in; parameters.A_fact_R = 0.0; parameters.A_fact_B = 0.0;
-    parameters.recoll = beta; parameters.wett_R = 0.0; parameters.tau_nh = 1.0;
+    parameters.recoll = beta; parameters.wett_R = argd("wett", 0.0); parameters.tau_nh = 1.0;
     parameters.n_steps = n_steps; parameters.n_files = 0; parameters.n_threads = 1;
 
     LATTICE lattice;
@@ -215,7 +252,7 @@ int main(int argc, char **argv) {
                 dump_bin(out + nm, forca, 3 * (size_t)n_fluid);
             }
         }
-        const bool medir = (step % passos_diag == 0) || (step == n_steps - 1);
+        const bool medir = passos_diag > 0 && ((step % passos_diag == 0) || (step == n_steps - 1));
         if (medir) {
             if (!incl) {
                 const bool ok = calc_galilean(geometry, lattice, drop, gi, v0, step, x0_ini, y0_ini);

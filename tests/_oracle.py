@@ -58,6 +58,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
         L.orc_step.argtypes = [C.POINTER(Case), dp, dp, dp, dp, C.c_void_p, C.c_void_p]
+        L.orc_step_geo.argtypes = [C.POINTER(Case), np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                   C.c_double, dp, dp, dp, dp, C.c_void_p, C.c_void_p]
         L.orc_init_drop.argtypes = [C.POINTER(Case), C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, dp, dp]
         L.orc_init_band.argtypes = [C.POINTER(Case), C.c_int, C.c_int, C.c_double, dp, dp]
         L.orc_force_field.argtypes = [C.POINTER(Case), dp, dp, dp]
@@ -96,8 +98,11 @@ def _ptr(a):
 class Sim:
     """Oracle simulation holding the reference's state arrays (pre-step, post-stream)."""
 
-    def __init__(self, case):
+    def __init__(self, case, solid=None, wett=0.0):
+        """solid: optional nx*ny mask (1 = solid site, site = x + nx*y) and the wall's wettability."""
         self.cs = case
+        self.solid = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8)
+        self.wett = float(wett)
         n = case.nx * case.ny
         self.n = n
         self.fR = np.zeros(n * Q)
@@ -120,7 +125,11 @@ class Sim:
 
     def step(self):
         """One step; afterwards post_R/post_B hold post-collision populations, fR/fB the next state."""
-        lib().orc_step(C.byref(self.cs), self.fR, self.fB, self.fR_new, self.fB_new, _ptr(self.F), _ptr(self.gm))
+        if self.solid is None:
+            lib().orc_step(C.byref(self.cs), self.fR, self.fB, self.fR_new, self.fB_new, _ptr(self.F), _ptr(self.gm))
+        else:
+            lib().orc_step_geo(C.byref(self.cs), self.solid, self.wett, self.fR, self.fB, self.fR_new, self.fB_new,
+                               _ptr(self.F), _ptr(self.gm))
         self.post_R, self.post_B = self.fR, self.fB
         self.fR, self.fR_new = self.fR_new, self.fR
         self.fB, self.fB_new = self.fB_new, self.fB

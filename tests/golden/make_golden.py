@@ -11,6 +11,9 @@ Run in a container that holds the reference tree (default /root/reference) after
           (oracle/_ref/harness_*): SHA-256 of the post-collision populations / force field at
           chosen steps and the full-precision diagnostics, for small boxes and all operators,
           with the "nofma" (-ffp-contract=off) build and with the optimised "fma" build.
+  ref_geo the same hashes on a box with solid sites (geometry file read by the reference's read_geo;
+          fluid sites only, in site order).
+  read_geo the numbering read_geo gives the sites of geometry files (with added columns).
 
 Usage:  python3 tests/golden/make_golden.py [--ref /root/reference]
 """
@@ -53,6 +56,66 @@ SMALL.append(("incl 30 P3 0.5 (2,2)", "incl",
                    diag=3, dump="0,9")))
 
 
+# boxes with solid sites (the reference library's bounce-back, wall mediators and force-stencil
+# reflection; its drop and band programs never use them): a 24 x 20 box with a solid bottom row, a
+# 3 x 2 block, single sites and one solid site inside the drop's interface; all operators, two wall
+# wettabilities.  Populations are dumped for the fluid sites only (the reference stores no others).
+GEO_SOLIDS = ([[x, 0] for x in range(24)] + [[x, y] for x in range(4, 7) for y in range(12, 14)]
+              + [[18, 5], [20, 15], [21, 16], [16, 10]])
+GEO = []
+for k, op in enumerate(OPS):
+    A = "0.225" if op[0] == "P" else "0.1"
+    GEO.append((f"geo24x20 {op}", dict(nx=24, ny=20, R=5, width=3.45 if k % 2 == 0 else -1, op=op, tau=0.8, A=A,
+                                         beta=0.8, wett=0.3 if k % 2 == 0 else -0.5, steps=8, diag=0, dump="0,7",
+                                         solid=GEO_SOLIDS)))
+
+
+def write_geometry(path, nx, ny, solids):
+    """meio.vtk in the layout read_geo reads (LBM_functions.cpp:280-354): 10 header lines, then nx * ny
+    integers in site order, 0 = solid."""
+    m = [[1] * nx for _ in range(ny)]
+    for x, y in solids:
+        m[y][x] = 0
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 2.0\nGeometria\nASCII\nDATASET STRUCTURED_POINTS\n")
+        f.write(f"DIMENSIONS {nx} {ny} 1\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\nPOINT_DATA {nx * ny}\n")
+        f.write("SCALARS Geometria float\nLOOKUP_TABLE default\n")
+        for row in m:
+            f.write(" ".join(str(v) for v in row) + " \n")
+
+
+def write_values(path, dims, values):
+    """A geometry file with arbitrary integer values (nx * ny * nz, site order)."""
+    nx, ny, nz = dims
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 2.0\nGeometria\nASCII\nDATASET STRUCTURED_POINTS\n")
+        f.write(f"DIMENSIONS {nx} {ny} {nz}\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\nPOINT_DATA {nx * ny * nz}\n")
+        f.write("SCALARS Geometria float\nLOOKUP_TABLE default\n")
+        for r in range(ny * nz):
+            f.write(" ".join(str(v) for v in values[r * nx:(r + 1) * nx]) + " \n")
+
+
+# geometry files read by the reference's read_geo: its numbering of every site (with added columns)
+def readgeo_cases():
+    box = [1] * (24 * 20)
+    for x, y in GEO_SOLIDS:
+        box[x + 24 * y] = 0
+    vol = [(3 * i + 1) % 4 for i in range(5 * 4 * 2)]   # 0 = solid; 1, 2, 3 all fluid
+    return [("box 24x20 solids", (24, 20, 1), box, 0, 0), ("box 24x20 solids + 2/3 columns", (24, 20, 1), box, 2, 3),
+            ("volume 5x4x2", (5, 4, 2), vol, 1, 1)]
+
+
+def run_readgeo(dims, values, pin, pout):
+    exe = os.path.join(REFBIN, "harness_drop_nofma")
+    with tempfile.TemporaryDirectory() as d:
+        geo = os.path.join(d, "meio.vtk")
+        write_values(geo, dims, values)
+        subprocess.run([exe, "mode=readgeo", f"geo={geo}", f"pts_in={pin}", f"pts_out={pout}", f"out={d}"],
+                       check=True, stdout=subprocess.DEVNULL)
+        n, nx, ny, nz = [int(v) for v in open(os.path.join(d, "n.txt")).read().split()]
+        return {"n_fluid": n, "box": [nx, ny, nz], "meio_sha": sha(os.path.join(d, "meio.bin"))}
+
+
 def sha(path):
     with open(path, "rb") as f:
         return hashlib.sha256(f.read()).hexdigest()
@@ -61,7 +124,13 @@ def sha(path):
 def run_harness(kind, mode, args):
     exe = os.path.join(REFBIN, f"harness_{'incl' if mode == 'incl' else 'drop'}_{kind}")
     with tempfile.TemporaryDirectory() as d:
-        cmd = [exe, f"mode={mode}", f"out={d}"] + [f"{k}={v}" for k, v in args.items()]
+        extra = []
+        if "solid" in args:
+            geo = os.path.join(d, "meio.vtk")
+            write_geometry(geo, int(args["nx"]), int(args["ny"]), args["solid"])
+            extra = [f"geo={geo}"]
+        cmd = ([exe, f"mode={mode}", f"out={d}"] + [f"{k}={v}" for k, v in args.items() if k != "solid"]
+               + extra)
         subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
         lines = open(os.path.join(d, "diag.txt")).read().splitlines()
         names = lines[0][2:].split()
@@ -143,6 +212,15 @@ def main():
         diag_f, _ = run_harness("fma", mode, args)
         out["ref"].append({"label": label, "mode": mode, "args": args, "diag_nofma": diag_n,
                            "hash_nofma": hash_n, "diag_fma": diag_f})
+
+    out["read_geo"] = []
+    for label, dims, values, pin, pout in readgeo_cases():
+        out["read_geo"].append({"label": label, "dims": list(dims), "values": values, "pts_in": pin, "pts_out": pout,
+                                **run_readgeo(dims, values, pin, pout)})
+    out["ref_geo"] = []
+    for label, args in GEO:
+        _, hash_n = run_harness("nofma", "drop", args)
+        out["ref_geo"].append({"label": label, "mode": "drop", "args": args, "hash_nofma": hash_n})
 
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(out, f, indent=1)

@@ -99,10 +99,16 @@ def test_incl_rejects_zero_normal(tmp_path):
 
 def test_options_file(tmp_path):
     K.make_case(str(tmp_path), 16, "P1", 1.0, 0.225, 10, options={"precisao": "f32", "raio": 5, "espessura_ini": -1,
-                                                               "passos_diag": 100, "digitos": 17, "referencia": 0})
+                                                               "passos_diag": 100, "digitos": 17, "referencia": 0,
+                                                               "vtk": "binario"})
     r = K.run("Imm_OPS", [], str(tmp_path), timeout=60)
     out = _stderr_out(r)
     assert "raio inicial      = 5" in out and "precisao          = f32" in out
+    with open(tmp_path / "itolbm.txt", "a") as f:
+        f.write("vtk xml\n")
+    r = K.run("Imm_OPS", [], str(tmp_path), timeout=60)
+    assert r.returncode == 1 and "bad value for vtk" in _stderr_out(r)
+    K.make_case(str(tmp_path), 16, "P1", 1.0, 0.225, 10, options={"vtk": "binario_double"})
     with open(tmp_path / "itolbm.txt", "a") as f:
         f.write("cor azul\n")
     r = K.run("Imm_OPS", [], str(tmp_path), timeout=60)
@@ -298,6 +304,35 @@ def test_vtk_fields_match_oracle(tmp_path):
     u = ((fR + fB) @ c.T - 0.5 * sim.F.reshape(-1, 3)[:, :2]) / rho[:, None]
     assert np.allclose(vel[:, :2], u, rtol=2e-5, atol=1e-10)
     assert np.abs(vel[:, 2]).max() <= 1e-15
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form,dt", [("binario", ">f4"), ("binario_double", ">f8")])
+def test_vtk_binary_option(tmp_path, form, dt):
+    """itolbm.txt "vtk binario[_double]": the same fields as the ASCII files of the same run, in the
+    binary legacy form (big-endian float / double after the header, then a newline)."""
+    import numpy as np
+    say(f"[gpu] Imm_OPS vtk {form}, 24x24 P3 0.5, 20 steps")
+    da, db = str(tmp_path / "a"), str(tmp_path / "b")
+    for d, opt in ((da, {}), (db, {"vtk": form})):
+        os.makedirs(d)
+        K.make_case(d, 24, "P3 0.5", 1.0, 0.225, 20, options=opt)
+        r = K.run("Imm_OPS", [], d, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+    for name, ncomp in (("rho_R19.vtk", 1), ("rho_B19.vtk", 1), ("vel_19.vtk", 3)):
+        with open(os.path.join(da, name)) as f:
+            lines = f.read().splitlines()
+        start = next(i for i, ln in enumerate(lines) if ln.startswith(("LOOKUP_TABLE", "VECTORS"))) + 1
+        asc = np.array([float(v) for ln in lines[start:] for v in ln.split()])
+        raw = open(os.path.join(db, name), "rb").read()
+        head = raw.split(b"\n")[:start]
+        assert [h.decode() for h in head] == [ln.replace("ASCII", "BINARY").replace(
+            "double", "float" if dt == ">f4" else "double") for ln in lines[:start]]
+        body = raw[len(b"\n".join(head)) + 1:]
+        assert body.endswith(b"\n") and len(body) - 1 == asc.size * np.dtype(dt).itemsize
+        binv = np.frombuffer(body[:-1], dtype=dt).astype(float)
+        assert np.allclose(binv, asc, rtol=6e-6, atol=1e-12), name   # ASCII keeps 6 digits
+        assert asc.size == 24 * 24 * ncomp
 
 
 @pytest.mark.gpu
