@@ -29,15 +29,16 @@ Also reported:
                  --tune-steps more untimed steps): algorithmic bytes per launch (the form's stored
                  bytes) / average launch duration; traffic = HBM bytes per launch from rocprofv3
                  FETCH_SIZE / WRITE_SIZE runs (profiles/traffic_<form>.json), when present;
-                 stream_GBs = the library's own
-                 streaming kernel (read + write of 1 GiB, 16-byte loads, nontemporal stores) timed the
-                 same way, the bandwidth a plain streaming kernel reaches on this GPU
+                 stream_GBs = the library's own streaming kernel (read + write of 1 GiB, 16-byte
+                 loads, nontemporal stores) timed the same way, the bandwidth a plain streaming
+                 kernel reaches on this GPU
   large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in FP32 (all 19
                  populations, 304 B per update), FP32 z-symmetric and FP64 storage
   sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
                  numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
                  batched reduced-precision form vs the plain FP64 one-launch-per-case path over
-                 --sweep-steps steps in all (the first 400 untimed), delta_p agreement at the end
+                 --sweep-steps steps in all (the first 400 untimed), delta_p agreement at the end;
+                 FP16 z-symmetric storage batched the same way, for its delta_p agreement
   cpu_baseline   the reference program Imm_OPS (built from the reference's sources by oracle/Makefile)
                  on this host's cores, over a bounded sample of the same workload
 
@@ -281,7 +282,10 @@ def bench_sweep(m, timer, stream, form, steps):
     sites = sum(d.nx * d.ny for d in descs)
     timed = steps - SWEEP_WARMUP
     res = {}
-    for label, prec in (("f64_per_case", 0), (f"{form}_batched", PREC_ID[form])):
+    runs = [("f64_per_case", 0), (f"{form}_batched", PREC_ID[form])]
+    if form != "f16z":
+        runs.append(("f16z_batched", PREC_ID["f16z"]))   # FP16 storage, for its delta_p agreement
+    for label, prec in runs:
         say(f"bench:   sweep {label}")
         ctx = m.Context(descs, precision=prec)
         ctx.set_stream(stream.cuda_stream)
@@ -310,12 +314,23 @@ def bench_sweep(m, timer, stream, form, steps):
         ctx.close()
         res[label] = {"wall_s": wall_s, "ms_per_step": wall_s / timed * 1e3, "dp": dp, "ok": ok}
     a, b = res["f64_per_case"], res[f"{form}_batched"]
-    good = (a["ok"] == 1) & (b["ok"] == 1) & np.isfinite(a["dp"]) & np.isfinite(b["dp"]) & (np.abs(a["dp"]) > 1e-9)
-    err = np.abs(b["dp"][good] - a["dp"][good]) / np.abs(a["dp"][good])
+
+    def agreement(b):
+        good = (a["ok"] == 1) & (b["ok"] == 1) & np.isfinite(a["dp"]) & np.isfinite(b["dp"]) & (np.abs(a["dp"]) > 1e-9)
+        return good, np.abs(b["dp"][good] - a["dp"][good]) / np.abs(a["dp"][good])
+
+    good, err = agreement(b)
     labels = [lab for (_, _, lab), g in zip(cases, good) if g]
     per_hour = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
     over = [[labels[j], float(err[j])] for j in np.argsort(err)[::-1] if err[j] > 1e-4]
+    fp16 = None
+    if "f16z_batched" in res:
+        g16, e16 = agreement(res["f16z_batched"])
+        fp16 = {"batched_ms_per_step": res["f16z_batched"]["ms_per_step"], "dp_compared_cases": int(g16.sum()),
+                "dp_rel_err_median": float(np.median(e16)) if e16.size else None,
+                "dp_within_1e-4": int((e16 <= 1e-4).sum())}
     return {"cases": len(descs), "sites": sites, "steps": steps, "untimed_steps": SWEEP_WARMUP, "form": form,
+            "f16z": fp16,
             "f64_per_case_ms_per_step": a["ms_per_step"], "batched_ms_per_step": b["ms_per_step"],
             "speedup": a["ms_per_step"] / b["ms_per_step"],
             "cases_per_hour_20k_steps": {"f64_per_case": per_hour(a["ms_per_step"]),
