@@ -4,8 +4,9 @@
 Workload (the 512^2 static-drop configuration): a 512 x 512 x 1 periodic box holding a sharp
 cylindrical drop of radius 100 at the centre, tau in {0.6, 1.0}, sigma = 0.1 (A = 9 sigma / 4 tau for
 the perturbation operators), beta = 0.8, for all seven operators (P1, P3 chi = -0.4 / 0.5 / 2, F1,
-F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (one launch per operator
-family per step).  A bench step advances every case by one time step.  Inputs are synthetic
+F2, F3 xi = 1.569988): 14 cases resident on the GPU, advanced together (in the launch form the
+library chooses by timing: one launch per operator family per step, or, FP64 only, one launch for all
+of them, reported as launch_form).  A bench step advances every case by one time step.  Inputs are synthetic
 (generated by the library's initial condition), resident in HBM before timing.
 
 Before anything is timed, 2 s of untimed streaming copies bring the GPU to its sustained clocks.
@@ -108,6 +109,25 @@ class Timer:
         return e0.elapsed_time(e1) * 1e-3, time.perf_counter() - w0
 
 
+class launch_form:
+    """Fixes lbm_step's launch form (ITOLBM_FUSED: 0 one launch per family, 1 one launch, 2 one
+    persistent launch) while the block runs; None leaves the choice to the library."""
+
+    def __init__(self, mode):
+        self.mode, self.old = mode, None
+
+    def __enter__(self):
+        self.old = os.environ.get("ITOLBM_FUSED")
+        if self.mode is not None:
+            os.environ["ITOLBM_FUSED"] = str(self.mode)
+
+    def __exit__(self, *exc):
+        if self.old is None:
+            os.environ.pop("ITOLBM_FUSED", None)
+        else:
+            os.environ["ITOLBM_FUSED"] = self.old
+
+
 def shapes_of(ctx, kinds):
     out = {}
     for fam, name in enumerate(FAMILY_NAMES):
@@ -141,8 +161,10 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     total = tune_steps + warmup + steps + 1
     dp = [ctx.diag_drop(0.0, total - 1, case=i).delta_p for i in range(len(cases))]
     n_sites = NX * NY * len(cases)
+    mode = ctx.step_mode()   # the launch form the library chose for the 14 cases
     # the dominant kernel alone: the P1/P3 family launch (8 cases), at the tile shape the library tunes
-    # for it running alone (leaving a member out and back in re-tunes the family)
+    # for it running alone (leaving a member out and back in re-tunes the family), in the launch form
+    # of the 14-case step
     for i, k in enumerate(kinds):
         if k != "P":
             ctx.set_active(False, case=i)
@@ -151,16 +173,19 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     ctx.set_active(True, case=p0)
     nP = sum(1 for k in kinds if k == "P")
     ksteps = max(steps, 200)
-    ctx.step(tune_steps)
-    ctx.sync()
-    kshape = ctx.tile_shape(case=p0)
-    kern_s, _ = timer.run(lambda: ctx.step(ksteps))
+    with launch_form(mode):
+        ctx.step(tune_steps)
+        ctx.sync()
+        kshape = ctx.tile_shape(case=p0)
+        kern_s, _ = timer.run(lambda: ctx.step(ksteps))
     ctx.close()
     avg = kern_s / ksteps
     bytes_launch = B_LU[form] * NX * NY * nP
     return {"MLUPS": n_sites * steps / wall_s / 1e6, "ms_per_step": wall_s / steps * 1e3,
             "gpu_ms_per_step": gpu_s / steps * 1e3, "tile_shapes": shapes,
-            "dominant_kernel": KERNEL[form], "kernel_cases_per_launch": nP,
+            "launch_form": mode,
+            "dominant_kernel": KERNEL[form] if mode <= 0 else "step_fused (one launch; here the P1/P3 family's cases)",
+            "kernel_cases_per_launch": nP,
             "kernel_tile_shape": {"rows_per_segment": kshape[0], "depth": kshape[1], "tuned": kshape[2]},
             "kernel_launches_timed": ksteps, "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
             "achieved_GBs": bytes_launch / avg / 1e9, "frac": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
@@ -181,9 +206,10 @@ def bench_family(m, timer, stream, family, form, steps, tune_steps):
     ctx.set_stream(stream.cuda_stream)
     for i in range(len(descs)):
         ctx.init_drop(R, width=-1.0, case=i)
-    ctx.step(tune_steps)
-    ctx.sync()
-    rows, depth, tuned = ctx.tile_shape()
+    with launch_form(0):   # the family's own kernel
+        ctx.step(tune_steps)
+        ctx.sync()
+        rows, depth, tuned = ctx.tile_shape()
     gpu_s, _ = timer.run(lambda: ctx.step(steps))
     ctx.close()
     avg = gpu_s / steps
@@ -212,10 +238,12 @@ def bench_large(m, timer, stream, form, steps):
     ctx.sync()
     rows, depth, tuned = ctx.tile_shape()
     gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
+    mode = ctx.step_mode()
     ctx.close()
     avg = gpu_s / steps
     return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
             "steps": steps, "tile_shape": {"rows_per_segment": rows, "depth": depth, "tuned": tuned},
+            "launch_form": mode,
             "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
             "bytes_per_lattice_update": B_LU[form],
             "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS}

@@ -176,6 +176,14 @@ int lbm_set_graph(lbm_ctx* ctx, int steps_per_graph);
  * default (0).  No reference counterpart (the reference has no tiles); for reports and benchmarks. */
 int lbm_tile_shape(const lbm_ctx* ctx, int case_id, int* rows, int* depth, int* tuned);
 
+/* How lbm_step launches the cases of the strip-kernel families: 0 one launch per family (on
+ * concurrent streams), 1 one launch for all of them (FP64: a wavefront per tile, each running its
+ * family's code), 2 that launch with a grid of the resident capacity, each wavefront looping over
+ * tiles (persistent); -1 not chosen yet.  lbm_step chooses by timing real steps once the tile shapes
+ * are tuned (FP32/FP16 forms: always 0); ITOLBM_FUSED=0|1|2 fixes it.  The results are the same bit
+ * for bit.  No reference counterpart (the reference runs one case per process). */
+int lbm_step_mode(const lbm_ctx* ctx, int* mode);
+
 /* Copies `bytes` bytes of device memory from src to dst (both 16-byte aligned) on the HIP stream
  * hip_stream (NULL: the null stream) with the library's plain streaming kernel (16-byte loads,
  * nontemporal stores).  No reference counterpart: it measures the bandwidth a read-once / write-once

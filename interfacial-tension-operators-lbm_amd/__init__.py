@@ -34,7 +34,7 @@ SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "l
            "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case",
            "lbm_step_case_noalloc", "lbm_tile_shape", "lbm_stream_copy", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
            "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl", "lbm_read_geometry", "lbm_set_geometry",
-           "lbm_write_vtk"]
+           "lbm_write_vtk", "lbm_step_mode"]
 
 
 class CaseDesc(C.Structure):
@@ -105,6 +105,7 @@ def lib():
         L.lbm_download_aux.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p]
         L.lbm_diag_drop.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(DropDiag)]
         L.lbm_diag_incl.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(InclDiag), C.c_void_p]
+        L.lbm_step_mode.argtypes = [P, ip]
         L.lbm_read_geometry.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, ip, ip, C.c_void_p, C.c_longlong]
         L.lbm_read_geometry.restype = C.c_longlong
         L.lbm_set_geometry.argtypes = [P, C.c_int, C.c_void_p, C.c_double]
@@ -243,6 +244,12 @@ class Context:
     def step_noalloc(self, n=1, record=False, case=0):
         """lbm_step_case_noalloc: one case, nothing allocated (static scratch on the device)."""
         self._chk(lib().lbm_step_case_noalloc(self.h, case, n, STEP_RECORD_AUX if record else 0))
+
+    def step_mode(self):
+        """lbm_step_mode: -1 undecided, 0 one launch per family, 1 one launch, 2 one persistent launch."""
+        v = C.c_int()
+        self._chk(lib().lbm_step_mode(self.h, C.byref(v)))
+        return v.value
 
     def tile_shape(self, case=0):
         """(rows per segment, pipeline depth, tuned) lbm_step uses for the case's kernel family."""

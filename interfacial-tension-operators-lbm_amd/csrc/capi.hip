@@ -62,6 +62,9 @@ struct CaseState {
     double* geo_f = nullptr;
 };
 
+// The group of every strip-kernel case of an FP64 context, stepped by one launch (step_fused).
+constexpr int KIND_FUSED = 100;
+
 // A set of cases advanced by one launch per step (cases of one kernel family).
 struct Group {
     int kind = 0;
@@ -110,6 +113,11 @@ struct lbm_ctx {
     void* scratch = nullptr;           // that device's static scratch array (lbm_step_case_noalloc)
     Group family[N_KINDS];             // all cases of each kernel family (lbm_step)
     std::vector<Group> single;         // one group per case (lbm_step_case)
+    // lbm_step's launch form for the strip families (FP64): -1 not chosen yet, 0 one launch per family
+    // (on forked streams), 1 one launch of all of them (a wavefront per tile), 2 the same launch with
+    // a grid of the resident capacity (persistent); chosen by timing, or ITOLBM_FUSED
+    int fuse_mode = -1;
+    Group fused;                       // the table of that one launch (kind KIND_FUSED)
 };
 
 namespace {
@@ -477,6 +485,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         }
     }
     for (int kd = 0; kd < N_KINDS; ++kd) c->family[kd].kind = kd;
+    c->fused.kind = KIND_FUSED;
     c->single.resize(n_cases);
     for (int i = 0; i < n_cases; ++i) {
         c->family[c->cases[i].kind].members.push_back(i);
@@ -509,6 +518,7 @@ int lbm_destroy(lbm_ctx* c)
     };
     for (Group& g : c->family) free_group(g);
     for (Group& g : c->single) free_group(g);
+    free_group(c->fused);
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
     if (c->graph.graph) (void)hipGraphDestroy(c->graph.graph);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -771,6 +781,7 @@ int group_sync(lbm_ctx* c, Group& g)
         d.nstrip = strips_of(k.d);
         d.nseg = (k.d.ny + d.ly - 1) / d.ly;
         d.tile0 = tile0;
+        d.kind = k.kind;
         const int nt = d.nstrip * d.nseg;
         tile_case.insert(tile_case.end(), nt, j);
         tile0 += nt;
@@ -832,6 +843,13 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
     a.record = record;
     a.depth = g.depth;
     const CaseState& k0 = c->cases[g.members[0]];   // a context has one storage form
+    a.ntiles = g.ntiles;
+    if (g.kind == KIND_FUSED) {   // every strip family of the FP64 context in one launch
+        const int cap = step_f64_fused_capacity();
+        const int grid = (c->fuse_mode == 2 && cap > 0) ? std::min(cap, (g.ntiles + 7) / 8 * 8) : g.ntiles;
+        HIPCHK(c, launch_step_f64_fused(a, grid, st));
+        return LBM_OK;
+    }
     HIPCHK(c, k0.elem == 8 ? launch_step_f64(g.kind, a, g.ntiles, st)
                            : launch_step_lp(k0.elem, k0.sym, g.kind, a, g.ntiles, st));
     return LBM_OK;
@@ -1023,6 +1041,82 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
     return LBM_OK;
 }
 
+// The fused group's members: every active case of the strip families, in case order, each with its
+// family's rows per segment.  Rewritten (table stale) only when that set or a shape changed.
+void fused_members(lbm_ctx* c)
+{
+    std::vector<int> mem, ly;
+    for (int id = 0; id < (int)c->cases.size(); ++id) {
+        const CaseState& k = c->cases[id];
+        if (!k.active || k.kind >= KIND_GEO) continue;
+        const Group& f = c->family[k.kind];
+        const auto it = std::find(f.members.begin(), f.members.end(), id);
+        const size_t j = (size_t)(it - f.members.begin());
+        mem.push_back(id);
+        ly.push_back(j < f.ly.size() ? f.ly[j] : group_rows(c->cases, f.members));
+    }
+    Group& g = c->fused;
+    if (mem != g.members || ly != g.ly) {
+        g.members = mem;
+        g.ly = ly;
+        g.written = false;
+    }
+}
+
+// The groups lbm_step launches: the families, or the one fused launch (plus cases with solids).
+std::vector<Group*> run_groups(lbm_ctx* c, const std::vector<Group*>& act)
+{
+    if (c->fuse_mode <= 0 || c->fused.members.empty()) return act;
+    std::vector<Group*> run{&c->fused};
+    for (Group* g : act)
+        if (g->kind == KIND_GEO) run.push_back(g);
+    return run;
+}
+
+// Chooses lbm_step's launch form for an FP64 context once its families are tuned: one launch per
+// family, one launch of all (step_fused, a wavefront per tile) or that launch persistent (a grid of
+// the resident capacity), each timed over W untimed + 2T timed real steps; one launch per family
+// stays unless another form is faster by more than 1 %.  ITOLBM_FUSED=0|1|2 fixes the form.  The
+// forms give the same bits (the same per-tile code).  The steps used are subtracted from `left`.
+int fuse_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
+{
+    int strip = 0;
+    for (const Group* g : act) strip += g->kind < KIND_GEO;
+    if (c->cases.empty() || c->cases[0].elem != 8 || strip == 0) {
+        c->fuse_mode = 0;
+        return LBM_OK;
+    }
+    if (const char* e = std::getenv("ITOLBM_FUSED")) {
+        c->fuse_mode = std::max(0, std::min(2, std::atoi(e)));
+        return LBM_OK;
+    }
+    constexpr int W = 1, T = 12;
+    if (!c->autotune || left < 3 * (W + T)) return LBM_OK;   // undecided: one launch per family for now
+    fused_members(c);
+    float ms[3] = {0.f, 0.f, 0.f};
+    for (int mode = 0; mode < 3; ++mode) {
+        c->fuse_mode = mode;
+        const std::vector<Group*> run = run_groups(c, act);
+        for (Group* g : run)   // the other form's steps moved the buffers: tables are rewritten
+            if (int r = group_prepare(c, *g, false)) return r;
+        if (int r = queue_steps(c, run, W, 0)) return r;
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+        if (int r = queue_steps(c, run, T, 0)) return r;
+        HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&ms[mode], c->ev[0], c->ev[1]));
+        left -= W + T;
+    }
+    int best = 0;
+    for (int mode = 1; mode < 3; ++mode)
+        if (ms[mode] < ms[best] && ms[mode] < 0.99f * ms[0]) best = mode;
+    c->fuse_mode = best;
+    if (std::getenv("ITOLBM_TUNE_LOG"))
+        std::fprintf(stderr, "itolbm step tuner: launch form %d (per family %.1f us, one launch %.1f us, persistent %.1f us)\n",
+                     best, 1000.0 * ms[0] / T, 1000.0 * ms[1] / T, 1000.0 * ms[2] / T);
+    return LBM_OK;
+}
+
 // Host bookkeeping of n steps run by a graph replay (the launches themselves were captured).
 void advance(lbm_ctx* c, Group& g, int n_steps)
 {
@@ -1040,9 +1134,9 @@ void drop_graph(lbm_ctx* c)
     c->graph = StepGraph{};
 }
 
-std::vector<long long> graph_key(const std::vector<Group*>& act)
+std::vector<long long> graph_key(const lbm_ctx* c, const std::vector<Group*>& act)
 {
-    std::vector<long long> key;
+    std::vector<long long> key{c->fuse_mode};
     for (const Group* g : act) {
         key.push_back(g->kind);
         key.push_back(g->version);
@@ -1057,7 +1151,7 @@ std::vector<long long> graph_key(const std::vector<Group*>& act)
 // Capturing queues nothing, so the host bookkeeping of the captured steps is undone afterwards.
 int ensure_graph(lbm_ctx* c, const std::vector<Group*>& act, int steps)
 {
-    std::vector<long long> key = graph_key(act);
+    std::vector<long long> key = graph_key(c, act);
     if (c->graph.exec && c->graph.steps == steps && c->graph.key == key) return LBM_OK;
     drop_graph(c);
     std::vector<int> k0, cur0;
@@ -1189,20 +1283,37 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
     // the call's own steps when there are enough of them (the recording step excepted).
     int left = n_steps;
     if (untuned) {
+        c->fuse_mode = -1;   // the families' shapes change: the launch form is chosen again
         int avail = left - (rec ? 1 : 0);
         const int before = avail;
         if (int r = online_tune(c, act, avail)) return r;
         left -= before - avail;
     }
+    bool tuned = true;
+    for (const Group* g : act) tuned = tuned && (g->tuned || !c->autotune || g->kind == KIND_GEO);
+    if (!cap && c->fuse_mode < 0 && (tuned || std::getenv("ITOLBM_FUSED"))) {
+        if (!c->ev[0]) {
+            HIPCHK(c, hipEventCreate(&c->ev[0]));
+            HIPCHK(c, hipEventCreate(&c->ev[1]));
+        }
+        int avail = left - (rec ? 1 : 0);
+        const int before = avail;
+        if (int r = fuse_tune(c, act, avail)) return r;
+        left -= before - avail;
+    }
+    if (c->fuse_mode > 0 && !cap) fused_members(c);
+    const std::vector<Group*> run = run_groups(c, act);
+    for (Group* g : run)   // tables written before tuning steps of another launch form are stale
+        if (int r = group_prepare(c, *g, rec)) return r;
     const int G = cap ? 0 : c->graph_steps;
     if (G > 0 && left - (rec ? 1 : 0) >= G) {
         const int reps = (left - (rec ? 1 : 0)) / G;
-        if (int r = ensure_graph(c, act, G)) return r;
+        if (int r = ensure_graph(c, run, G)) return r;
         for (int i = 0; i < reps; ++i) HIPCHK(c, hipGraphLaunch(c->graph.exec, c->stream));
-        for (Group* g : act) advance(c, *g, reps * G);
+        for (Group* g : run) advance(c, *g, reps * G);
         left -= reps * G;
     }
-    if (left > 0) return queue_steps(c, act, left, flags);
+    if (left > 0) return queue_steps(c, run, left, flags);
     return LBM_OK;
 }
 
@@ -1250,6 +1361,13 @@ int lbm_tile_shape(const lbm_ctx* c, int id, int* rows, int* depth, int* tuned)
     return LBM_OK;
 }
 
+int lbm_step_mode(const lbm_ctx* c, int* mode)
+{
+    if (!c || !mode) return LBM_E_ARG;
+    *mode = c->fuse_mode;
+    return LBM_OK;
+}
+
 int lbm_set_active(lbm_ctx* c, int id, int active)
 {
     if (int r = check_case(c, id)) return r;
@@ -1263,6 +1381,7 @@ int lbm_set_active(lbm_ctx* c, int id, int active)
     g.ly.clear();   // row segments are re-chosen for the new member set
     g.written = false;
     g.tuned = false;
+    c->fuse_mode = -1;
     return LBM_OK;
 }
 
@@ -1330,6 +1449,7 @@ int lbm_set_geometry(lbm_ctx* c, int id, const int* meio, double wett_R)
         one.tuned = false;
     }
     drop_graph(c);   // a captured graph holds the old wettability and family launches
+    c->fuse_mode = -1;
     return lbm_upload_state(c, id, fR.data(), fB.data());
 }
 

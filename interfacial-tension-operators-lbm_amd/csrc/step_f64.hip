@@ -381,14 +381,15 @@ struct TileCtx {
     gdouble* aux;
     long long S;     // aux stride
     int strip, y0, y1;
+    int kind;        // kernel family of the case
 };
 
-__device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtx& t)
+__device__ __forceinline__ bool tile_ctx_at(const StepLaunch& L, int tile, TileCtx& t)
 {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int c = L.tile_case[tile];
     const CaseDev& d = L.tab[c];
     t.cc = d.cc;
+    t.kind = d.kind;
     t.in = (gcdouble*)d.pop[L.parity];
     t.out = (gdouble*)d.pop[L.parity ^ 1];
     t.S = d.S;
@@ -401,14 +402,17 @@ __device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtx& t)
     return t.y0 < d.cc.ny;
 }
 
+__device__ __forceinline__ bool tile_ctx(const StepLaunch& L, TileCtx& t)
+{
+    return tile_ctx_at(L, xcd_tile(blockIdx.x, gridDim.x), t);
+}
+
 // ------------------------------------------------------------------ P1 / P3 and F3 (1-ring)
 // Aligned 64-column strips; the colour field of the two columns beside the strip comes from one
 // extra 38-lane load per colour and row (Halo1), summed in the same order as every other site's.
 template <bool F3>
-__global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
+__device__ __forceinline__ void ring1_tile(const TileCtx& t, double (&hbuf)[2][2 * Q])
 {
-    TileCtx t;
-    if (!tile_ctx(launch, t)) return;
     const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
     const Lane L = make_lane_aligned(nx, t.strip);
@@ -419,10 +423,9 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
         vr = t.in[e];
         vb = t.in[e + (unsigned)(Q * nx)];
     };
-    // colour field of the two halo columns: their 38 pulled values go through LDS to lane 0 (left
-    // site) and lane 63 (right site), the only lanes that use them, each summing in index order as
-    // dens() (summing on the wavefront's uniform values took 152 lane reads per row)
-    __shared__ double hbuf[2][2 * Q];
+    // colour field of the two halo columns: their 38 pulled values go through LDS (hbuf) to lane 0
+    // (left site) and lane 63 (right site), the only lanes that use them, each summing in index order
+    // as dens() (summing on the wavefront's uniform values took 152 lane reads per row)
     auto halo_phase = [&](double vr, double vb, double& l, double& r) {
         const int lane = L.lane;
         if (lane < 2 * Q) {
@@ -521,6 +524,15 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     }
 }
 
+template <bool F3>
+__global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
+{
+    __shared__ double hbuf[2][2 * Q];
+    TileCtx t;
+    if (!tile_ctx(launch, t)) return;
+    ring1_tile<F3>(t, hbuf);
+}
+
 // ------------------------------------------------------------------ F1 / F2 (2-ring)
 struct Aux6 { double a[6]; };
 
@@ -567,16 +579,22 @@ __device__ __forceinline__ Aux6 aux_of(const CaseConst& cc, double g0, double g1
 // 2.2x the algorithmic reads).  Every value is computed with the same expressions as before, so the
 // results are unchanged bit for bit.
 template <int OP>
-__global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
+struct Ring2Lds {
+    static constexpr int NA = (OP == OP_F1) ? 3 : 6;
+    double phir[5][68];      // colour-field ring columns: x0 - 2 (0) .. x0 + 65 (67)
+    double ring[3][NA][66];  // n / T ring columns: x0 - 1 (0) .. x0 + 64 (65)
+    double hbuf[2][4 * Q];   // pulled populations of the four halo sites, by colour
+};
+
+template <int OP>
+__device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
 {
-    constexpr int NA = (OP == OP_F1) ? 3 : 6;
-    constexpr int NP = 68;   // colour-field ring columns: x0 - 2 (0) .. x0 + 65 (67)
-    constexpr int NC = 66;   // n / T ring columns: x0 - 1 (0) .. x0 + 64 (65)
-    __shared__ double phir[5][NP];
-    __shared__ double ring[3][NA][NC];
-    __shared__ double hbuf[2][4 * Q];   // pulled populations of the four halo sites, by colour
-    TileCtx t;
-    if (!tile_ctx(launch, t)) return;
+    constexpr int NA = Ring2Lds<OP>::NA;
+    constexpr int NP = 68;
+    constexpr int NC = 66;
+    auto& phir = lds.phir;
+    auto& ring = lds.ring;
+    auto& hbuf = lds.hbuf;
     const CaseConst& cc = t.cc;
     const int nx = cc.nx, ny = cc.ny;
     const Lane L = make_lane_aligned(nx, t.strip);
@@ -728,6 +746,42 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
         g00 = gp0; g01 = gp1; g02 = gp2;
+    }
+}
+
+template <int OP>
+__global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
+{
+    __shared__ Ring2Lds<OP> lds;
+    TileCtx t;
+    if (!tile_ctx(launch, t)) return;
+    ring2_tile<OP>(t, lds);
+}
+
+// All four families in one launch: each wavefront takes the tiles blockIdx.x, blockIdx.x + gridDim.x,
+// ... of a table covering every case (gridDim.x a multiple of 8, so a wavefront's tiles all lie in
+// its XCD's contiguous range), and runs the family kernel of each tile's case.  With a grid of one
+// block per tile every wavefront takes one tile; with a grid of the resident capacity the launch is
+// persistent.  The bodies are the family kernels' own, so the results are the same bit for bit.
+union FusedLds {
+    double ring1[2][2 * Q];
+    Ring2Lds<OP_F1> f1;
+    Ring2Lds<OP_F2> f2;
+};
+
+__global__ void __launch_bounds__(64) step_fused(StepLaunch launch)
+{
+    __shared__ FusedLds lds;
+    for (int vb = blockIdx.x; vb < launch.ntiles; vb += gridDim.x) {
+        TileCtx t;
+        if (!tile_ctx_at(launch, xcd_tile(vb, launch.ntiles), t)) continue;
+        switch (t.kind) {
+        case KIND_P: ring1_tile<false>(t, lds.ring1); break;
+        case KIND_F3: ring1_tile<true>(t, lds.ring1); break;
+        case KIND_F1: ring2_tile<OP_F1>(t, lds.f1); break;
+        default: ring2_tile<OP_F2>(t, lds.f2); break;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // the next tile writes the LDS again
     }
 }
 
@@ -978,6 +1032,27 @@ hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long l
     const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(f64_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, nx, n, S, soa);
     return hipGetLastError();
+}
+
+hipError_t launch_step_f64_fused(const StepLaunch& a, int grid, hipStream_t stream)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(step_fused, dim3(grid), dim3(64), 0, stream, a);
+    return hipGetLastError();
+}
+
+int step_f64_fused_capacity()
+{
+    static int cap = 0;
+    if (cap == 0) {
+        int dev = 0, per_cu = 0;
+        hipDeviceProp_t prop{};
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_fused, 64, 0) != hipSuccess || per_cu < 1)
+            return 0;
+        cap = (prop.multiProcessorCount * per_cu + 7) / 8 * 8;
+    }
+    return cap;
 }
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)

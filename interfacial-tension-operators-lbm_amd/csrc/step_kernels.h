@@ -20,6 +20,7 @@ struct CaseDev {
     int ly;              // rows per wavefront segment
     int nstrip, nseg;    // tiles of the case: strips of 64 - 2H columns x segments of ly rows
     int tile0;           // first tile of the case in the launch
+    int kind;            // kernel family of the case (KIND_*)
 };
 
 struct StepLaunch {
@@ -28,6 +29,7 @@ struct StepLaunch {
     int parity;              // which of pop[0] / pop[1] holds the input state
     int record;              // write aux for cases that have it
     int depth;               // FP32/FP16 kernels: rows in flight ahead (0 = default)
+    int ntiles;              // tiles of the launch (read by the one-launch FP64 kernel)
 };
 
 int step_halo(int op);
@@ -60,6 +62,11 @@ hipError_t launch_fill_static_table(void* base, const CaseDev& d, int ntiles, hi
 hipError_t launch_stream_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
+// every family of an FP64 table in one launch of `grid` blocks (a.ntiles tiles; grid = a.ntiles: a
+// wavefront per tile; grid = step_f64_fused_capacity(): persistent)
+hipError_t launch_step_f64_fused(const StepLaunch& a, int grid, hipStream_t stream);
+// blocks of step_fused resident on the current device at once, a multiple of 8 (0 if unknown)
+int step_f64_fused_capacity();
 
 // One FP64 step of one case with solid sites (step_f64.hip): `in` holds the post-collision state at
 // its own site, `out` receives the next one; phi [S], g [3][S] and nt [6][S] are the case's field

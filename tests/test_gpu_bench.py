@@ -44,6 +44,8 @@ def test_bench_json_line():
     # the tuning steps ran before the timed ones, whatever --warmup is
     for form in d["forms"].values():
         assert form["tile_shapes"] and all(s["tuned"] for s in form["tile_shapes"].values()), form["tile_shapes"]
+    # the launch form was chosen (FP64: by timing; FP32 / FP16: one launch per family)
+    assert d["forms"]["f64"]["launch_form"] in (0, 1, 2) and d["forms"]["f32z"]["launch_form"] == 0
     # FP32 z-symmetric storage tracks FP64 in delta_p after the same ~426 steps (sharp drops, still relaxing)
     agree = d["forms"]["f32z"]["delta_p_vs_f64"]
     assert agree["cases"] == 14 and agree["max_rel"] < 1e-3, agree
