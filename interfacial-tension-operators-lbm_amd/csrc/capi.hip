@@ -921,16 +921,21 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
 // repeated on one buffer pair re-read an input that the last-level cache then partly holds, which
 // favoured the wrong shapes), and all active families run together, as they do in a step, so a
 // family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
-// fixed: the pipeline depth (FP32/FP16 kernels) at the current rows per segment, then the rows per
-// segment (1 .. 64) at that depth, each candidate over W untimed + T timed steps; the two fastest are
-// timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
+// fixed: the pipeline depth (FP32/FP16 kernels; FP64 F1 / F2: rows read again or kept in registers)
+// at the current rows per segment, then the rows per segment (1 .. 64) at that depth, each candidate
+// over W untimed + T timed steps; the two fastest are timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
 // members changed).  The steps used are subtracted from `left`; nothing is done unless `left`
 // covers the whole search.
 int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
 {
     constexpr int W = 1, T = 6;
     constexpr int LYS[] = {1, 2, 4, 8, 16, 32, 64};   // short segments suit small cases (more wavefronts)
-    auto depths_of = [&](const Group& g) { return c->cases[g.members[0]].elem == 8 ? 1 : 3; };
+    // FP32/FP16: rows in flight ahead 1 .. 3; FP64 F1 / F2: 1 reads a row's populations again for its
+    // collision, 2 keeps them in registers (one wavefront per SIMD); other FP64 kernels: one form
+    auto depths_of = [&](const Group& g) {
+        if (c->cases[g.members[0]].elem != 8) return 3;
+        return (g.kind == KIND_F1 || g.kind == KIND_F2) ? 2 : 1;
+    };
     constexpr int COMMON[] = {4, 8, 16, 32, 64};
     long long budget = 0;
     int untuned = 0;

@@ -586,7 +586,10 @@ struct Ring2Lds {
     double hbuf[2][4 * Q];   // pulled populations of the four halo sites, by colour
 };
 
-template <int OP>
+// CACHED: the populations pulled for a row's colour field are kept in registers (a ring of three
+// rows) until the row is collided, two iterations later, instead of being read again; 114 more
+// doubles per lane, so the kernel runs one wavefront per SIMD.
+template <int OP, bool CACHED>
 __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
 {
     constexpr int NA = Ring2Lds<OP>::NA;
@@ -610,8 +613,7 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
     // colour field of row yy into its ring slot: own lanes, and the four halo columns.  The halo sites'
     // 76 pulled values (two loads per colour, value g = 19 s + i in lane g of load A or g - 64 of load B)
     // go through LDS to lanes 0 .. 3, each of which sums one site's values in index order, as dens().
-    auto phi_row = [&](int yy) {
-        double fR[Q], fB[Q];
+    auto phi_row_into = [&](int yy, double (&fR)[Q], double (&fB)[Q]) {
         load_row(t.in, nx, ny, yy, L, fR, fB);
         const unsigned long long e0 = elem_index(nx, wrap_row3(yy, h.dy[0], ny), 0, h.i[0], h.x[0]);
         const unsigned long long e1 = elem_index(nx, wrap_row3(yy, h.dy[1], ny), 0, h.i[1], h.x[1]);
@@ -638,6 +640,10 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
             row[lane < 2 ? lane : NP - 4 + lane] = phase(rR, rB);
         }
         lds_order();   // hbuf is written again by the next row
+    };
+    auto phi_row = [&](int yy) {
+        double fR[Q], fB[Q];
+        phi_row_into(yy, fR, fB);
     };
     // 3 x 3 colour field around ring column c (colour-field ring coordinates) of row yy
     auto phi9 = [&](int yy, int c) {
@@ -668,10 +674,18 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
         }
     };
 
+    // raw rows kept for the collision (CACHED): rows y, y + 1, y + 2 in slots (y - y0) mod 3
+    double rawR[CACHED ? 3 : 1][Q], rawB[CACHED ? 3 : 1][Q];
     // one row of populations in flight at a time (the scheduler would otherwise issue all four rows'
     // loads together and size the kernel's registers for them)
     for (int k = -2; k <= 1; ++k) {
-        phi_row(t.y0 + k);
+        if constexpr (CACHED) {
+            if (k == 0) phi_row_into(t.y0, rawR[0], rawB[0]);
+            else if (k == 1) phi_row_into(t.y0 + 1, rawR[1], rawB[1]);
+            else phi_row(t.y0 + k);
+        } else {
+            phi_row(t.y0 + k);
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
     lds_order();
@@ -679,8 +693,12 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
     nt_row(t.y0 - 1, g00, g01, g02);
     nt_row(t.y0, g00, g01, g02);
 
-    for (int y = t.y0; y < t.y1; ++y) {
-        phi_row(y + 2);
+    // one row: the colour field of row y + 2 (its populations into nR / nB when CACHED), n / T of row
+    // y + 1, then the force, mediator gradient and collision of row y (its populations from cR / cB
+    // when CACHED, read again otherwise)
+    auto row_step = [&](int y, double (&cR)[Q], double (&cB)[Q], double (&nR)[Q], double (&nB)[Q]) {
+        if constexpr (CACHED) phi_row_into(y + 2, nR, nB);
+        else phi_row(y + 2);
         lds_order();
         double gp0, gp1, gp2;
         nt_row(y + 1, gp0, gp1, gp2);
@@ -727,7 +745,12 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
         // reads and hold both at once
         __builtin_amdgcn_sched_barrier(0);
         double fR[Q], fB[Q];
-        load_row(t.in, nx, ny, y, L, fR, fB);
+        if constexpr (CACHED) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) { fR[i] = cR[i]; fB[i] = cB[i]; }
+        } else {
+            load_row(t.in, nx, ny, y, L, fR, fB);
+        }
 
         if (L.interior) {
             const unsigned site = (unsigned)(y * nx + L.col);
@@ -740,12 +763,30 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
                 recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) { store_pop(ro, nx, xoff, i, r, b); });
             } else {   // left untouched (rare): the raw populations again
                 double uR[Q], uB[Q];
-                load_row(t.in, nx, ny, y, L, uR, uB);
+                if constexpr (CACHED) {
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) { uR[i] = cR[i]; uB[i] = cB[i]; }
+                } else {
+                    load_row(t.in, nx, ny, y, L, uR, uB);
+                }
                 store_row(ro, nx, xoff, uR, uB);
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
         g00 = gp0; g01 = gp1; g02 = gp2;
+    };
+
+    if constexpr (CACHED) {
+        // unrolled by three so that every raw-row slot is a fixed register range
+        for (int y = t.y0; y < t.y1; y += 3) {
+            row_step(y, rawR[0], rawB[0], rawR[2], rawB[2]);
+            if (y + 1 >= t.y1) break;
+            row_step(y + 1, rawR[1], rawB[1], rawR[0], rawB[0]);
+            if (y + 2 >= t.y1) break;
+            row_step(y + 2, rawR[2], rawB[2], rawR[1], rawB[1]);
+        }
+    } else {
+        for (int y = t.y0; y < t.y1; ++y) row_step(y, rawR[0], rawB[0], rawR[0], rawB[0]);
     }
 }
 
@@ -755,7 +796,17 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
     __shared__ Ring2Lds<OP> lds;
     TileCtx t;
     if (!tile_ctx(launch, t)) return;
-    ring2_tile<OP>(t, lds);
+    ring2_tile<OP, false>(t, lds);
+}
+
+// the raw rows kept in registers: one wavefront per SIMD (the register file's 512 per lane)
+template <int OP>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) step_ring2c(StepLaunch launch)
+{
+    __shared__ Ring2Lds<OP> lds;
+    TileCtx t;
+    if (!tile_ctx(launch, t)) return;
+    ring2_tile<OP, true>(t, lds);
 }
 
 // All four families in one launch: each wavefront takes the tiles blockIdx.x, blockIdx.x + gridDim.x,
@@ -778,8 +829,8 @@ __global__ void __launch_bounds__(64) step_fused(StepLaunch launch)
         switch (t.kind) {
         case KIND_P: ring1_tile<false>(t, lds.ring1); break;
         case KIND_F3: ring1_tile<true>(t, lds.ring1); break;
-        case KIND_F1: ring2_tile<OP_F1>(t, lds.f1); break;
-        default: ring2_tile<OP_F2>(t, lds.f2); break;
+        case KIND_F1: ring2_tile<OP_F1, false>(t, lds.f1); break;
+        default: ring2_tile<OP_F2, false>(t, lds.f2); break;
         }
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // the next tile writes the LDS again
     }
@@ -1062,8 +1113,14 @@ hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_
     switch (kind) {
     case KIND_P: hipLaunchKernelGGL(step_ring1<false>, grid, block, 0, stream, a); break;
     case KIND_F3: hipLaunchKernelGGL(step_ring1<true>, grid, block, 0, stream, a); break;
-    case KIND_F1: hipLaunchKernelGGL(step_ring2<OP_F1>, grid, block, 0, stream, a); break;
-    case KIND_F2: hipLaunchKernelGGL(step_ring2<OP_F2>, grid, block, 0, stream, a); break;
+    case KIND_F1:
+        if (a.depth >= 2) hipLaunchKernelGGL(step_ring2c<OP_F1>, grid, block, 0, stream, a);
+        else hipLaunchKernelGGL(step_ring2<OP_F1>, grid, block, 0, stream, a);
+        break;
+    case KIND_F2:
+        if (a.depth >= 2) hipLaunchKernelGGL(step_ring2c<OP_F2>, grid, block, 0, stream, a);
+        else hipLaunchKernelGGL(step_ring2<OP_F2>, grid, block, 0, stream, a);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
