@@ -11,7 +11,7 @@ of them, reported as launch_form).  A bench step advances every case by one time
 
 Before anything is timed, 2 s of untimed streaming copies bring the GPU to its sustained clocks.
 Every form first runs --tune-steps untimed steps (default 500, whatever --warmup is): the library
-tunes its tile shapes on the first ~340-400 of them, and the shapes it chose are reported.  Then
+tunes its tile shapes on the first ~360-450 of them, and the shapes it chose are reported.  Then
 --warmup untimed steps, then --steps timed steps.  The library replays runs of 32 steps from a
 captured HIP graph; when --steps is shorter, the graph is sized to --steps and captured on one
 untimed call of that length first, so the timed call replays it as a longer run does.  The headline (--prec, default f64) is the FP64
@@ -40,7 +40,7 @@ Also reported:
   sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
                  numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
                  batched reduced-precision form vs the plain FP64 one-launch-per-case path over
-                 --sweep-steps steps in all (the first 400 untimed), delta_p agreement at the end;
+                 --sweep-steps steps in all (the first 500 untimed), delta_p agreement at the end;
                  FP16 z-symmetric storage batched the same way, for its delta_p agreement
   cpu_baseline   the reference program Imm_OPS (built from the reference's sources by oracle/Makefile)
                  on this host's cores, over a bounded sample of the same workload
@@ -67,7 +67,7 @@ SIGMA = 0.1
 NX = NY = 512
 R = 100.0
 PEAK_HBM_GBS = 8000.0
-SWEEP_WARMUP = 400   # untimed sweep steps (the batched path tunes its tile shapes on them)
+SWEEP_WARMUP = 500   # untimed sweep steps (the batched path tunes its tile shapes on them)
 LARGE_TUNE = 120     # untimed steps of the 8192^2 case (tuning)
 B_LU = {"f64": 608, "f32": 304, "f16": 152, "f32z": 224, "f16z": 112}
 PREC_ID = {"f64": 0, "f32": 1, "f16": 2, "f32z": 3, "f16z": 4}

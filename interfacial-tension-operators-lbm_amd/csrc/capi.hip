@@ -922,8 +922,9 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
 // favoured the wrong shapes), and all active families run together, as they do in a step, so a
 // family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
 // fixed: the pipeline depth (FP32/FP16 kernels; FP64 F1 / F2: rows read again or kept in registers)
-// at the current rows per segment, then the rows per segment (1 .. 64) at that depth, each candidate
-// over W untimed + T timed steps; the two fastest are timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
+// at the current rows per segment, then the rows per segment (1 .. 64) at that depth, then the other
+// depths at the best rows per segment, each candidate over W untimed + T timed steps; the two fastest
+// are timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
 // members changed).  The steps used are subtracted from `left`; nothing is done unless `left`
 // covers the whole search.
 int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
@@ -941,7 +942,7 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
     int untuned = 0;
     for (const Group* g : act)
         if (!g->tuned) {
-            budget += (long long)(depths_of(*g) + 6) * (W + T) + 2 * (W + 2 * T);
+            budget += (long long)(2 * depths_of(*g) + 5) * (W + T) + 2 * (W + 2 * T);
             ++untuned;
         }
     if (untuned > 1) budget += (long long)(sizeof(COMMON) / sizeof(COMMON[0])) * (W + T);
@@ -1030,6 +1031,15 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
             if (int r = time_of(cd, T)) return r;
             tried.push_back(cd);
         }
+        // the other depths again at the best segment length (the best depth can change with it)
+        const int best_ly = std::min_element(tried.begin(), tried.end(), by_time)->ly;
+        if (lp && best_ly != start)
+            for (int d = 1; d <= depths_of(*g); ++d) {
+                if (d == best_depth) continue;
+                Cand cd{best_ly, d, 0.f};
+                if (int r = time_of(cd, T)) return r;
+                tried.push_back(cd);
+            }
         std::sort(tried.begin(), tried.end(), by_time);
         for (int j = 0; j < 2; ++j)
             if (int r = time_of(tried[j], 2 * T)) return r;

@@ -75,8 +75,8 @@ def test_online_tuning_keeps_results(prec):
     """A call long enough for the library to tune its tile shapes on the call's own steps (every
     family's candidate shapes timed in turn) ends in the same state as a fixed shape."""
     m = _pkg.load()
-    say(f"online tuning: precision {prec}, 420 steps")
-    ref = run(m, prec, (8, 2), steps=420)
-    got = run(m, prec, None, steps=420)
+    say(f"online tuning: precision {prec}, 480 steps")
+    ref = run(m, prec, (8, 2), steps=480)
+    got = run(m, prec, None, steps=480)
     for j, (x, y) in enumerate(zip(ref, got)):
         assert np.array_equal(x, y), f"array {j}"
