@@ -33,8 +33,10 @@ struct CaseState {
     lbm_case_desc d{};
     CaseConst cc{};
     long long n = 0, S = 0;
-    double* pop[2] = {nullptr, nullptr};   // FP64: row blocks [y][colour][19][nx]; FP32/FP16: pairs ((y*NS + slot)*nx + x)
+    double* pop[2] = {nullptr, nullptr};   // FP64: row blocks [y][colour][19][nx] or the compact form [y][CSLOTS][nx];
+                                            // FP32/FP16: pairs ((y*NS + slot)*nx + x)
     int cur = 0;                            // buffer holding the latest post-collision state
+    bool compact = false;                   // FP64 compact form (P1/P3 and F3 cases without solid sites)
     double* aux = nullptr;                  // [6][S] gm, F
     bool aux_valid = false;
     int kind = 0;                           // kernel family (KIND_*)
@@ -106,6 +108,7 @@ struct lbm_ctx {
     hipStream_t side[N_KINDS] = {};
     hipEvent_t join[N_KINDS] = {};
     bool autotune = true;
+    bool f64_compact = true;           // FP64 P1/P3 and F3 cases in the compact form (ITOLBM_F64_COMPACT=0: not)
     int graph_steps = 32;              // steps per captured graph in lbm_step (0: no graphs)
     StepGraph graph;
     long long bytes = 0;
@@ -405,7 +408,8 @@ int state_f64(lbm_ctx* c, CaseState& k, const double** out)
         if (int r = dev_alloc(c, (void**)&k.f64, sizeof(double) * 2 * Q * (size_t)k.S)) return r;
     }
     if (k.elem == 8) {
-        HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.d.nx, k.n, k.S, k.f64, c->stream));
+        HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.compact, k.cc.beta, k.d.nx, k.n, k.S, k.f64,
+                                         c->stream));
     } else {
         HIPCHK(c, launch_lp_to_f64(k.elem, k.sym, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
                                    k.f64 + Q * k.S, c->stream));
@@ -456,12 +460,15 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         lbm_destroy(c);
         return LBM_E_ARG;
     }
+    if (const char* e = std::getenv("ITOLBM_F64_COMPACT")) c->f64_compact = std::strcmp(e, "0") != 0;
     c->cases.resize(n_cases);
     for (int i = 0; i < n_cases; ++i) {
         const lbm_case_desc& d = cases[i];
         if (d.nx < 1 || d.ny < 1 || !valid_op(d.op) || !(d.tau > 0.0) || !std::isfinite(d.A) ||
             !std::isfinite(d.beta) || (d.op == OP_F3 && !(d.xi != 0.0)) ||
-            (long long)d.nx * d.ny > (1LL << 31) / Q) {
+            (long long)d.nx * d.ny > (1LL << 31) / Q ||
+            (long long)d.nx * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
+                                                                                       // offsets are 32-bit
             lbm_destroy(c);
             return LBM_E_ARG;
         }
@@ -474,9 +481,11 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.elem = elem;
         k.sym = sym;
         k.nslot = n_slots(sym);
+        k.compact = elem == 8 && c->f64_compact && (k.kind == KIND_P || k.kind == KIND_F3);
         // state buffers hold 2 x nslot values per site in row blocks (the step kernels index them by
-        // nx and ny only), so their size follows from n and not from the padded stride S
-        const size_t pop_bytes = (size_t)elem * 2 * k.nslot * (size_t)k.n;
+        // nx and ny only), so their size follows from n and not from the padded stride S; an FP64
+        // buffer holds either form (F64_SLOTS doubles per site), so a case can change form in place
+        const size_t pop_bytes = (size_t)elem * (elem == 8 ? F64_SLOTS : 2 * k.nslot) * (size_t)k.n;
         for (int b = 0; b < 2; ++b) {
             if (dev_alloc(c, (void**)&k.pop[b], pop_bytes) || hipMemsetAsync(k.pop[b], 0, pop_bytes, c->stream) != hipSuccess) {
                 lbm_destroy(c);
@@ -560,7 +569,23 @@ int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
                 h[(size_t)(Q + i) * k.S + site] = fB[(size_t)src * Q + j];
             }
         }
-    if (k.elem == 8) {
+    if (k.elem == 8 && k.compact) {
+        // compact row blocks [y][CSLOTS][nx], every site raw: f_R,i in slot i, f_B,i in slot cs_raw_b(i)
+        std::vector<double> rows((size_t)CSLOTS * k.n, 0.0);
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t p = (size_t)y * nx + x;
+                double* blk = rows.data() + (size_t)y * CSLOTS * nx + x;
+                for (int i = 0; i < Q; ++i) {
+                    blk[(size_t)i * nx] = h[(size_t)i * k.S + p];
+                    blk[(size_t)cs_raw_b(i) * nx] = h[(size_t)(Q + i) * k.S + p];
+                }
+                blk[(size_t)CS_RHO * nx] = CS_RAW;
+            }
+        HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else if (k.elem == 8) {
         // row blocks [y][colour][19][nx]
         std::vector<double> rows(2 * Q * (size_t)k.n);
         for (int y = 0; y < ny; ++y)
@@ -844,8 +869,9 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
     a.depth = g.depth;
     const CaseState& k0 = c->cases[g.members[0]];   // a context has one storage form
     a.ntiles = g.ntiles;
+    a.compact = (k0.elem == 8 && c->f64_compact) ? 1 : 0;
     if (g.kind == KIND_FUSED) {   // every strip family of the FP64 context in one launch
-        const int cap = step_f64_fused_capacity();
+        const int cap = step_f64_fused_capacity(a.compact != 0);
         const int grid = (c->fuse_mode == 2 && cap > 0) ? std::min(cap, (g.ntiles + 7) / 8 * 8) : g.ntiles;
         HIPCHK(c, launch_step_f64_fused(a, grid, st));
         return LBM_OK;
@@ -1255,6 +1281,7 @@ int lbm_step_case_noalloc(lbm_ctx* c, int id, int n_steps, int flags)
     a.tab = scratch_table(c->scratch);
     a.tile_case = scratch_tiles(c->scratch);
     a.depth = g.depth;
+    a.compact = k.compact ? 1 : 0;
     for (int s = 0; s < n_steps; ++s) {
         a.parity = s & 1;
         a.record = (rec && s == n_steps - 1) ? 1 : 0;
@@ -1448,6 +1475,7 @@ int lbm_set_geometry(lbm_ctx* c, int id, const int* meio, double wett_R)
         // the case changes kernel family: both families' members and tables are redone
         const int old = k.kind;
         k.kind = kind;
+        k.compact = c->f64_compact && (kind == KIND_P || kind == KIND_F3);
         for (int kd : {old, kind}) {
             Group& g = c->family[kd];
             g.members.clear();

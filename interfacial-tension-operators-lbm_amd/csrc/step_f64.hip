@@ -533,6 +533,437 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
     ring1_tile<F3>(t, hbuf);
 }
 
+// ------------------------------------------------------------------ compact form (P1/P3, F3)
+// Recolouring (LBM_functions.cpp:1712-1746) forms a collided site's 38 outgoing populations as
+//   f_R,i = cR f_i - t_k,   f_B,i = cB f_i + t_k      (-t_k for the second velocity of pair k)
+// with cB = 1 - cR and t_k = (beta cR cB w_k rho) (c_k . gm) / (|gm| |c_k|): functions of the site's 19
+// totals f_i and of five scalars, cR, rho and gm.  The compact form stores those 24 doubles per site
+// (layout in step_kernels.h), and the site that pulls population i forms f_R,i and f_B,i from its
+// sender's stored values with the sender's own expressions, so the values are the ones the sender
+// would have stored, bit for bit, and a step moves 384 instead of 608 bytes per lattice update.
+//
+// A lane forms the terms t_k of its own column's sites once per row; the x - 1 and x + 1 neighbours'
+// terms come from the neighbouring lanes by DPP (whole-wavefront shifts by one lane), and those of
+// the two columns beside the strip from two halo lanes (x0 - 1 in lane 0, x0 + 64 in lane 63).  A
+// row with a raw site among its senders is rebuilt population by population from loads (rare: the
+// first step after an upload, or a site with rho <= 1e-30).
+__device__ __forceinline__ long long cblock_doubles(int nx) { return (long long)CSLOTS * nx; }
+__device__ __forceinline__ unsigned cblock_bytes(int nx) { return (unsigned)(cblock_doubles(nx) * (long long)sizeof(double)); }
+__device__ __forceinline__ unsigned cslot_off(int slot, int nx) { return (unsigned)(slot * nx) * (unsigned)sizeof(double); }
+
+// t_k of recolour() for pair k = 0 .. 8, the pairs (1,2) (3,4) (5,6) (7,8) (9,10) (11,12) (13,14) (16,15)
+// (18,17): the axis pairs with fw1 and |gm|, the diagonal ones with fw2 and |gm| sqrt 2
+template <int K>
+__device__ __forceinline__ double pair_t(double fw1, double fw2, double gx, double gy, double gz, double mod,
+                                         double mod_d)
+{
+    double prod;
+    if constexpr (K == 0) prod = gx;
+    else if constexpr (K == 1) prod = gy;
+    else if constexpr (K == 2) prod = gz;
+    else if constexpr (K == 3) prod = gx + gy;
+    else if constexpr (K == 4) prod = gx - gy;
+    else if constexpr (K == 5) prod = gx + gz;
+    else if constexpr (K == 6) prod = gx - gz;
+    else if constexpr (K == 7) prod = gy + gz;
+    else prod = gy - gz;
+    const double cosp = (mod != 0.0) ? prod / (K < 3 ? mod : mod_d) : 0.0;
+    return (K < 3 ? fw1 : fw2) * cosp;
+}
+
+// recolouring terms of a stored site, formed as its recolour() formed them
+struct Rc {
+    double cR, cB;
+    double t[9];
+};
+
+__device__ __forceinline__ Rc rc_of(double beta, double cR, double rho, double gx, double gy, double gz)
+{
+    Rc r;
+    r.cR = cR;
+    r.cB = 1.0 - cR;
+    const double fator = beta * cR * r.cB;
+    const double fw1 = fator * W1 * rho, fw2 = fator * W2 * rho;
+    const double mod = sqrt(gx * gx + gy * gy + gz * gz);
+    const double mod_d = mod * SQRT2;
+    r.t[0] = pair_t<0>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[1] = pair_t<1>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[2] = pair_t<2>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[3] = pair_t<3>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[4] = pair_t<4>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[5] = pair_t<5>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[6] = pair_t<6>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[7] = pair_t<7>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    r.t[8] = pair_t<8>(fw1, fw2, gx, gy, gz, mod, mod_d);
+    return r;
+}
+
+// first velocity of its recolouring pair (t_k), or the second (-t_k)
+__host__ __device__ constexpr bool pair_first(int i) { return i <= 14 ? (i & 1) != 0 : (i & 1) == 0; }
+
+// f_R,i and f_B,i of velocity I from the sender's total f and terms, as recolour() forms them
+template <int I>
+__device__ __forceinline__ void csplit(double f, double cR, double cB, double t, double& r, double& b)
+{
+    if constexpr (I == 0) {
+        r = cR * f;
+        b = cB * f;
+    } else if constexpr (pair_first(I)) {
+        r = cR * f - t;
+        b = cB * f + t;
+    } else {
+        r = cR * f - (-t);
+        b = cB * f + (-t);
+    }
+}
+
+// One stored site read for population i (run-time, per lane): its total and scalars.
+struct CPop {
+    double f, cR, rho, gx, gy, gz;
+};
+
+__device__ __forceinline__ CPop cpop_load(gcdouble* __restrict__ in, long long e0, int nx, int i)
+{
+    CPop p;
+    p.f = in[e0 + (long long)i * nx];
+    p.cR = in[e0 + (long long)CS_CR * nx];
+    p.rho = in[e0 + (long long)CS_RHO * nx];
+    p.gx = in[e0 + (long long)CS_G * nx];
+    p.gy = in[e0 + (long long)(CS_G + 1) * nx];
+    p.gz = in[e0 + (long long)(CS_G + 2) * nx];
+    return p;
+}
+
+// t_k of pair k chosen at run time (per lane): the operations of rc_of / pair_t for that pair
+__device__ __forceinline__ double pair_t_rt(int k, double beta, double cR, double rho, double gx, double gy, double gz)
+{
+    const double cB = 1.0 - cR;
+    const double fator = beta * cR * cB;
+    const double fw = fator * (k < 3 ? W1 : W2) * rho;
+    const double mod = sqrt(gx * gx + gy * gy + gz * gz);
+    const double den = (k < 3) ? mod : mod * SQRT2;
+    const int base = (k <= 6) ? 2 * k + 1 : 2 * k + 2;   // first velocity of pair k
+    const double prod = cdot_rt(base, gx, gy, gz);
+    return fw * ((mod != 0.0) ? prod / den : 0.0);
+}
+
+// f_R,i and f_B,i of the site whose slot 0 is element e0 (velocity i chosen at run time), or its stored
+// values when it is raw
+__device__ __forceinline__ void cpop_split(gcdouble* __restrict__ in, long long e0, int nx, double beta, int i,
+                                           const CPop& p, double& r, double& b)
+{
+    if (p.rho <= 1.0e-30) {
+        r = p.f;
+        b = in[e0 + (long long)cs_raw_b(i) * nx];
+        return;
+    }
+    const double cR = p.cR, cB = 1.0 - cR;
+    if (i == 0) {
+        r = cR * p.f;
+        b = cB * p.f;
+        return;
+    }
+    const double t = pair_t_rt((i - 1) >> 1, beta, cR, p.rho, p.gx, p.gy, p.gz);
+    if (pair_first(i)) {
+        r = cR * p.f - t;
+        b = cB * p.f + t;
+    } else {
+        r = cR * p.f - (-t);
+        b = cB * p.f + (-t);
+    }
+}
+
+// element of slot 0 of the site that site (col, Y) pulls velocity i from
+__device__ __forceinline__ long long csender(int nx, int ny, const Lane& L, int Y, int i)
+{
+    const int xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
+    return (long long)wrapi(Y - CY[i], ny) * cblock_doubles(nx) + xs;
+}
+
+// the 38 populations pulled by site (col, Y), each from its sender's stored values (any sender raw)
+__device__ __forceinline__ void cpull_site(gcdouble* __restrict__ in, int nx, int ny, double beta, const Lane& L,
+                                           int Y, double fR[Q], double fB[Q])
+{
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const long long e0 = csender(nx, ny, L, Y, i);
+        const CPop p = cpop_load(in, e0, nx, i);
+        cpop_split(in, e0, nx, beta, i, p, fR[i], fB[i]);
+    }
+}
+
+// site_sums() given f_i = fR_i + fB_i and the colour sums rR, rB already formed
+__device__ __forceinline__ void site_sums_f(Site& s, const double f[Q], double rR, double rB)
+{
+    s.rho = rR + rB;
+    s.skip = (s.rho <= 1.0e-30);
+    s.cR = rR / s.rho;
+    s.cB = 1.0 - s.cR;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) s.f[i] = f[i];
+    double rho2 = f[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) rho2 = rho2 + f[i];
+    double mx = f[1] - f[2];
+    mx = mx + f[7]; mx = mx - f[8]; mx = mx + f[9]; mx = mx - f[10];
+    mx = mx + f[11]; mx = mx - f[12]; mx = mx + f[13]; mx = mx - f[14];
+    double my = f[3] - f[4];
+    my = my + f[7]; my = my - f[8]; my = my - f[9]; my = my + f[10];
+    my = my - f[15]; my = my + f[16]; my = my - f[17]; my = my + f[18];
+    double mz = f[5] - f[6];
+    mz = mz + f[11]; mz = mz - f[12]; mz = mz - f[13]; mz = mz + f[14];
+    mz = mz - f[15]; mz = mz + f[16]; mz = mz + f[17]; mz = mz - f[18];
+    s.rho2 = rho2; s.mx = mx; s.my = my; s.mz = mz;
+}
+
+// The recolouring terms of the three stored rows a pulled row reads (y - 1, y, y + 1), in an LDS ring:
+// field 0 cR, fields 1 + k the t_k, columns x0 - 1 (0) .. x0 + 64 (65).  Each lane writes its own
+// column's terms once per stored row and reads its own and its x-neighbours' for every population it
+// pulls; the two halo columns' terms (the pairs with c_x != 0 only) are formed by lanes 0 - 9, one pair
+// per lane.
+struct Ring1cLds {
+    double rc[3][10][66];
+    double hbuf[2][2 * Q];
+};
+constexpr int HALO_PAIRS[5] = {0, 3, 4, 5, 6};   // pairs (1,2) (7,8) (9,10) (11,12) (13,14)
+
+template <bool F3>
+__device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
+{
+    const CaseConst& cc = t.cc;
+    const int nx = cc.nx, ny = cc.ny;
+    const Lane L = make_lane_aligned(nx, t.strip);
+    const Halo1 h = make_halo1(nx, t.strip);
+    const int lane = L.lane;
+    const long long rb = cblock_doubles(nx);
+    const unsigned bytes = cblock_bytes(nx);
+    const unsigned ooff = (unsigned)L.col * 8u;
+    // the halo columns' scalars: lanes 0 - 4 read column x0 - 1, lanes 5 - 9 column x0 + 64; the other
+    // lanes read past the buffer resource's extent, which returns 0 without a memory access
+    const bool hlane = lane < 10;
+    const int hk = HALO_PAIRS[lane % 5];
+    const int hcol = lane < 5 ? 0 : 65;
+    const unsigned hoff = hlane ? (unsigned)wrapi(t.strip * 64 + (lane < 5 ? -1 : 64), nx) * 8u : 0x80000000u;
+    auto slot = [&](int r) { return (r - t.y0 + 2) % 3; };
+    auto lds_order = [] { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); };
+
+    struct Scal {
+        double v[5];   // cR, rho, gx, gy, gz
+    };
+    auto scal_load = [&](int r, Scal& o, Scal& hs) {
+        const rsrc_t rr = make_rsrc(t.in + wrapi(r, ny) * rb, bytes);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            o.v[q] = buf_load<double>(rr, ooff, cslot_off(CS_CR + q, nx));
+            hs.v[q] = buf_load<double>(rr, hoff, cslot_off(CS_CR + q, nx));
+        }
+    };
+    // terms of stored row r into the ring; returns whether a site of the row (strip or halo columns) is raw
+    auto rc_write = [&](int r, const Scal& o, const Scal& hs) {
+        const Rc a = rc_of(cc.beta, o.v[0], o.v[1], o.v[2], o.v[3], o.v[4]);
+        auto& R = lds.rc[slot(r)];
+        R[0][lane + 1] = a.cR;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R[1 + k][lane + 1] = a.t[k];
+        if (hlane) {
+            R[1 + hk][hcol] = pair_t_rt(hk, cc.beta, hs.v[0], hs.v[1], hs.v[2], hs.v[3], hs.v[4]);
+            R[0][hcol] = hs.v[0];
+        }
+        return __ballot(o.v[1] <= 1.0e-30 || (hlane && hs.v[1] <= 1.0e-30)) != 0;
+    };
+    // pulled totals of row Y (the x-shifted loads of load_row, one value per velocity)
+    auto load_tot = [&](int Y, double (&f)[Q]) {
+        const rsrc_t rm = make_rsrc(t.in + wrapi(Y - 1, ny) * rb, bytes);
+        const rsrc_t r0 = make_rsrc(t.in + wrapi(Y, ny) * rb, bytes);
+        const rsrc_t rp = make_rsrc(t.in + wrapi(Y + 1, ny) * rb, bytes);
+        const unsigned om = (unsigned)L.xm * 8u, op = (unsigned)L.xp * 8u;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : ooff);
+            const rsrc_t r = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
+            f[i] = buf_load<double>(r, xs, cslot_off(i, nx));
+        }
+    };
+    // Row Y's pulled populations, formed in index order from the totals and the ring (or, when a sender
+    // is raw, from loads): their totals f_i = fR_i + fB_i and the colour sums rR, rB, as dens() sums them
+    auto pull_sums = [&](int Y, const double (&tot)[Q], bool raw, double (&f)[Q], double& rR, double& rB) {
+        const auto& Rm = lds.rc[slot(Y - 1)];
+        const auto& R0 = lds.rc[slot(Y)];
+        const auto& Rp = lds.rc[slot(Y + 1)];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            double r, b;
+            if (!raw) {
+                const auto& Rs = CY[i] > 0 ? Rm : (CY[i] < 0 ? Rp : R0);
+                const int c = lane + 1 - CX[i];
+                const double cR = Rs[0][c];
+                const double tk = (i == 0) ? 0.0 : Rs[1 + ((i - 1) >> 1)][c];
+                switch (i) {   // csplit needs the velocity as a constant
+#define ITOLBM_CS(n) case n: csplit<n>(tot[i], cR, 1.0 - cR, tk, r, b); break;
+                ITOLBM_CS(0) ITOLBM_CS(1) ITOLBM_CS(2) ITOLBM_CS(3) ITOLBM_CS(4) ITOLBM_CS(5) ITOLBM_CS(6)
+                ITOLBM_CS(7) ITOLBM_CS(8) ITOLBM_CS(9) ITOLBM_CS(10) ITOLBM_CS(11) ITOLBM_CS(12) ITOLBM_CS(13)
+                ITOLBM_CS(14) ITOLBM_CS(15) ITOLBM_CS(16) ITOLBM_CS(17) default: csplit<18>(tot[i], cR, 1.0 - cR, tk, r, b);
+#undef ITOLBM_CS
+                }
+            } else {
+                const long long e0 = csender(nx, ny, L, Y, i);
+                const CPop p = cpop_load(t.in, e0, nx, i);
+                cpop_split(t.in, e0, nx, cc.beta, i, p, r, b);
+            }
+            if (i == 0) {
+                rR = r;
+                rB = b;
+            } else {
+                rR = rR + r;
+                rB = rB + b;
+            }
+            f[i] = r + b;
+        }
+    };
+    // the halo columns' colour field: lane j < 38 pulls velocity h.i of halo site j / 19 from its
+    // sender's stored values; the pairs go through LDS to lanes 0 and 63, which sum them in index order
+    auto halo_e0 = [&](int Y) { return (long long)wrap_row3(Y, h.dy, ny) * rb + h.x; };
+    auto halo_phase = [&](int Y, const CPop& hp, double& l, double& r) {
+        double vr, vb;
+        cpop_split(t.in, halo_e0(Y), nx, cc.beta, h.i, hp, vr, vb);
+        if (lane < 2 * Q) {
+            lds.hbuf[0][lane] = vr;
+            lds.hbuf[1][lane] = vb;
+        }
+        lds_order();
+        double v = 0.0;
+        if (lane == 0 || lane == 63) {
+            const double* pr = lds.hbuf[0] + (lane ? Q : 0);
+            const double* pb = lds.hbuf[1] + (lane ? Q : 0);
+            double rR = pr[0], rB = pb[0];
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                rR = rR + pr[i];
+                rB = rB + pb[i];
+            }
+            v = phase(rR, rB);
+        }
+        lds_order();   // hbuf is written again by the next row
+        l = v;
+        r = v;
+    };
+
+    Scal sc, hsc;
+    double tot[Q], fn[Q];
+    double rRn, rBn;
+    double ph[3], hl[3], hr[3];   // colour field of rows y-1, y, y+1 and of the two halo columns
+    CPop hp;
+    // terms of the stored rows y0 - 2 .. y0 (the senders of pulled row y0 - 1)
+    scal_load(t.y0 - 2, sc, hsc);
+    bool Wm = rc_write(t.y0 - 2, sc, hsc);
+    scal_load(t.y0 - 1, sc, hsc);
+    bool W0 = rc_write(t.y0 - 1, sc, hsc);
+    scal_load(t.y0, sc, hsc);
+    bool Wp = rc_write(t.y0, sc, hsc);
+    lds_order();
+    load_tot(t.y0 - 1, tot);
+    hp = cpop_load(t.in, halo_e0(t.y0 - 1), nx, h.i);
+    pull_sums(t.y0 - 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
+    ph[0] = phase(rRn, rBn);
+    halo_phase(t.y0 - 1, hp, hl[0], hr[0]);
+    // pulled row y0 (senders in stored rows y0 - 1 .. y0 + 1)
+    scal_load(t.y0 + 1, sc, hsc);
+    Wm = W0; W0 = Wp;
+    lds_order();   // the ring slot of stored row y0 - 2 is written again
+    Wp = rc_write(t.y0 + 1, sc, hsc);
+    lds_order();
+    load_tot(t.y0, tot);
+    hp = cpop_load(t.in, halo_e0(t.y0), nx, h.i);
+    pull_sums(t.y0, tot, Wm || W0 || Wp, fn, rRn, rBn);
+    ph[1] = phase(rRn, rBn);
+    halo_phase(t.y0, hp, hl[1], hr[1]);
+
+    for (int y = t.y0; y < t.y1; ++y) {
+        // the next row's loads go out first (its pulled totals, the scalars of stored row y + 2 and the
+        // halo pulls); the gradient-independent part of this row's collision runs while they are in flight
+        load_tot(y + 1, tot);
+        scal_load(y + 2, sc, hsc);
+        hp = cpop_load(t.in, halo_e0(y + 1), nx, h.i);
+        Site s;
+        site_sums_f(s, fn, rRn, rBn);
+        if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
+
+        Wm = W0; W0 = Wp;
+        lds_order();   // the ring slot of stored row y - 1 is written again
+        Wp = rc_write(y + 2, sc, hsc);
+        lds_order();
+        pull_sums(y + 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
+        ph[2] = phase(rRn, rBn);
+        halo_phase(y + 1, hp, hl[2], hr[2]);
+        const Nine p9 = gather9h(ph, hl, hr, L.lane);
+        double gx, gy, gz;
+        mediator_gradient(p9, gx, gy, gz);
+        const double mod2 = gx * gx + gy * gy + gz * gz;
+        const double mod = sqrt(mod2);
+
+        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+        if constexpr (F3) {
+            // mu grad phi (Operadores_tensao.cpp:273-287, 325-330)
+            double g0, g1, g2;
+            phase_gradient(p9, g0, g1, g2);
+            const double phc = ph[1];
+            double lap = 0.0;
+#define ITOLBM_LAP(i) lap += (i < 7 ? W1 : W2) * (p9.at(CX[i], CY[i]) - phc);
+            ITOLBM_LAP(1) ITOLBM_LAP(2) ITOLBM_LAP(3) ITOLBM_LAP(4) ITOLBM_LAP(5) ITOLBM_LAP(6)
+            ITOLBM_LAP(7) ITOLBM_LAP(8) ITOLBM_LAP(9) ITOLBM_LAP(10) ITOLBM_LAP(11) ITOLBM_LAP(12)
+            ITOLBM_LAP(13) ITOLBM_LAP(14) ITOLBM_LAP(15) ITOLBM_LAP(16) ITOLBM_LAP(17) ITOLBM_LAP(18)
+#undef ITOLBM_LAP
+            lap *= 6.0;
+            const double mu = 4.0 * cc.bf * phc * (phc * phc - 1.0) - cc.kf * lap;
+            Fx = mu * g0; Fy = mu * g1; Fz = mu * g2;
+            bgk<true>(cc, s, Fx, Fy, Fz);
+        } else {
+            perturb(cc, s.f, gx, gy, gz, mod2, mod);
+        }
+
+        if (L.interior) {
+            const unsigned site = (unsigned)(y * nx + L.col);
+            const rsrc_t ro = make_rsrc(t.out + (long long)y * rb, bytes);
+            if (!s.skip) {
+                // the totals and the scalars the recolouring is formed from
+#pragma unroll
+                for (int i = 0; i < Q; ++i) buf_store<double>(s.f[i], ro, ooff, cslot_off(i, nx));
+                buf_store<double>(s.cR, ro, ooff, cslot_off(CS_CR, nx));
+                buf_store<double>(s.rho, ro, ooff, cslot_off(CS_RHO, nx));
+                buf_store<double>(gx, ro, ooff, cslot_off(CS_G, nx));
+                buf_store<double>(gy, ro, ooff, cslot_off(CS_G + 1, nx));
+                buf_store<double>(gz, ro, ooff, cslot_off(CS_G + 2, nx));
+            } else {   // left untouched (rare): the pulled populations, raw
+                // (a rolled loop: unrolled, its loads took the registers of a second row)
+#pragma unroll 1
+                for (int i = 0; i < Q; ++i) {
+                    const int xs = cx_of(i) > 0 ? L.xm : (cx_of(i) < 0 ? L.xp : L.col);
+                    const long long e0 = (long long)wrapi(y - cy_of(i), ny) * rb + xs;
+                    const CPop p = cpop_load(t.in, e0, nx, i);
+                    double r, b;
+                    cpop_split(t.in, e0, nx, cc.beta, i, p, r, b);
+                    buf_store<double>(r, ro, ooff, cslot_off(i, nx));
+                    buf_store<double>(b, ro, ooff, cslot_off(cs_raw_b(i), nx));
+                }
+                buf_store<double>(CS_RAW, ro, ooff, cslot_off(CS_RHO, nx));
+            }
+            if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
+        }
+        ph[0] = ph[1]; ph[1] = ph[2];
+        hl[0] = hl[1]; hl[1] = hl[2];
+        hr[0] = hr[1]; hr[1] = hr[2];
+    }
+}
+
+template <bool F3>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) step_ring1c(StepLaunch launch)
+{
+    __shared__ Ring1cLds lds;
+    TileCtx t;
+    if (!tile_ctx(launch, t)) return;
+    ring1c_tile<F3>(t, lds);
+}
+
 // ------------------------------------------------------------------ F1 / F2 (2-ring)
 struct Aux6 { double a[6]; };
 
@@ -816,10 +1247,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 // persistent.  The bodies are the family kernels' own, so the results are the same bit for bit.
 union FusedLds {
     double ring1[2][2 * Q];
+    Ring1cLds ring1c;
     Ring2Lds<OP_F1> f1;
     Ring2Lds<OP_F2> f2;
 };
 
+// CMP: the P1/P3 and F3 cases hold the compact form
+template <bool CMP>
 __global__ void __launch_bounds__(64) step_fused(StepLaunch launch)
 {
     __shared__ FusedLds lds;
@@ -827,8 +1261,14 @@ __global__ void __launch_bounds__(64) step_fused(StepLaunch launch)
         TileCtx t;
         if (!tile_ctx_at(launch, xcd_tile(vb, launch.ntiles), t)) continue;
         switch (t.kind) {
-        case KIND_P: ring1_tile<false>(t, lds.ring1); break;
-        case KIND_F3: ring1_tile<true>(t, lds.ring1); break;
+        case KIND_P:
+            if constexpr (CMP) ring1c_tile<false>(t, lds.ring1c);
+            else ring1_tile<false>(t, lds.ring1);
+            break;
+        case KIND_F3:
+            if constexpr (CMP) ring1c_tile<true>(t, lds.ring1c);
+            else ring1_tile<true>(t, lds.ring1);
+            break;
         case KIND_F1: ring2_tile<OP_F1, false>(t, lds.f1); break;
         default: ring2_tile<OP_F2, false>(t, lds.f2); break;
         }
@@ -1076,34 +1516,58 @@ __global__ void __launch_bounds__(256) f64_rows_to_soa(const double* __restrict_
         for (int j = 0; j < 2 * Q; ++j) soa[j * S + p] = row[(long long)j * nx];
     }
 }
+
+// the compact form -> [colour][19][S]: each site's 38 populations formed from its own stored values
+__global__ void __launch_bounds__(256) f64c_rows_to_soa(const double* __restrict__ pop, double beta, int nx, long long n,
+                                                        long long S, double* __restrict__ soa)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long y = p / nx, x = p - y * nx;
+        const long long e0 = y * cblock_doubles(nx) + x;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const CPop c = cpop_load((gcdouble*)pop, e0, nx, i);
+            double r, b;
+            cpop_split((gcdouble*)pop, e0, nx, beta, i, c, r, b);
+            soa[i * S + p] = r;
+            soa[(Q + i) * S + p] = b;
+        }
+    }
+}
 }  // namespace
 
-hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream)
+hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, long long n, long long S,
+                                  double* soa, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(f64_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, nx, n, S, soa);
+    if (compact) hipLaunchKernelGGL(f64c_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, beta, nx, n, S, soa);
+    else hipLaunchKernelGGL(f64_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, nx, n, S, soa);
     return hipGetLastError();
 }
 
 hipError_t launch_step_f64_fused(const StepLaunch& a, int grid, hipStream_t stream)
 {
     if (a.ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(step_fused, dim3(grid), dim3(64), 0, stream, a);
+    if (a.compact) hipLaunchKernelGGL(step_fused<true>, dim3(grid), dim3(64), 0, stream, a);
+    else hipLaunchKernelGGL(step_fused<false>, dim3(grid), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 
-int step_f64_fused_capacity()
+int step_f64_fused_capacity(bool compact)
 {
-    static int cap = 0;
-    if (cap == 0) {
+    static int cap[2] = {0, 0};
+    int& c = cap[compact ? 1 : 0];
+    if (c == 0) {
         int dev = 0, per_cu = 0;
         hipDeviceProp_t prop{};
-        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_fused, 64, 0) != hipSuccess || per_cu < 1)
+        const hipError_t e = compact ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_fused<true>, 64, 0)
+                                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_fused<false>, 64, 0);
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess || e != hipSuccess ||
+            per_cu < 1)
             return 0;
-        cap = (prop.multiProcessorCount * per_cu + 7) / 8 * 8;
+        c = (prop.multiProcessorCount * per_cu + 7) / 8 * 8;
     }
-    return cap;
+    return c;
 }
 
 hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_t stream)
@@ -1111,8 +1575,14 @@ hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_
     if (ntiles <= 0) return hipSuccess;
     const dim3 grid(ntiles), block(64);
     switch (kind) {
-    case KIND_P: hipLaunchKernelGGL(step_ring1<false>, grid, block, 0, stream, a); break;
-    case KIND_F3: hipLaunchKernelGGL(step_ring1<true>, grid, block, 0, stream, a); break;
+    case KIND_P:
+        if (a.compact) hipLaunchKernelGGL(step_ring1c<false>, grid, block, 0, stream, a);
+        else hipLaunchKernelGGL(step_ring1<false>, grid, block, 0, stream, a);
+        break;
+    case KIND_F3:
+        if (a.compact) hipLaunchKernelGGL(step_ring1c<true>, grid, block, 0, stream, a);
+        else hipLaunchKernelGGL(step_ring1<true>, grid, block, 0, stream, a);
+        break;
     case KIND_F1:
         if (a.depth >= 2) hipLaunchKernelGGL(step_ring2c<OP_F1>, grid, block, 0, stream, a);
         else hipLaunchKernelGGL(step_ring2<OP_F1>, grid, block, 0, stream, a);

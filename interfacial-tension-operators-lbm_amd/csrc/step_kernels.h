@@ -30,7 +30,21 @@ struct StepLaunch {
     int record;              // write aux for cases that have it
     int depth;               // FP32/FP16 kernels: rows in flight ahead (0 = default)
     int ntiles;              // tiles of the launch (read by the one-launch FP64 kernel)
+    int compact;             // FP64: the P1/P3 and F3 cases hold the compact form (below)
 };
+
+// Compact FP64 form (P1/P3 and F3 cases): row blocks [y][CSLOTS][nx] of doubles.  A collided site
+// stores its 19 post-collision totals f_i in slots 0..18 and the scalars its recolouring was formed
+// from in slots CS_CR (colour fraction cR), CS_RHO (density), CS_G .. CS_G + 2 (mediator gradient);
+// slots 24..38 are unused.  A raw site (one the collision left untouched, or an uploaded state) holds
+// f_R,i in slot i, f_B,i in slot cs_raw_b(i) and CS_RAW in slot CS_RHO (a collided site's density is
+// > 1e-30 or NaN, never <= 1e-30).
+constexpr int CSLOTS = 39;
+constexpr int CS_CR = 19, CS_RHO = 20, CS_G = 21;
+constexpr double CS_RAW = -1.0;
+__host__ __device__ constexpr int cs_raw_b(int i) { return i == 0 ? 19 : 20 + i; }
+// doubles per site of an FP64 state buffer (either form)
+constexpr int F64_SLOTS = CSLOTS;
 
 int step_halo(int op);
 // kernel family of an operator (OpId)
@@ -65,8 +79,9 @@ hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_
 // every family of an FP64 table in one launch of `grid` blocks (a.ntiles tiles; grid = a.ntiles: a
 // wavefront per tile; grid = step_f64_fused_capacity(): persistent)
 hipError_t launch_step_f64_fused(const StepLaunch& a, int grid, hipStream_t stream);
-// blocks of step_fused resident on the current device at once, a multiple of 8 (0 if unknown)
-int step_f64_fused_capacity();
+// blocks of step_fused resident on the current device at once, a multiple of 8 (0 if unknown), for
+// the compact or the full form of the P1/P3 and F3 cases
+int step_f64_fused_capacity(bool compact);
 
 // One FP64 step of one case with solid sites (step_f64.hip): `in` holds the post-collision state at
 // its own site, `out` receives the next one; phi [S], g [3][S] and nt [6][S] are the case's field
@@ -85,8 +100,10 @@ struct GeoLaunch {
 };
 hipError_t launch_step_geo(int kind, const GeoLaunch& a, hipStream_t stream);
 
-// FP64 state in row blocks [y][colour][19][nx] -> [colour][19][S]
-hipError_t launch_f64_rows_to_soa(const double* pop, int nx, long long n, long long S, double* soa, hipStream_t stream);
+// FP64 state in row blocks [y][colour][19][nx], or in the compact form [y][CSLOTS][nx] (recolouring
+// factor beta), -> [colour][19][S]: every site's post-collision populations at the site
+hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, long long n, long long S,
+                                  double* soa, hipStream_t stream);
 
 // FP32-compute forms; storage FP32 (elem = 4) or FP16 (elem = 2): per population the pair
 // (s_i / w_i, e_i), s_i = fR_i + fB_i - w_i, e_i = (fR_i - fB_i) / (fR_i + fB_i), at ((y * NS + slot) nx + x)
