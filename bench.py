@@ -69,9 +69,12 @@ R = 100.0
 PEAK_HBM_GBS = 8000.0
 SWEEP_WARMUP = 500   # untimed sweep steps (the batched path tunes its tile shapes on them)
 LARGE_TUNE = 120     # untimed steps of the 8192^2 case (tuning)
-B_LU = {"f64": 608, "f32": 304, "f16": 152, "f32z": 224, "f16z": 112}
+# storage forms; the bytes a step moves per lattice update come from the library for each case
+# (lbm_stored_bytes): FP64 384 (the compact form of the P1/P3 and F3 cases) or 608, FP32 304, FP32
+# z-symmetric 224, FP16 152, FP16 z-symmetric 112
+FORMS = ("f64", "f32", "f16", "f32z", "f16z")
 PREC_ID = {"f64": 0, "f32": 1, "f16": 2, "f32z": 3, "f16z": 4}
-KERNEL = {"f64": "step_ring1<false> (P1/P3 family)", "f32": "lp_pipe<float,P,depth,false> (P1/P3 family)",
+KERNEL = {"f64": "step_ring1c<false> (P1/P3 family, compact FP64 form)", "f32": "lp_pipe<float,P,depth,false> (P1/P3 family)",
           "f16": "lp_pipe<_Float16,P,depth,false> (P1/P3 family)",
           "f32z": "lp_pipe<float,P,depth,true> (P1/P3 family)",
           "f16z": "lp_pipe<_Float16,P,depth,true> (P1/P3 family)"}
@@ -177,6 +180,9 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     dp = [ctx.diag_drop(0.0, total - 1, case=i).delta_p for i in range(len(cases))]
     n_sites = NX * NY * len(cases)
     mode = ctx.step_mode()   # the launch form the library chose for the 14 cases
+    blu = {}
+    for i, k in enumerate(kinds):
+        blu["P1/P3" if k == "P" else k] = ctx.stored_bytes(case=i)
     # the dominant kernel alone: the P1/P3 family launch (8 cases), at the tile shape the library tunes
     # for it running alone (leaving a member out and back in re-tunes the family), in the launch form
     # of the 14-case step
@@ -187,6 +193,7 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
     ctx.set_active(False, case=p0)
     ctx.set_active(True, case=p0)
     nP = sum(1 for k in kinds if k == "P")
+    bytes_launch = sum(ctx.stored_bytes(case=i) for i, k in enumerate(kinds) if k == "P") * NX * NY
     ksteps = max(steps, 200)
     with launch_form(mode):
         ctx.step(tune_steps)
@@ -195,7 +202,6 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
         kern_s, _ = timer.run(lambda: ctx.step(ksteps))
     ctx.close()
     avg = kern_s / ksteps
-    bytes_launch = B_LU[form] * NX * NY * nP
     return {"MLUPS": n_sites * steps / wall_s / 1e6, "ms_per_step": wall_s / steps * 1e3,
             "gpu_ms_per_step": gpu_s / steps * 1e3, "tile_shapes": shapes,
             "launch_form": mode,
@@ -204,7 +210,7 @@ def bench_form(m, timer, stream, form, steps, warmup, tune_steps):
             "kernel_tile_shape": {"rows_per_segment": kshape[0], "depth": kshape[1], "tuned": kshape[2]},
             "kernel_launches_timed": ksteps, "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": bytes_launch,
             "achieved_GBs": bytes_launch / avg / 1e9, "frac": bytes_launch / avg / 1e9 / PEAK_HBM_GBS,
-            "bytes_per_lattice_update": B_LU[form], "steps_before_delta_p": total, "_dp": dp}
+            "bytes_per_lattice_update": blu, "steps_before_delta_p": total, "_dp": dp}
 
 
 FAMILY_OPS = {"P": "P3 0.5", "F3": "F3 1.569988", "F1": "F1", "F2": "F2"}
@@ -226,11 +232,13 @@ def bench_family(m, timer, stream, family, form, steps, tune_steps):
         ctx.sync()
         rows, depth, tuned = ctx.tile_shape()
     gpu_s, _ = timer.run(lambda: ctx.step(steps))
+    blu = ctx.stored_bytes()
     ctx.close()
     avg = gpu_s / steps
-    nbytes = B_LU[form] * NX * NY * len(descs)
+    nbytes = blu * NX * NY * len(descs)
     return {"family": family, "operator": op, "form": form, "cases": len(descs), "steps": steps,
             "tile_shape": {"rows_per_segment": rows, "depth": depth, "tuned": tuned},
+            "bytes_per_lattice_update": blu,
             "kernel_avg_launch_us": avg * 1e6, "bytes_per_launch": nbytes, "achieved_GBs": nbytes / avg / 1e9,
             "frac": nbytes / avg / 1e9 / PEAK_HBM_GBS}
 
@@ -255,14 +263,15 @@ def bench_large(m, timer, stream, form, steps):
     prime_graph(ctx, steps)
     gpu_s, wall_s = timer.run(lambda: ctx.step(steps))
     mode = ctx.step_mode()
+    blu = ctx.stored_bytes()
     ctx.close()
     avg = gpu_s / steps
     return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
             "steps": steps, "tile_shape": {"rows_per_segment": rows, "depth": depth, "tuned": tuned},
             "launch_form": mode,
             "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
-            "bytes_per_lattice_update": B_LU[form],
-            "achieved_GBs": B_LU[form] * n * n / avg / 1e9, "frac": B_LU[form] * n * n / avg / 1e9 / PEAK_HBM_GBS}
+            "bytes_per_lattice_update": blu,
+            "achieved_GBs": blu * n * n / avg / 1e9, "frac": blu * n * n / avg / 1e9 / PEAK_HBM_GBS}
 
 
 def stream_ceiling(m, timer, stream, reps=10):
@@ -451,7 +460,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50, help="untimed steps after the tuning steps")
     ap.add_argument("--tune-steps", type=int, default=500,
                     help="untimed steps first, on which the library tunes its tile shapes (needs ~400)")
-    ap.add_argument("--prec", default="f64", choices=list(B_LU), help="headline storage form")
+    ap.add_argument("--prec", default="f64", choices=FORMS, help="headline storage form")
     ap.add_argument("--forms", default="f32,f32z,f16,f16z", help="forms measured beside the headline")
     ap.add_argument("--sweep-steps", type=int, default=20000, help="0 skips the 168-case sweep")
     ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 cases")
@@ -520,7 +529,7 @@ def main():
                                "(P1, P3 chi -0.4/0.5/2, F1, F2, F3 xi 1.569988), sigma 0.1, beta 0.8; 14 cases "
                                f"batched; {FORM_TEXT[a.prec]}",
                    "cases": 14, "sites_per_case": NX * NY, "storage": a.prec,
-                   "bytes_per_lattice_update": B_LU[a.prec]},
+                   "bytes_per_lattice_update": head["bytes_per_lattice_update"]},
         "roofline": {"bound": "hbm", "kernel": head["dominant_kernel"], "achieved": head["achieved_GBs"],
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": head["frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": head["bytes_per_launch"],

@@ -185,6 +185,14 @@ int lbm_tile_shape(const lbm_ctx* ctx, int case_id, int* rows, int* depth, int* 
  * for bit.  No reference counterpart (the reference runs one case per process). */
 int lbm_step_mode(const lbm_ctx* ctx, int* mode);
 
+/* Bytes a time step reads and writes per lattice update of the case in its stored form (every stored
+ * value read once and written once): 384 for the FP64 compact form of the P1/P3 and F3 cases (19
+ * totals and 5 recolouring scalars per site, from which the receiving site forms the 38 colour
+ * populations bit for bit), 608 for the FP64 form storing the 38 populations, 304 / 224 / 152 / 112
+ * for the FP32, FP32 z-symmetric, FP16 and FP16 z-symmetric forms.  No reference counterpart; for
+ * reports and roofline figures. */
+int lbm_stored_bytes(const lbm_ctx* ctx, int case_id, int* bytes_per_update);
+
 /* Copies `bytes` bytes of device memory from src to dst (both 16-byte aligned) on the HIP stream
  * hip_stream (NULL: the null stream) with the library's plain streaming kernel (16-byte loads,
  * nontemporal stores).  No reference counterpart: it measures the bandwidth a read-once / write-once

@@ -34,7 +34,7 @@ SYMBOLS = ["lbm_create", "lbm_destroy", "lbm_last_error", "lbm_device_bytes", "l
            "lbm_download_state", "lbm_init_drop", "lbm_init_band", "lbm_step", "lbm_step_case",
            "lbm_step_case_noalloc", "lbm_tile_shape", "lbm_stream_copy", "lbm_sync", "lbm_set_stream", "lbm_set_reference", "lbm_set_active", "lbm_set_graph",
            "lbm_download_aux", "lbm_diag_drop", "lbm_diag_incl", "lbm_read_geometry", "lbm_set_geometry",
-           "lbm_write_vtk", "lbm_step_mode"]
+           "lbm_write_vtk", "lbm_step_mode", "lbm_stored_bytes"]
 
 
 class CaseDesc(C.Structure):
@@ -106,6 +106,7 @@ def lib():
         L.lbm_diag_drop.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(DropDiag)]
         L.lbm_diag_incl.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(InclDiag), C.c_void_p]
         L.lbm_step_mode.argtypes = [P, ip]
+        L.lbm_stored_bytes.argtypes = [P, C.c_int, ip]
         L.lbm_read_geometry.argtypes = [C.c_char_p, C.c_int, C.c_int, ip, ip, ip, C.c_void_p, C.c_longlong]
         L.lbm_read_geometry.restype = C.c_longlong
         L.lbm_set_geometry.argtypes = [P, C.c_int, C.c_void_p, C.c_double]
@@ -249,6 +250,12 @@ class Context:
         """lbm_step_mode: -1 undecided, 0 one launch per family, 1 one launch, 2 one persistent launch."""
         v = C.c_int()
         self._chk(lib().lbm_step_mode(self.h, C.byref(v)))
+        return v.value
+
+    def stored_bytes(self, case=0):
+        """lbm_stored_bytes: bytes a step reads and writes per lattice update of the case's stored form."""
+        v = C.c_int()
+        self._chk(lib().lbm_stored_bytes(self.h, case, C.byref(v)))
         return v.value
 
     def tile_shape(self, case=0):

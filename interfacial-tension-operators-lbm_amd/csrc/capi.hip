@@ -1410,6 +1410,15 @@ int lbm_step_mode(const lbm_ctx* c, int* mode)
     return LBM_OK;
 }
 
+int lbm_stored_bytes(const lbm_ctx* c, int id, int* bytes_per_update)
+{
+    if (!c || id < 0 || id >= (int)c->cases.size() || !bytes_per_update) return LBM_E_ARG;
+    const CaseState& k = c->cases[id];
+    const int per_site = k.elem == 8 ? (k.compact ? Q + 5 : 2 * Q) * 8 : 2 * k.nslot * k.elem;
+    *bytes_per_update = 2 * per_site;   // read once, written once
+    return LBM_OK;
+}
+
 int lbm_set_active(lbm_ctx* c, int id, int active)
 {
     if (int r = check_case(c, id)) return r;

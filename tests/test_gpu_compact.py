@@ -1,0 +1,142 @@
+"""GPU: the compact FP64 form of the P1/P3 and F3 cases against the oracle and against the full form.
+
+In the compact form a site stores its 19 post-collision totals and the five scalars its recolouring
+is formed from (cR, rho, the mediator gradient); the site that pulls a population forms its two colour
+parts with the sender's own expressions.  These tests check, through the C ABI:
+
+- the bytes per lattice update each case reports (lbm_stored_bytes);
+- the full FP64 form (ITOLBM_F64_COMPACT=0) still equals the oracle bit for bit;
+- raw sites (near-empty sites the collision leaves untouched, Operadores_tensao.cpp:347) placed on
+  strip edges, on the box's periodic seam and across segment boundaries, where the kernel falls back
+  to per-population loads, give the oracle's bits;
+- a batch of all seven operators steps to the same bits in both forms.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import _pkg
+
+pytestmark = pytest.mark.gpu
+
+OPS7 = [("P1", 0.225), ("P3 -0.4", 0.225), ("P3 0.5", 0.225), ("P3 2", 0.225), ("F1", 0.1), ("F2", 0.1),
+        ("F3 1.569988", 0.1)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_same(a, b, what):
+    bad = np.flatnonzero(~(a == b))
+    assert bad.size == 0, f"{what}: {bad.size} values differ, first {bad[:5]}, max |d| {np.abs(a - b).max()}"
+
+
+def oracle_steps(m, sim, ctx, checkpoints):
+    done = 0
+    for target in checkpoints:
+        n = target - done
+        for _ in range(n):
+            sim.step()
+        ctx.step(n, record=True)
+        done = target
+        gR, gB = ctx.download(m.STATE_POST_COLLISION)
+        assert_same(gR, sim.post_R, f"red after {target} steps")
+        assert_same(gB, sim.post_B, f"blue after {target} steps")
+        gm, F = ctx.aux()
+        assert_same(gm, sim.gm, f"mediator momentum after {target} steps")
+        if sim.F is not None:
+            assert_same(F, sim.F, f"force after {target} steps")
+        nR, nB = ctx.download(m.STATE_POST_STREAM)
+        assert_same(nR, sim.fR, f"post-stream red after {target} steps")
+        assert_same(nB, sim.fB, f"post-stream blue after {target} steps")
+
+
+def test_stored_bytes(monkeypatch):
+    m = _pkg.load()
+    descs = [m.case_desc(40, 30, op, 1.0, A, 0.8) for op, A in OPS7]
+    expect = {"P1": 384, "P3": 384, "F3": 384, "F1": 608, "F2": 608}
+    ctx = m.Context(descs)
+    got = [ctx.stored_bytes(case=i) for i in range(len(descs))]
+    ctx.close()
+    assert got == [expect[op.split()[0]] for op, _ in OPS7], got
+    monkeypatch.setenv("ITOLBM_F64_COMPACT", "0")
+    ctx = m.Context(descs)
+    assert [ctx.stored_bytes(case=i) for i in range(len(descs))] == [608] * len(descs)
+    ctx.close()
+    monkeypatch.delenv("ITOLBM_F64_COMPACT")
+    for prec, b in ((1, 304), (2, 152), (3, 224), (4, 112)):
+        ctx = m.Context(descs[:1], precision=prec)
+        assert ctx.stored_bytes() == b, (prec, ctx.stored_bytes())
+        ctx.close()
+
+
+@pytest.mark.parametrize("op,A", [OPS7[2], OPS7[6]])
+def test_full_form_bitwise(op, A, monkeypatch):
+    """The full FP64 form (38 populations stored per site), chosen with ITOLBM_F64_COMPACT=0."""
+    say(f"[gpu] full FP64 form parity drop 70x45 {op}")
+    monkeypatch.setenv("ITOLBM_F64_COMPACT", "0")
+    m = _pkg.load()
+    sim = O.Sim(O.make_case(70, 45, op, 0.8, A, 0.8))
+    ctx = m.Context(m.case_desc(70, 45, op, 0.8, A, 0.8))
+    assert ctx.stored_bytes() == 608
+    sim.init_drop(11.0, width=3.45)
+    ctx.upload(sim.fR, sim.fB)
+    oracle_steps(m, sim, ctx, [1, 2, 7, 20])
+    ctx.close()
+
+
+@pytest.mark.parametrize("op,A", [OPS7[0], OPS7[2], OPS7[6]])
+def test_compact_raw_sites_at_strip_edges(op, A, monkeypatch):
+    """3 x 3 blocks of sites scaled to a density of ~1e-33 (left uncollided, so stored raw) on the
+    edge between two 64-column strips, on the periodic seam x = 0 / nx - 1, on a partial last strip
+    and across 4-row segment boundaries; rows pulling from them take the per-population path."""
+    say(f"[gpu] compact raw sites 130x40 {op}")
+    monkeypatch.setenv("ITOLBM_LY", "4")
+    m = _pkg.load()
+    nx, ny = 130, 40
+    sim = O.Sim(O.make_case(nx, ny, op, 1.0, A, 0.8))
+    ctx = m.Context(m.case_desc(nx, ny, op, 1.0, A, 0.8))
+    assert ctx.stored_bytes() == 384
+    sim.init_drop(14.0, width=3.45)
+    fR = sim.fR.reshape(ny, nx, 19)
+    fB = sim.fB.reshape(ny, nx, 19)
+    for y, x in [(20, 64), (8, 0), (33, 128), (16, 100), (4, 63)]:
+        ys = [(y + d) % ny for d in (-1, 0, 1)]
+        xs = [(x + d) % nx for d in (-1, 0, 1)]
+        for yy in ys:
+            for xx in xs:
+                fR[yy, xx, :] *= 1e-33
+                fB[yy, xx, :] *= 1e-33
+    ctx.upload(sim.fR, sim.fB)
+    oracle_steps(m, sim, ctx, [1, 2, 3, 6])
+    ctx.close()
+
+
+def test_compact_equals_full_batch(monkeypatch):
+    """All seven operators at two tau on uneven boxes, batched: 60 steps in the compact form and in
+    the full form give the same populations and the same drop diagnostics."""
+    say("[gpu] compact vs full FP64 form, 14-case batch")
+    m = _pkg.load()
+    descs, sizes = [], [(96, 70), (130, 33)]
+    for k, (op, A) in enumerate(OPS7 * 2):
+        nx, ny = sizes[k % 2]
+        descs.append(m.case_desc(nx, ny, op, 0.6 if k < 7 else 1.0, A, 0.8))
+    out = {}
+    for form in ("1", "0"):
+        monkeypatch.setenv("ITOLBM_F64_COMPACT", form)
+        ctx = m.Context(descs)
+        for i in range(len(descs)):
+            ctx.init_drop(12.0, width=-1.0, v0=0.03 * (i % 3), case=i)
+        ctx.step(60, record=True)
+        states = [ctx.download(m.STATE_POST_COLLISION, case=i) for i in range(len(descs))]
+        diags = [ctx.diag_drop(0.0, 60, case=i).delta_p for i in range(len(descs))]
+        out[form] = (states, diags)
+        ctx.close()
+    for i in range(len(descs)):
+        assert_same(out["1"][0][i][0], out["0"][0][i][0], f"case {i} red")
+        assert_same(out["1"][0][i][1], out["0"][0][i][1], f"case {i} blue")
+    assert out["1"][1] == out["0"][1]
