@@ -200,6 +200,18 @@ long long stride_pad()
     return v;
 }
 
+// Elements added to the slot pitch of the FP64 compact form (px = nx + pad).  ITOLBM_CPAD sets it
+// (0 .. 4096); the default is 0.
+int compact_pad(int nx)
+{
+    (void)nx;
+    if (const char* e = std::getenv("ITOLBM_CPAD")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 0 && v <= 4096) return (int)v;
+    }
+    return 0;
+}
+
 // Whether the context's stream is capturing a HIP graph (a caller's capture).  Work that allocates,
 // frees, copies from host memory or synchronises must not be recorded into a graph: a captured
 // host-to-device copy keeps the address of a host buffer that is gone when the graph replays, and a
@@ -408,8 +420,8 @@ int state_f64(lbm_ctx* c, CaseState& k, const double** out)
         if (int r = dev_alloc(c, (void**)&k.f64, sizeof(double) * 2 * Q * (size_t)k.S)) return r;
     }
     if (k.elem == 8) {
-        HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.compact, k.cc.beta, k.d.nx, k.n, k.S, k.f64,
-                                         c->stream));
+        HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.compact, k.cc.beta, k.d.nx, k.cc.px, k.n, k.S,
+                                         k.f64, c->stream));
     } else {
         HIPCHK(c, launch_lp_to_f64(k.elem, k.sym, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
                                    k.f64 + Q * k.S, c->stream));
@@ -467,7 +479,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         if (d.nx < 1 || d.ny < 1 || !valid_op(d.op) || !(d.tau > 0.0) || !std::isfinite(d.A) ||
             !std::isfinite(d.beta) || (d.op == OP_F3 && !(d.xi != 0.0)) ||
             (long long)d.nx * d.ny > (1LL << 31) / Q ||
-            (long long)d.nx * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
+            ((long long)d.nx + compact_pad(d.nx)) * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
                                                                                        // offsets are 32-bit
             lbm_destroy(c);
             return LBM_E_ARG;
@@ -482,10 +494,13 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.sym = sym;
         k.nslot = n_slots(sym);
         k.compact = elem == 8 && c->f64_compact && (k.kind == KIND_P || k.kind == KIND_F3);
+        k.cc.px = d.nx + compact_pad(d.nx);
         // state buffers hold 2 x nslot values per site in row blocks (the step kernels index them by
         // nx and ny only), so their size follows from n and not from the padded stride S; an FP64
-        // buffer holds either form (F64_SLOTS doubles per site), so a case can change form in place
-        const size_t pop_bytes = (size_t)elem * (elem == 8 ? F64_SLOTS : 2 * k.nslot) * (size_t)k.n;
+        // buffer holds either form (F64_SLOTS doubles per site of the padded pitch px), so a case can
+        // change form in place
+        const size_t pop_bytes = elem == 8 ? sizeof(double) * F64_SLOTS * (size_t)k.cc.px * (size_t)d.ny
+                                           : (size_t)elem * 2 * k.nslot * (size_t)k.n;
         for (int b = 0; b < 2; ++b) {
             if (dev_alloc(c, (void**)&k.pop[b], pop_bytes) || hipMemsetAsync(k.pop[b], 0, pop_bytes, c->stream) != hipSuccess) {
                 lbm_destroy(c);
@@ -570,17 +585,18 @@ int lbm_upload_state(lbm_ctx* c, int id, const double* fR, const double* fB)
             }
         }
     if (k.elem == 8 && k.compact) {
-        // compact row blocks [y][CSLOTS][nx], every site raw: f_R,i in slot i, f_B,i in slot cs_raw_b(i)
-        std::vector<double> rows((size_t)CSLOTS * k.n, 0.0);
+        // compact row blocks [y][CSLOTS][px], every site raw: f_R,i in slot i, f_B,i in slot cs_raw_b(i)
+        const size_t px = (size_t)k.cc.px;
+        std::vector<double> rows((size_t)CSLOTS * px * ny, 0.0);
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 const size_t p = (size_t)y * nx + x;
-                double* blk = rows.data() + (size_t)y * CSLOTS * nx + x;
+                double* blk = rows.data() + (size_t)y * CSLOTS * px + x;
                 for (int i = 0; i < Q; ++i) {
-                    blk[(size_t)i * nx] = h[(size_t)i * k.S + p];
-                    blk[(size_t)cs_raw_b(i) * nx] = h[(size_t)(Q + i) * k.S + p];
+                    blk[(size_t)i * px] = h[(size_t)i * k.S + p];
+                    blk[(size_t)cs_raw_b(i) * px] = h[(size_t)(Q + i) * k.S + p];
                 }
-                blk[(size_t)CS_RHO * nx] = CS_RAW;
+                blk[(size_t)CS_RHO * px] = CS_RAW;
             }
         HIPCHK(c, hipMemcpyAsync(k.pop[k.cur], rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice,
                                  c->stream));

@@ -94,6 +94,7 @@ enum OpId { OP_P1 = 1, OP_P3 = 3, OP_F1 = 11, OP_F2 = 12, OP_F3 = 13 };
 struct CaseConst {
     int op;            // OpId
     int nx, ny;
+    int px;            // FP64 compact form: elements between the slots of a row block (>= nx)
     double one_over_tau;   // 1.0 / tau            (Collision_operators.cpp:24)
     double A;              // fat_R_B              (P amplitude; sigma for F)
     double B0, B1, B2;     // coef_B by |c|^2 = 0, 1, 2   (Operadores_tensao.cpp:94-109)

@@ -547,9 +547,10 @@ __global__ void __launch_bounds__(64) step_ring1(StepLaunch launch)
 // the two columns beside the strip from two halo lanes (x0 - 1 in lane 0, x0 + 64 in lane 63).  A
 // row with a raw site among its senders is rebuilt population by population from loads (rare: the
 // first step after an upload, or a site with rho <= 1e-30).
-__device__ __forceinline__ long long cblock_doubles(int nx) { return (long long)CSLOTS * nx; }
-__device__ __forceinline__ unsigned cblock_bytes(int nx) { return (unsigned)(cblock_doubles(nx) * (long long)sizeof(double)); }
-__device__ __forceinline__ unsigned cslot_off(int slot, int nx) { return (unsigned)(slot * nx) * (unsigned)sizeof(double); }
+// px: elements from one slot of a row block to the next (the slot pitch, nx or more)
+__device__ __forceinline__ long long cblock_doubles(int px) { return (long long)CSLOTS * px; }
+__device__ __forceinline__ unsigned cblock_bytes(int px) { return (unsigned)(cblock_doubles(px) * (long long)sizeof(double)); }
+__device__ __forceinline__ unsigned cslot_off(int slot, int px) { return (unsigned)(slot * px) * (unsigned)sizeof(double); }
 
 // t_k of recolour() for pair k = 0 .. 8, the pairs (1,2) (3,4) (5,6) (7,8) (9,10) (11,12) (13,14) (16,15)
 // (18,17): the axis pairs with fw1 and |gm|, the diagonal ones with fw2 and |gm| sqrt 2
@@ -622,15 +623,15 @@ struct CPop {
     double f, cR, rho, gx, gy, gz;
 };
 
-__device__ __forceinline__ CPop cpop_load(gcdouble* __restrict__ in, long long e0, int nx, int i)
+__device__ __forceinline__ CPop cpop_load(gcdouble* __restrict__ in, long long e0, int px, int i)
 {
     CPop p;
-    p.f = in[e0 + (long long)i * nx];
-    p.cR = in[e0 + (long long)CS_CR * nx];
-    p.rho = in[e0 + (long long)CS_RHO * nx];
-    p.gx = in[e0 + (long long)CS_G * nx];
-    p.gy = in[e0 + (long long)(CS_G + 1) * nx];
-    p.gz = in[e0 + (long long)(CS_G + 2) * nx];
+    p.f = in[e0 + (long long)i * px];
+    p.cR = in[e0 + (long long)CS_CR * px];
+    p.rho = in[e0 + (long long)CS_RHO * px];
+    p.gx = in[e0 + (long long)CS_G * px];
+    p.gy = in[e0 + (long long)(CS_G + 1) * px];
+    p.gz = in[e0 + (long long)(CS_G + 2) * px];
     return p;
 }
 
@@ -649,12 +650,12 @@ __device__ __forceinline__ double pair_t_rt(int k, double beta, double cR, doubl
 
 // f_R,i and f_B,i of the site whose slot 0 is element e0 (velocity i chosen at run time), or its stored
 // values when it is raw
-__device__ __forceinline__ void cpop_split(gcdouble* __restrict__ in, long long e0, int nx, double beta, int i,
+__device__ __forceinline__ void cpop_split(gcdouble* __restrict__ in, long long e0, int px, double beta, int i,
                                            const CPop& p, double& r, double& b)
 {
     if (p.rho <= 1.0e-30) {
         r = p.f;
-        b = in[e0 + (long long)cs_raw_b(i) * nx];
+        b = in[e0 + (long long)cs_raw_b(i) * px];
         return;
     }
     const double cR = p.cR, cB = 1.0 - cR;
@@ -674,21 +675,21 @@ __device__ __forceinline__ void cpop_split(gcdouble* __restrict__ in, long long 
 }
 
 // element of slot 0 of the site that site (col, Y) pulls velocity i from
-__device__ __forceinline__ long long csender(int nx, int ny, const Lane& L, int Y, int i)
+__device__ __forceinline__ long long csender(int px, int ny, const Lane& L, int Y, int i)
 {
     const int xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
-    return (long long)wrapi(Y - CY[i], ny) * cblock_doubles(nx) + xs;
+    return (long long)wrapi(Y - CY[i], ny) * cblock_doubles(px) + xs;
 }
 
 // the 38 populations pulled by site (col, Y), each from its sender's stored values (any sender raw)
-__device__ __forceinline__ void cpull_site(gcdouble* __restrict__ in, int nx, int ny, double beta, const Lane& L,
+__device__ __forceinline__ void cpull_site(gcdouble* __restrict__ in, int px, int ny, double beta, const Lane& L,
                                            int Y, double fR[Q], double fB[Q])
 {
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const long long e0 = csender(nx, ny, L, Y, i);
-        const CPop p = cpop_load(in, e0, nx, i);
-        cpop_split(in, e0, nx, beta, i, p, fR[i], fB[i]);
+        const long long e0 = csender(px, ny, L, Y, i);
+        const CPop p = cpop_load(in, e0, px, i);
+        cpop_split(in, e0, px, beta, i, p, fR[i], fB[i]);
     }
 }
 
@@ -735,8 +736,9 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
     const Lane L = make_lane_aligned(nx, t.strip);
     const Halo1 h = make_halo1(nx, t.strip);
     const int lane = L.lane;
-    const long long rb = cblock_doubles(nx);
-    const unsigned bytes = cblock_bytes(nx);
+    const int px = cc.px;
+    const long long rb = cblock_doubles(px);
+    const unsigned bytes = cblock_bytes(px);
     const unsigned ooff = (unsigned)L.col * 8u;
     // the halo columns' scalars: lanes 0 - 4 read column x0 - 1, lanes 5 - 9 column x0 + 64; the other
     // lanes read past the buffer resource's extent, which returns 0 without a memory access
@@ -754,8 +756,8 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
         const rsrc_t rr = make_rsrc(t.in + wrapi(r, ny) * rb, bytes);
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            o.v[q] = buf_load<double>(rr, ooff, cslot_off(CS_CR + q, nx));
-            hs.v[q] = buf_load<double>(rr, hoff, cslot_off(CS_CR + q, nx));
+            o.v[q] = buf_load<double>(rr, ooff, cslot_off(CS_CR + q, px));
+            hs.v[q] = buf_load<double>(rr, hoff, cslot_off(CS_CR + q, px));
         }
     };
     // terms of stored row r into the ring; returns whether a site of the row (strip or halo columns) is raw
@@ -781,7 +783,7 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
         for (int i = 0; i < Q; ++i) {
             const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : ooff);
             const rsrc_t r = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
-            f[i] = buf_load<double>(r, xs, cslot_off(i, nx));
+            f[i] = buf_load<double>(r, xs, cslot_off(i, px));
         }
     };
     // Row Y's pulled populations, formed in index order from the totals and the ring (or, when a sender
@@ -806,9 +808,9 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
 #undef ITOLBM_CS
                 }
             } else {
-                const long long e0 = csender(nx, ny, L, Y, i);
-                const CPop p = cpop_load(t.in, e0, nx, i);
-                cpop_split(t.in, e0, nx, cc.beta, i, p, r, b);
+                const long long e0 = csender(px, ny, L, Y, i);
+                const CPop p = cpop_load(t.in, e0, px, i);
+                cpop_split(t.in, e0, px, cc.beta, i, p, r, b);
             }
             if (i == 0) {
                 rR = r;
@@ -825,7 +827,7 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
     auto halo_e0 = [&](int Y) { return (long long)wrap_row3(Y, h.dy, ny) * rb + h.x; };
     auto halo_phase = [&](int Y, const CPop& hp, double& l, double& r) {
         double vr, vb;
-        cpop_split(t.in, halo_e0(Y), nx, cc.beta, h.i, hp, vr, vb);
+        cpop_split(t.in, halo_e0(Y), px, cc.beta, h.i, hp, vr, vb);
         if (lane < 2 * Q) {
             lds.hbuf[0][lane] = vr;
             lds.hbuf[1][lane] = vb;
@@ -862,7 +864,7 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
     bool Wp = rc_write(t.y0, sc, hsc);
     lds_order();
     load_tot(t.y0 - 1, tot);
-    hp = cpop_load(t.in, halo_e0(t.y0 - 1), nx, h.i);
+    hp = cpop_load(t.in, halo_e0(t.y0 - 1), px, h.i);
     pull_sums(t.y0 - 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
     ph[0] = phase(rRn, rBn);
     halo_phase(t.y0 - 1, hp, hl[0], hr[0]);
@@ -873,7 +875,7 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
     Wp = rc_write(t.y0 + 1, sc, hsc);
     lds_order();
     load_tot(t.y0, tot);
-    hp = cpop_load(t.in, halo_e0(t.y0), nx, h.i);
+    hp = cpop_load(t.in, halo_e0(t.y0), px, h.i);
     pull_sums(t.y0, tot, Wm || W0 || Wp, fn, rRn, rBn);
     ph[1] = phase(rRn, rBn);
     halo_phase(t.y0, hp, hl[1], hr[1]);
@@ -883,7 +885,7 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
         // halo pulls); the gradient-independent part of this row's collision runs while they are in flight
         load_tot(y + 1, tot);
         scal_load(y + 2, sc, hsc);
-        hp = cpop_load(t.in, halo_e0(y + 1), nx, h.i);
+        hp = cpop_load(t.in, halo_e0(y + 1), px, h.i);
         Site s;
         site_sums_f(s, fn, rRn, rBn);
         if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
@@ -927,25 +929,25 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
             if (!s.skip) {
                 // the totals and the scalars the recolouring is formed from
 #pragma unroll
-                for (int i = 0; i < Q; ++i) buf_store<double>(s.f[i], ro, ooff, cslot_off(i, nx));
-                buf_store<double>(s.cR, ro, ooff, cslot_off(CS_CR, nx));
-                buf_store<double>(s.rho, ro, ooff, cslot_off(CS_RHO, nx));
-                buf_store<double>(gx, ro, ooff, cslot_off(CS_G, nx));
-                buf_store<double>(gy, ro, ooff, cslot_off(CS_G + 1, nx));
-                buf_store<double>(gz, ro, ooff, cslot_off(CS_G + 2, nx));
+                for (int i = 0; i < Q; ++i) buf_store<double>(s.f[i], ro, ooff, cslot_off(i, px));
+                buf_store<double>(s.cR, ro, ooff, cslot_off(CS_CR, px));
+                buf_store<double>(s.rho, ro, ooff, cslot_off(CS_RHO, px));
+                buf_store<double>(gx, ro, ooff, cslot_off(CS_G, px));
+                buf_store<double>(gy, ro, ooff, cslot_off(CS_G + 1, px));
+                buf_store<double>(gz, ro, ooff, cslot_off(CS_G + 2, px));
             } else {   // left untouched (rare): the pulled populations, raw
                 // (a rolled loop: unrolled, its loads took the registers of a second row)
 #pragma unroll 1
                 for (int i = 0; i < Q; ++i) {
                     const int xs = cx_of(i) > 0 ? L.xm : (cx_of(i) < 0 ? L.xp : L.col);
                     const long long e0 = (long long)wrapi(y - cy_of(i), ny) * rb + xs;
-                    const CPop p = cpop_load(t.in, e0, nx, i);
+                    const CPop p = cpop_load(t.in, e0, px, i);
                     double r, b;
-                    cpop_split(t.in, e0, nx, cc.beta, i, p, r, b);
-                    buf_store<double>(r, ro, ooff, cslot_off(i, nx));
-                    buf_store<double>(b, ro, ooff, cslot_off(cs_raw_b(i), nx));
+                    cpop_split(t.in, e0, px, cc.beta, i, p, r, b);
+                    buf_store<double>(r, ro, ooff, cslot_off(i, px));
+                    buf_store<double>(b, ro, ooff, cslot_off(cs_raw_b(i), px));
                 }
-                buf_store<double>(CS_RAW, ro, ooff, cslot_off(CS_RHO, nx));
+                buf_store<double>(CS_RAW, ro, ooff, cslot_off(CS_RHO, px));
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
@@ -1518,17 +1520,17 @@ __global__ void __launch_bounds__(256) f64_rows_to_soa(const double* __restrict_
 }
 
 // the compact form -> [colour][19][S]: each site's 38 populations formed from its own stored values
-__global__ void __launch_bounds__(256) f64c_rows_to_soa(const double* __restrict__ pop, double beta, int nx, long long n,
-                                                        long long S, double* __restrict__ soa)
+__global__ void __launch_bounds__(256) f64c_rows_to_soa(const double* __restrict__ pop, double beta, int nx, int px,
+                                                        long long n, long long S, double* __restrict__ soa)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
         const long long y = p / nx, x = p - y * nx;
-        const long long e0 = y * cblock_doubles(nx) + x;
+        const long long e0 = y * cblock_doubles(px) + x;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const CPop c = cpop_load((gcdouble*)pop, e0, nx, i);
+            const CPop c = cpop_load((gcdouble*)pop, e0, px, i);
             double r, b;
-            cpop_split((gcdouble*)pop, e0, nx, beta, i, c, r, b);
+            cpop_split((gcdouble*)pop, e0, px, beta, i, c, r, b);
             soa[i * S + p] = r;
             soa[(Q + i) * S + p] = b;
         }
@@ -1536,11 +1538,11 @@ __global__ void __launch_bounds__(256) f64c_rows_to_soa(const double* __restrict
 }
 }  // namespace
 
-hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, long long n, long long S,
-                                  double* soa, hipStream_t stream)
+hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, int px, long long n,
+                                  long long S, double* soa, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
-    if (compact) hipLaunchKernelGGL(f64c_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, beta, nx, n, S, soa);
+    if (compact) hipLaunchKernelGGL(f64c_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, beta, nx, px, n, S, soa);
     else hipLaunchKernelGGL(f64_rows_to_soa, dim3(nb), dim3(256), 0, stream, pop, nx, n, S, soa);
     return hipGetLastError();
 }

@@ -33,7 +33,8 @@ struct StepLaunch {
     int compact;             // FP64: the P1/P3 and F3 cases hold the compact form (below)
 };
 
-// Compact FP64 form (P1/P3 and F3 cases): row blocks [y][CSLOTS][nx] of doubles.  A collided site
+// Compact FP64 form (P1/P3 and F3 cases): row blocks [y][CSLOTS][px] of doubles (px >= nx: the slot
+// pitch, CaseConst::px).  A collided site
 // stores its 19 post-collision totals f_i in slots 0..18 and the scalars its recolouring was formed
 // from in slots CS_CR (colour fraction cR), CS_RHO (density), CS_G .. CS_G + 2 (mediator gradient);
 // slots 24..38 are unused.  A raw site (one the collision left untouched, or an uploaded state) holds
@@ -100,10 +101,10 @@ struct GeoLaunch {
 };
 hipError_t launch_step_geo(int kind, const GeoLaunch& a, hipStream_t stream);
 
-// FP64 state in row blocks [y][colour][19][nx], or in the compact form [y][CSLOTS][nx] (recolouring
-// factor beta), -> [colour][19][S]: every site's post-collision populations at the site
-hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, long long n, long long S,
-                                  double* soa, hipStream_t stream);
+// FP64 state in row blocks [y][colour][19][nx], or in the compact form [y][CSLOTS][px] (recolouring
+// factor beta, slot pitch px), -> [colour][19][S]: every site's post-collision populations at the site
+hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, int px, long long n,
+                                  long long S, double* soa, hipStream_t stream);
 
 // FP32-compute forms; storage FP32 (elem = 4) or FP16 (elem = 2): per population the pair
 // (s_i / w_i, e_i), s_i = fR_i + fB_i - w_i, e_i = (fR_i - fB_i) / (fR_i + fB_i), at ((y * NS + slot) nx + x)
