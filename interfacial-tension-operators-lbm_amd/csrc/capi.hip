@@ -976,8 +976,9 @@ int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
     // FP32/FP16: rows in flight ahead 1 .. 3; FP64 F1 / F2: 1 reads a row's populations again for its
     // collision, 2 keeps them in registers (one wavefront per SIMD); other FP64 kernels: one form
     auto depths_of = [&](const Group& g) {
-        if (c->cases[g.members[0]].elem != 8) return 3;
-        return (g.kind == KIND_F1 || g.kind == KIND_F2) ? 2 : 1;
+        const CaseState& k0 = c->cases[g.members[0]];
+        if (k0.elem != 8) return 3;
+        return (g.kind == KIND_F1 || g.kind == KIND_F2) && !k0.compact ? 2 : 1;
     };
     constexpr int COMMON[] = {4, 8, 16, 32, 64};
     long long budget = 0;

@@ -728,56 +728,76 @@ struct Ring1cLds {
 };
 constexpr int HALO_PAIRS[5] = {0, 3, 4, 5, 6};   // pairs (1,2) (7,8) (9,10) (11,12) (13,14)
 
-template <bool F3>
-__device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
-{
-    const CaseConst& cc = t.cc;
-    const int nx = cc.nx, ny = cc.ny;
-    const Lane L = make_lane_aligned(nx, t.strip);
-    const Halo1 h = make_halo1(nx, t.strip);
-    const int lane = L.lane;
-    const int px = cc.px;
-    const long long rb = cblock_doubles(px);
-    const unsigned bytes = cblock_bytes(px);
-    const unsigned ooff = (unsigned)L.col * 8u;
-    // the halo columns' scalars: lanes 0 - 4 read column x0 - 1, lanes 5 - 9 column x0 + 64; the other
-    // lanes read past the buffer resource's extent, which returns 0 without a memory access
-    const bool hlane = lane < 10;
-    const int hk = HALO_PAIRS[lane % 5];
-    const int hcol = lane < 5 ? 0 : 65;
-    const unsigned hoff = hlane ? (unsigned)wrapi(t.strip * 64 + (lane < 5 ? -1 : 64), nx) * 8u : 0x80000000u;
-    auto slot = [&](int r) { return (r - t.y0 + 2) % 3; };
-    auto lds_order = [] { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); };
-
+// The pulling side of the compact form for one tile (an aligned 64-column strip): loads of the stored
+// scalars and of the pulled totals, the recolouring terms of three stored rows in an LDS ring
+// (rc[slot][field][column]), and a pulled row's populations formed from them.
+struct CPull {
     struct Scal {
         double v[5];   // cR, rho, gx, gy, gz
     };
-    auto scal_load = [&](int r, Scal& o, Scal& hs) {
-        const rsrc_t rr = make_rsrc(t.in + wrapi(r, ny) * rb, bytes);
+    gcdouble* in;
+    double (*rc)[10][66];
+    Lane L;
+    long long rb;      // doubles per row block
+    unsigned bytes;    // bytes per row block
+    unsigned ooff;     // byte offset of the lane's column
+    unsigned hoff;     // halo lanes: byte offset of the halo column, else past the block
+    int ny, px, hk, hcol, first;
+    bool hlane;
+    double beta;
+
+    // the halo columns' scalars: lanes 0 - 4 read column x0 - 1, lanes 5 - 9 column x0 + 64, one pair of
+    // their terms each; the other lanes read past the buffer resource's extent, which returns 0 without
+    // a memory access.  `first` is the first stored row the ring holds.
+    __device__ __forceinline__ CPull(const TileCtx& t, const Lane& lane_, double (*ring)[10][66], int first_row)
+        : in(t.in), rc(ring), L(lane_), first(first_row)
+    {
+        const int nx = t.cc.nx;
+        ny = t.cc.ny;
+        px = t.cc.px;
+        rb = cblock_doubles(px);
+        bytes = cblock_bytes(px);
+        ooff = (unsigned)L.col * 8u;
+        hlane = L.lane < 10;
+        hk = HALO_PAIRS[L.lane % 5];
+        hcol = L.lane < 5 ? 0 : 65;
+        hoff = hlane ? (unsigned)wrapi(t.strip * 64 + (L.lane < 5 ? -1 : 64), nx) * 8u : 0x80000000u;
+        beta = t.cc.beta;
+    }
+    __device__ __forceinline__ int slot(int r) const { return (r - first) % 3; }
+
+    // live false: nothing is read, through an empty buffer resource; the loads are still issued, so every
+    // path leaves the same loads outstanding
+    __device__ __forceinline__ void scal_load(int r, Scal& o, Scal& hs, bool live = true) const
+    {
+        const rsrc_t rr = make_rsrc(in + wrapi(r, ny) * rb, live ? bytes : 0u);
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             o.v[q] = buf_load<double>(rr, ooff, cslot_off(CS_CR + q, px));
             hs.v[q] = buf_load<double>(rr, hoff, cslot_off(CS_CR + q, px));
         }
-    };
+    }
     // terms of stored row r into the ring; returns whether a site of the row (strip or halo columns) is raw
-    auto rc_write = [&](int r, const Scal& o, const Scal& hs) {
-        const Rc a = rc_of(cc.beta, o.v[0], o.v[1], o.v[2], o.v[3], o.v[4]);
-        auto& R = lds.rc[slot(r)];
-        R[0][lane + 1] = a.cR;
+    __device__ __forceinline__ bool rc_write(int r, const Scal& o, const Scal& hs) const
+    {
+        const Rc a = rc_of(beta, o.v[0], o.v[1], o.v[2], o.v[3], o.v[4]);
+        auto& R = rc[slot(r)];
+        R[0][L.lane + 1] = a.cR;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) R[1 + k][lane + 1] = a.t[k];
+        for (int k = 0; k < 9; ++k) R[1 + k][L.lane + 1] = a.t[k];
         if (hlane) {
-            R[1 + hk][hcol] = pair_t_rt(hk, cc.beta, hs.v[0], hs.v[1], hs.v[2], hs.v[3], hs.v[4]);
+            R[1 + hk][hcol] = pair_t_rt(hk, beta, hs.v[0], hs.v[1], hs.v[2], hs.v[3], hs.v[4]);
             R[0][hcol] = hs.v[0];
         }
         return __ballot(o.v[1] <= 1.0e-30 || (hlane && hs.v[1] <= 1.0e-30)) != 0;
-    };
+    }
     // pulled totals of row Y (the x-shifted loads of load_row, one value per velocity)
-    auto load_tot = [&](int Y, double (&f)[Q]) {
-        const rsrc_t rm = make_rsrc(t.in + wrapi(Y - 1, ny) * rb, bytes);
-        const rsrc_t r0 = make_rsrc(t.in + wrapi(Y, ny) * rb, bytes);
-        const rsrc_t rp = make_rsrc(t.in + wrapi(Y + 1, ny) * rb, bytes);
+    __device__ __forceinline__ void load_tot(int Y, double (&f)[Q], bool live = true) const
+    {
+        const unsigned nb = live ? bytes : 0u;
+        const rsrc_t rm = make_rsrc(in + wrapi(Y - 1, ny) * rb, nb);
+        const rsrc_t r0 = make_rsrc(in + wrapi(Y, ny) * rb, nb);
+        const rsrc_t rp = make_rsrc(in + wrapi(Y + 1, ny) * rb, nb);
         const unsigned om = (unsigned)L.xm * 8u, op = (unsigned)L.xp * 8u;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -785,19 +805,21 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
             const rsrc_t r = CY[i] > 0 ? rm : (CY[i] < 0 ? rp : r0);
             f[i] = buf_load<double>(r, xs, cslot_off(i, px));
         }
-    };
+    }
     // Row Y's pulled populations, formed in index order from the totals and the ring (or, when a sender
     // is raw, from loads): their totals f_i = fR_i + fB_i and the colour sums rR, rB, as dens() sums them
-    auto pull_sums = [&](int Y, const double (&tot)[Q], bool raw, double (&f)[Q], double& rR, double& rB) {
-        const auto& Rm = lds.rc[slot(Y - 1)];
-        const auto& R0 = lds.rc[slot(Y)];
-        const auto& Rp = lds.rc[slot(Y + 1)];
+    __device__ __forceinline__ void pull_sums(int Y, const double (&tot)[Q], bool raw, double (&f)[Q], double& rR,
+                                              double& rB) const
+    {
+        const auto& Rm = rc[slot(Y - 1)];
+        const auto& R0 = rc[slot(Y)];
+        const auto& Rp = rc[slot(Y + 1)];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             double r, b;
             if (!raw) {
                 const auto& Rs = CY[i] > 0 ? Rm : (CY[i] < 0 ? Rp : R0);
-                const int c = lane + 1 - CX[i];
+                const int c = L.lane + 1 - CX[i];
                 const double cR = Rs[0][c];
                 const double tk = (i == 0) ? 0.0 : Rs[1 + ((i - 1) >> 1)][c];
                 switch (i) {   // csplit needs the velocity as a constant
@@ -809,8 +831,8 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
                 }
             } else {
                 const long long e0 = csender(px, ny, L, Y, i);
-                const CPop p = cpop_load(t.in, e0, px, i);
-                cpop_split(t.in, e0, px, cc.beta, i, p, r, b);
+                const CPop p = cpop_load(in, e0, px, i);
+                cpop_split(in, e0, px, beta, i, p, r, b);
             }
             if (i == 0) {
                 rR = r;
@@ -821,7 +843,57 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
             }
             f[i] = r + b;
         }
-    };
+    }
+    // the site's own pulled populations stored raw (a site the collision leaves untouched); a rolled
+    // loop: unrolled, its loads took the registers of a second row
+    __device__ __forceinline__ void store_raw(gdouble* out, int y) const
+    {
+        // (the loop reads locals: with the members it kept the whole object in scratch memory)
+        gcdouble* const src = in;
+        const long long b0 = rb;
+        const int xm = L.xm, xp = L.xp, xc = L.col, n_y = ny, p_x = px;
+        const unsigned oo = ooff;
+        const double be = beta;
+        const rsrc_t ro = make_rsrc(out + (long long)y * b0, bytes);
+#pragma unroll 1
+        for (int i = 0; i < Q; ++i) {
+            const int xs = cx_of(i) > 0 ? xm : (cx_of(i) < 0 ? xp : xc);
+            const long long e0 = (long long)wrapi(y - cy_of(i), n_y) * b0 + xs;
+            const CPop p = cpop_load(src, e0, p_x, i);
+            double r, b;
+            cpop_split(src, e0, p_x, be, i, p, r, b);
+            buf_store<double>(r, ro, oo, cslot_off(i, p_x));
+            buf_store<double>(b, ro, oo, cslot_off(cs_raw_b(i), p_x));
+        }
+        buf_store<double>(CS_RAW, ro, oo, cslot_off(CS_RHO, p_x));
+    }
+    // a collided site: its totals and the scalars its recolouring is formed from
+    __device__ __forceinline__ void store(gdouble* out, int y, const Site& s, double gx, double gy, double gz) const
+    {
+        const rsrc_t ro = make_rsrc(out + (long long)y * rb, bytes);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) buf_store<double>(s.f[i], ro, ooff, cslot_off(i, px));
+        buf_store<double>(s.cR, ro, ooff, cslot_off(CS_CR, px));
+        buf_store<double>(s.rho, ro, ooff, cslot_off(CS_RHO, px));
+        buf_store<double>(gx, ro, ooff, cslot_off(CS_G, px));
+        buf_store<double>(gy, ro, ooff, cslot_off(CS_G + 1, px));
+        buf_store<double>(gz, ro, ooff, cslot_off(CS_G + 2, px));
+    }
+};
+
+template <bool F3>
+__device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
+{
+    const CaseConst& cc = t.cc;
+    const int nx = cc.nx, ny = cc.ny;
+    const Lane L = make_lane_aligned(nx, t.strip);
+    const Halo1 h = make_halo1(nx, t.strip);
+    const int lane = L.lane;
+    const CPull P(t, L, lds.rc, t.y0 - 2);
+    const int px = P.px;
+    const long long rb = P.rb;
+    auto lds_order = [] { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); };
+    using Scal = CPull::Scal;
     // the halo columns' colour field: lane j < 38 pulls velocity h.i of halo site j / 19 from its
     // sender's stored values; the pairs go through LDS to lanes 0 and 63, which sum them in index order
     auto halo_e0 = [&](int Y) { return (long long)wrap_row3(Y, h.dy, ny) * rb + h.x; };
@@ -856,35 +928,35 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
     double ph[3], hl[3], hr[3];   // colour field of rows y-1, y, y+1 and of the two halo columns
     CPop hp;
     // terms of the stored rows y0 - 2 .. y0 (the senders of pulled row y0 - 1)
-    scal_load(t.y0 - 2, sc, hsc);
-    bool Wm = rc_write(t.y0 - 2, sc, hsc);
-    scal_load(t.y0 - 1, sc, hsc);
-    bool W0 = rc_write(t.y0 - 1, sc, hsc);
-    scal_load(t.y0, sc, hsc);
-    bool Wp = rc_write(t.y0, sc, hsc);
+    P.scal_load(t.y0 - 2, sc, hsc);
+    bool Wm = P.rc_write(t.y0 - 2, sc, hsc);
+    P.scal_load(t.y0 - 1, sc, hsc);
+    bool W0 = P.rc_write(t.y0 - 1, sc, hsc);
+    P.scal_load(t.y0, sc, hsc);
+    bool Wp = P.rc_write(t.y0, sc, hsc);
     lds_order();
-    load_tot(t.y0 - 1, tot);
+    P.load_tot(t.y0 - 1, tot);
     hp = cpop_load(t.in, halo_e0(t.y0 - 1), px, h.i);
-    pull_sums(t.y0 - 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
+    P.pull_sums(t.y0 - 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
     ph[0] = phase(rRn, rBn);
     halo_phase(t.y0 - 1, hp, hl[0], hr[0]);
     // pulled row y0 (senders in stored rows y0 - 1 .. y0 + 1)
-    scal_load(t.y0 + 1, sc, hsc);
+    P.scal_load(t.y0 + 1, sc, hsc);
     Wm = W0; W0 = Wp;
     lds_order();   // the ring slot of stored row y0 - 2 is written again
-    Wp = rc_write(t.y0 + 1, sc, hsc);
+    Wp = P.rc_write(t.y0 + 1, sc, hsc);
     lds_order();
-    load_tot(t.y0, tot);
+    P.load_tot(t.y0, tot);
     hp = cpop_load(t.in, halo_e0(t.y0), px, h.i);
-    pull_sums(t.y0, tot, Wm || W0 || Wp, fn, rRn, rBn);
+    P.pull_sums(t.y0, tot, Wm || W0 || Wp, fn, rRn, rBn);
     ph[1] = phase(rRn, rBn);
     halo_phase(t.y0, hp, hl[1], hr[1]);
 
     for (int y = t.y0; y < t.y1; ++y) {
         // the next row's loads go out first (its pulled totals, the scalars of stored row y + 2 and the
         // halo pulls); the gradient-independent part of this row's collision runs while they are in flight
-        load_tot(y + 1, tot);
-        scal_load(y + 2, sc, hsc);
+        P.load_tot(y + 1, tot);
+        P.scal_load(y + 2, sc, hsc);
         hp = cpop_load(t.in, halo_e0(y + 1), px, h.i);
         Site s;
         site_sums_f(s, fn, rRn, rBn);
@@ -892,9 +964,9 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
 
         Wm = W0; W0 = Wp;
         lds_order();   // the ring slot of stored row y - 1 is written again
-        Wp = rc_write(y + 2, sc, hsc);
+        Wp = P.rc_write(y + 2, sc, hsc);
         lds_order();
-        pull_sums(y + 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
+        P.pull_sums(y + 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
         ph[2] = phase(rRn, rBn);
         halo_phase(y + 1, hp, hl[2], hr[2]);
         const Nine p9 = gather9h(ph, hl, hr, L.lane);
@@ -925,30 +997,8 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
 
         if (L.interior) {
             const unsigned site = (unsigned)(y * nx + L.col);
-            const rsrc_t ro = make_rsrc(t.out + (long long)y * rb, bytes);
-            if (!s.skip) {
-                // the totals and the scalars the recolouring is formed from
-#pragma unroll
-                for (int i = 0; i < Q; ++i) buf_store<double>(s.f[i], ro, ooff, cslot_off(i, px));
-                buf_store<double>(s.cR, ro, ooff, cslot_off(CS_CR, px));
-                buf_store<double>(s.rho, ro, ooff, cslot_off(CS_RHO, px));
-                buf_store<double>(gx, ro, ooff, cslot_off(CS_G, px));
-                buf_store<double>(gy, ro, ooff, cslot_off(CS_G + 1, px));
-                buf_store<double>(gz, ro, ooff, cslot_off(CS_G + 2, px));
-            } else {   // left untouched (rare): the pulled populations, raw
-                // (a rolled loop: unrolled, its loads took the registers of a second row)
-#pragma unroll 1
-                for (int i = 0; i < Q; ++i) {
-                    const int xs = cx_of(i) > 0 ? L.xm : (cx_of(i) < 0 ? L.xp : L.col);
-                    const long long e0 = (long long)wrapi(y - cy_of(i), ny) * rb + xs;
-                    const CPop p = cpop_load(t.in, e0, px, i);
-                    double r, b;
-                    cpop_split(t.in, e0, px, cc.beta, i, p, r, b);
-                    buf_store<double>(r, ro, ooff, cslot_off(i, px));
-                    buf_store<double>(b, ro, ooff, cslot_off(cs_raw_b(i), px));
-                }
-                buf_store<double>(CS_RAW, ro, ooff, cslot_off(CS_RHO, px));
-            }
+            if (!s.skip) P.store(t.out, y, s, gx, gy, gz);
+            else P.store_raw(t.out, y);   // left untouched (rare): the pulled populations, raw
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
         ph[0] = ph[1]; ph[1] = ph[2];

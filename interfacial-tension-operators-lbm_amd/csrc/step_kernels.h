@@ -81,7 +81,7 @@ hipError_t launch_step_f64(int kind, const StepLaunch& a, int ntiles, hipStream_
 // wavefront per tile; grid = step_f64_fused_capacity(): persistent)
 hipError_t launch_step_f64_fused(const StepLaunch& a, int grid, hipStream_t stream);
 // blocks of step_fused resident on the current device at once, a multiple of 8 (0 if unknown), for
-// the compact or the full form of the P1/P3 and F3 cases
+// P1/P3 and F3 cases in the compact or the full form
 int step_f64_fused_capacity(bool compact);
 
 // One FP64 step of one case with solid sites (step_f64.hip): `in` holds the post-collision state at

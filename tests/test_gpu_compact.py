@@ -74,11 +74,16 @@ def test_stored_bytes(monkeypatch):
         ctx.close()
 
 
-@pytest.mark.parametrize("op,A", [OPS7[2], OPS7[6]])
-def test_full_form_bitwise(op, A, monkeypatch):
-    """The full FP64 form (38 populations stored per site), chosen with ITOLBM_F64_COMPACT=0."""
-    say(f"[gpu] full FP64 form parity drop 70x45 {op}")
+@pytest.mark.parametrize("op,A,depth", [OPS7[2] + (None,), OPS7[6] + (None,), OPS7[4] + ("1",), OPS7[4] + ("2",),
+                                        OPS7[5] + ("1",), OPS7[5] + ("2",)])
+def test_full_form_bitwise(op, A, depth, monkeypatch):
+    """The full FP64 form (38 populations stored per site), chosen with ITOLBM_F64_COMPACT=0; F1 / F2
+    with the kernel that reads a row's populations again (depth 1) and the one that keeps them (2)."""
+    say(f"[gpu] full FP64 form parity drop 70x45 {op} depth {depth}")
     monkeypatch.setenv("ITOLBM_F64_COMPACT", "0")
+    if depth:
+        monkeypatch.setenv("ITOLBM_LY", "8")
+        monkeypatch.setenv("ITOLBM_DEPTH", depth)
     m = _pkg.load()
     sim = O.Sim(O.make_case(70, 45, op, 0.8, A, 0.8))
     ctx = m.Context(m.case_desc(70, 45, op, 0.8, A, 0.8))
@@ -100,7 +105,7 @@ def test_compact_raw_sites_at_strip_edges(op, A, monkeypatch):
     nx, ny = 130, 40
     sim = O.Sim(O.make_case(nx, ny, op, 1.0, A, 0.8))
     ctx = m.Context(m.case_desc(nx, ny, op, 1.0, A, 0.8))
-    assert ctx.stored_bytes() == 384
+    assert ctx.stored_bytes() == (608 if op in ("F1", "F2") else 384)
     sim.init_drop(14.0, width=3.45)
     fR = sim.fR.reshape(ny, nx, 19)
     fB = sim.fB.reshape(ny, nx, 19)
