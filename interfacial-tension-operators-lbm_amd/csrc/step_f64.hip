@@ -381,6 +381,7 @@ struct TileCtx {
     gdouble* aux;
     long long S;     // aux stride
     int strip, y0, y1;
+    int seg;         // segment index of the tile within its case
     int kind;        // kernel family of the case
 };
 
@@ -397,6 +398,7 @@ __device__ __forceinline__ bool tile_ctx_at(const StepLaunch& L, int tile, TileC
     const int local = tile - d.tile0;
     t.strip = local % d.nstrip;
     const int seg = local / d.nstrip;
+    t.seg = seg;
     t.y0 = seg * d.ly;
     t.y1 = min(t.y0 + d.ly, d.cc.ny);
     return t.y0 < d.cc.ny;
@@ -764,7 +766,11 @@ struct CPull {
         hoff = hlane ? (unsigned)wrapi(t.strip * 64 + (L.lane < 5 ? -1 : 64), nx) * 8u : 0x80000000u;
         beta = t.cc.beta;
     }
-    __device__ __forceinline__ int slot(int r) const { return (r - first) % 3; }
+    __device__ __forceinline__ int slot(int r) const
+    {
+        const int m = (r - first) % 3;   // rows on either side of `first` (a tile may march up)
+        return m < 0 ? m + 3 : m;
+    }
 
     // live false: nothing is read, through an empty buffer resource; the loads are still issued, so every
     // path leaves the same loads outstanding
@@ -922,54 +928,65 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
         r = v;
     };
 
+    // Even segments march down from their first row, odd ones up from their last: two neighbouring
+    // segments then read the rows beside their shared boundary both at the start or both at the end
+    // of their march, while those rows are in their common L2.  The rows are visited in march order
+    // (d = +1 or -1); the colour-field window is kept in march order and put back in row order for
+    // the stencils, so every site's arithmetic is the same in both directions.
+    const int d = (t.seg & 1) ? -1 : 1;
+    const int s0 = d > 0 ? t.y0 : t.y1 - 1;
     Scal sc, hsc;
     double tot[Q], fn[Q];
     double rRn, rBn;
-    double ph[3], hl[3], hr[3];   // colour field of rows y-1, y, y+1 and of the two halo columns
+    double ph[3], hl[3], hr[3];   // colour field of the rows behind, at and ahead of the current one
     CPop hp;
-    // terms of the stored rows y0 - 2 .. y0 (the senders of pulled row y0 - 1)
-    P.scal_load(t.y0 - 2, sc, hsc);
-    bool Wm = P.rc_write(t.y0 - 2, sc, hsc);
-    P.scal_load(t.y0 - 1, sc, hsc);
-    bool W0 = P.rc_write(t.y0 - 1, sc, hsc);
-    P.scal_load(t.y0, sc, hsc);
-    bool Wp = P.rc_write(t.y0, sc, hsc);
+    // terms of the stored rows s0 - 2d .. s0 (the senders of pulled row s0 - d, behind the first row)
+    P.scal_load(s0 - 2 * d, sc, hsc);
+    bool Wm = P.rc_write(s0 - 2 * d, sc, hsc);
+    P.scal_load(s0 - d, sc, hsc);
+    bool W0 = P.rc_write(s0 - d, sc, hsc);
+    P.scal_load(s0, sc, hsc);
+    bool Wp = P.rc_write(s0, sc, hsc);
     lds_order();
-    P.load_tot(t.y0 - 1, tot);
-    hp = cpop_load(t.in, halo_e0(t.y0 - 1), px, h.i);
-    P.pull_sums(t.y0 - 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
+    P.load_tot(s0 - d, tot);
+    hp = cpop_load(t.in, halo_e0(s0 - d), px, h.i);
+    P.pull_sums(s0 - d, tot, Wm || W0 || Wp, fn, rRn, rBn);
     ph[0] = phase(rRn, rBn);
-    halo_phase(t.y0 - 1, hp, hl[0], hr[0]);
-    // pulled row y0 (senders in stored rows y0 - 1 .. y0 + 1)
-    P.scal_load(t.y0 + 1, sc, hsc);
+    halo_phase(s0 - d, hp, hl[0], hr[0]);
+    // pulled row s0 (senders in stored rows s0 - d .. s0 + d)
+    P.scal_load(s0 + d, sc, hsc);
     Wm = W0; W0 = Wp;
-    lds_order();   // the ring slot of stored row y0 - 2 is written again
-    Wp = P.rc_write(t.y0 + 1, sc, hsc);
+    lds_order();   // the ring slot of stored row s0 - 2d is written again
+    Wp = P.rc_write(s0 + d, sc, hsc);
     lds_order();
-    P.load_tot(t.y0, tot);
-    hp = cpop_load(t.in, halo_e0(t.y0), px, h.i);
-    P.pull_sums(t.y0, tot, Wm || W0 || Wp, fn, rRn, rBn);
+    P.load_tot(s0, tot);
+    hp = cpop_load(t.in, halo_e0(s0), px, h.i);
+    P.pull_sums(s0, tot, Wm || W0 || Wp, fn, rRn, rBn);
     ph[1] = phase(rRn, rBn);
-    halo_phase(t.y0, hp, hl[1], hr[1]);
+    halo_phase(s0, hp, hl[1], hr[1]);
 
-    for (int y = t.y0; y < t.y1; ++y) {
-        // the next row's loads go out first (its pulled totals, the scalars of stored row y + 2 and the
+    for (int k = 0, y = s0; k < t.y1 - t.y0; ++k, y += d) {
+        // the next row's loads go out first (its pulled totals, the scalars of stored row y + 2d and the
         // halo pulls); the gradient-independent part of this row's collision runs while they are in flight
-        P.load_tot(y + 1, tot);
-        P.scal_load(y + 2, sc, hsc);
-        hp = cpop_load(t.in, halo_e0(y + 1), px, h.i);
+        P.load_tot(y + d, tot);
+        P.scal_load(y + 2 * d, sc, hsc);
+        hp = cpop_load(t.in, halo_e0(y + d), px, h.i);
         Site s;
         site_sums_f(s, fn, rRn, rBn);
         if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
 
         Wm = W0; W0 = Wp;
-        lds_order();   // the ring slot of stored row y - 1 is written again
-        Wp = P.rc_write(y + 2, sc, hsc);
+        lds_order();   // the ring slot of stored row y - d is written again
+        Wp = P.rc_write(y + 2 * d, sc, hsc);
         lds_order();
-        P.pull_sums(y + 1, tot, Wm || W0 || Wp, fn, rRn, rBn);
+        P.pull_sums(y + d, tot, Wm || W0 || Wp, fn, rRn, rBn);
         ph[2] = phase(rRn, rBn);
-        halo_phase(y + 1, hp, hl[2], hr[2]);
-        const Nine p9 = gather9h(ph, hl, hr, L.lane);
+        halo_phase(y + d, hp, hl[2], hr[2]);
+        // rows y - 1, y, y + 1
+        const double pr[3] = {d > 0 ? ph[0] : ph[2], ph[1], d > 0 ? ph[2] : ph[0]};
+        const double lr[3] = {d > 0 ? hl[0] : hl[2], hl[1], d > 0 ? hl[2] : hl[0]};
+        const double rr[3] = {d > 0 ? hr[0] : hr[2], hr[1], d > 0 ? hr[2] : hr[0]};
+        const Nine p9 = gather9h(pr, lr, rr, L.lane);
         double gx, gy, gz;
         mediator_gradient(p9, gx, gy, gz);
         const double mod2 = gx * gx + gy * gy + gz * gz;
