@@ -108,7 +108,8 @@ struct lbm_ctx {
     hipStream_t side[N_KINDS] = {};
     hipEvent_t join[N_KINDS] = {};
     bool autotune = true;
-    bool f64_compact = true;           // FP64 P1/P3 and F3 cases in the compact form (ITOLBM_F64_COMPACT=0: not)
+    int f64_compact = -1;              // FP64 P1/P3 and F3 cases' storage form: -1 chosen per launch by its size,
+                                       // 0 always the full form, 1 always the compact one (ITOLBM_F64_COMPACT)
     int graph_steps = 32;              // steps per captured graph in lbm_step (0: no graphs)
     StepGraph graph;
     long long bytes = 0;
@@ -200,6 +201,14 @@ long long stride_pad()
     return v;
 }
 
+// A launch of FP64 P1/P3 or F3 cases covering this many sites or more runs them in the compact form
+// (24 stored doubles per site instead of 38), a smaller one in the full form.  The compact kernel moves
+// 37 % fewer bytes but makes a longer chain of dependent work per row, which a launch of few wavefronts
+// cannot hide: measured per step, eight 192^2 cases (295 k sites) 39.2 us compact against 44.3 us full,
+// one 448^2 case (201 k) 32.1 against 33.7 us, one 384^2 case (147 k) 28.5 against 24.8 us, one 64^2
+// case 13.6 against 7.3 us.
+constexpr long long COMPACT_MIN_SITES = 196608;
+
 // Elements added to the slot pitch of the FP64 compact form (px = nx + pad).  ITOLBM_CPAD sets it
 // (0 .. 4096); the default is 0.
 int compact_pad(int nx)
@@ -210,6 +219,12 @@ int compact_pad(int nx)
         if (v >= 0 && v <= 4096) return (int)v;
     }
     return 0;
+}
+
+bool want_compact(const lbm_ctx* c, long long launch_sites)
+{
+    if (c->f64_compact >= 0) return c->f64_compact == 1;
+    return launch_sites >= COMPACT_MIN_SITES;
 }
 
 // Whether the context's stream is capturing a HIP graph (a caller's capture).  Work that allocates,
@@ -472,7 +487,10 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         lbm_destroy(c);
         return LBM_E_ARG;
     }
-    if (const char* e = std::getenv("ITOLBM_F64_COMPACT")) c->f64_compact = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("ITOLBM_F64_COMPACT")) {
+        if (!std::strcmp(e, "0")) c->f64_compact = 0;
+        else if (!std::strcmp(e, "1")) c->f64_compact = 1;
+    }
     c->cases.resize(n_cases);
     for (int i = 0; i < n_cases; ++i) {
         const lbm_case_desc& d = cases[i];
@@ -493,7 +511,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.elem = elem;
         k.sym = sym;
         k.nslot = n_slots(sym);
-        k.compact = elem == 8 && c->f64_compact && (k.kind == KIND_P || k.kind == KIND_F3);
+        k.compact = false;   // the form of the first launch is set below
         k.cc.px = d.nx + compact_pad(d.nx);
         // state buffers hold 2 x nslot values per site in row blocks (the step kernels index them by
         // nx and ny only), so their size follows from n and not from the padded stride S; an FP64
@@ -510,6 +528,13 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
     }
     for (int kd = 0; kd < N_KINDS; ++kd) c->family[kd].kind = kd;
     c->fused.kind = KIND_FUSED;
+    // FP64 P1/P3 and F3 cases start in the form lbm_step's launch of their family would use
+    for (CaseState& k : c->cases) {
+        if (k.elem != 8 || (k.kind != KIND_P && k.kind != KIND_F3)) continue;
+        long long sites = 0;
+        for (const CaseState& o : c->cases) sites += o.kind == k.kind ? o.n : 0;
+        k.compact = want_compact(c, sites);
+    }
     c->single.resize(n_cases);
     for (int i = 0; i < n_cases; ++i) {
         c->family[c->cases[i].kind].members.push_back(i);
@@ -775,6 +800,35 @@ int lbm_download_state(lbm_ctx* c, int id, int which, double* fR, double* fB)
 
 namespace {
 
+void drop_graph(lbm_ctx* c);
+
+// Converts an FP64 P1/P3 or F3 case to the compact form (to = true) or the full one on the device: the
+// state moves to the case's other buffer, every population unchanged.  Launch tables see the buffer
+// change and are written again; a captured graph of the old form is dropped.
+int set_form(lbm_ctx* c, CaseState& k, bool to)
+{
+    if (k.elem != 8 || k.compact == to || (to && k.kind != KIND_P && k.kind != KIND_F3)) return LBM_OK;
+    if (capturing(c)) return refuse_capture(c, "changing a case's FP64 storage form (a step of another launch size)");
+    HIPCHK(c, launch_f64_convert(k.pop[k.cur], k.pop[k.cur ^ 1], to, k.cc.beta, k.d.nx, k.cc.px, k.n, c->stream));
+    k.cur ^= 1;
+    k.compact = to;
+    drop_graph(c);
+    return LBM_OK;
+}
+
+// The form a launch of the P1/P3 or F3 cases in `g` uses: compact when it covers COMPACT_MIN_SITES
+// sites or more (or as ITOLBM_F64_COMPACT fixes); its members are converted to it.
+int group_form(lbm_ctx* c, const Group& g)
+{
+    if (g.kind != KIND_P && g.kind != KIND_F3) return LBM_OK;
+    long long sites = 0;
+    for (int id : g.members) sites += c->cases[id].n;
+    const bool to = want_compact(c, sites);
+    for (int id : g.members)
+        if (int r = set_form(c, c->cases[id], to)) return r;
+    return LBM_OK;
+}
+
 // (Re)writes a group's case table when a member's state buffer or aux pointer moved since.
 int group_sync(lbm_ctx* c, Group& g)
 {
@@ -885,7 +939,12 @@ int launch_group(lbm_ctx* c, Group& g, int parity, int record, hipStream_t st)
     a.depth = g.depth;
     const CaseState& k0 = c->cases[g.members[0]];   // a context has one storage form
     a.ntiles = g.ntiles;
-    a.compact = (k0.elem == 8 && c->f64_compact) ? 1 : 0;
+    a.compact = 0;   // the form of the launch's P1/P3 and F3 cases (all alike)
+    for (int id : g.members)
+        if (c->cases[id].kind == KIND_P || c->cases[id].kind == KIND_F3) {
+            a.compact = c->cases[id].compact ? 1 : 0;
+            break;
+        }
     if (g.kind == KIND_FUSED) {   // every strip family of the FP64 context in one launch
         const int cap = step_f64_fused_capacity(a.compact != 0);
         const int grid = (c->fuse_mode == 2 && cap > 0) ? std::min(cap, (g.ntiles + 7) / 8 * 8) : g.ntiles;
@@ -1140,7 +1199,12 @@ int fuse_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
 {
     int strip = 0;
     for (const Group* g : act) strip += g->kind < KIND_GEO;
-    if (c->cases.empty() || c->cases[0].elem != 8 || strip == 0) {
+    // one launch runs one FP64 form for its P1/P3 and F3 cases: when their families' launch sizes put
+    // them in different forms, each family keeps its own launch
+    int forms = 0;
+    for (const CaseState& k : c->cases)
+        if (k.active && (k.kind == KIND_P || k.kind == KIND_F3)) forms |= k.compact ? 2 : 1;
+    if (c->cases.empty() || c->cases[0].elem != 8 || strip == 0 || forms == 3) {
         c->fuse_mode = 0;
         return LBM_OK;
     }
@@ -1246,6 +1310,7 @@ int lbm_step_case(lbm_ctx* c, int id, int n_steps, int flags)
     const bool cap = capturing(c);
     if (cap && (n_steps & 1)) return fail(c, LBM_E_ARG, "a captured step call must advance an even number of steps");
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
+    if (int r = group_form(c, g)) return r;
     if (int r = group_prepare(c, g, rec)) return r;
     int left = n_steps;
     if (!cap && needs_tuning(c, g)) {   // as lbm_step: the tile shape is tuned on the call's own steps
@@ -1278,6 +1343,7 @@ int lbm_step_case_noalloc(lbm_ctx* c, int id, int n_steps, int flags)
                                     "with LBM_STEP_RECORD_AUX allocates");
     if (!c->scratch) return fail(c, LBM_E_STATE, "no static scratch on this device");
     const Group& g = c->single[id];
+    if (int r = group_form(c, g)) return r;   // a conversion kernel when the form changes: nothing allocated
     const int ly = g.ly.empty() ? group_rows(c->cases, g.members) : g.ly[0];
     CaseDev d{};
     d.cc = k.cc;
@@ -1326,6 +1392,8 @@ int lbm_step(lbm_ctx* c, int n_steps, int flags)
     if (cap && (n_steps & 1)) return fail(c, LBM_E_ARG, "a captured step call must advance an even number of steps");
     const bool rec = (flags & LBM_STEP_RECORD_AUX) != 0;
     bool untuned = false;
+    for (Group* g : act)
+        if (int r = group_form(c, *g)) return r;
     for (Group* g : act) {
         if (int r = group_prepare(c, *g, rec)) return r;
         if (!cap && needs_tuning(c, *g)) untuned = true;
@@ -1501,7 +1569,7 @@ int lbm_set_geometry(lbm_ctx* c, int id, const int* meio, double wett_R)
         // the case changes kernel family: both families' members and tables are redone
         const int old = k.kind;
         k.kind = kind;
-        k.compact = c->f64_compact && (kind == KIND_P || kind == KIND_F3);
+        k.compact = false;   // uploaded below in the full form; a later step chooses the form
         for (int kd : {old, kind}) {
             Group& g = c->family[kd];
             g.members.clear();

@@ -820,10 +820,10 @@ struct CPull {
         const auto& Rm = rc[slot(Y - 1)];
         const auto& R0 = rc[slot(Y)];
         const auto& Rp = rc[slot(Y + 1)];
+        if (!raw) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            double r, b;
-            if (!raw) {
+            for (int i = 0; i < Q; ++i) {
+                double r, b;
                 const auto& Rs = CY[i] > 0 ? Rm : (CY[i] < 0 ? Rp : R0);
                 const int c = L.lane + 1 - CX[i];
                 const double cR = Rs[0][c];
@@ -835,19 +835,33 @@ struct CPull {
                 ITOLBM_CS(14) ITOLBM_CS(15) ITOLBM_CS(16) ITOLBM_CS(17) default: csplit<18>(tot[i], cR, 1.0 - cR, tk, r, b);
 #undef ITOLBM_CS
                 }
-            } else {
-                const long long e0 = csender(px, ny, L, Y, i);
+                if (i == 0) {
+                    rR = r;
+                    rB = b;
+                } else {
+                    rR = rR + r;
+                    rB = rB + b;
+                }
+                f[i] = r + b;
+            }
+        } else {
+            // a raw sender among the rows (rare): each population from its sender's stored values, in a
+            // rolled loop (unrolled, its 19 copies made the kernel several times larger, and launches of
+            // a few wavefronts, which run its code once from a cold instruction cache, slower)
+            const int xm = L.xm, xp = L.xp, xc = L.col;
+#pragma unroll 1
+            for (int i = 0; i < Q; ++i) {
+                const int xs = cx_of(i) > 0 ? xm : (cx_of(i) < 0 ? xp : xc);
+                const long long e0 = (long long)wrapi(Y - cy_of(i), ny) * rb + xs;
                 const CPop p = cpop_load(in, e0, px, i);
+                double r, b;
                 cpop_split(in, e0, px, beta, i, p, r, b);
+                rR = (i == 0) ? r : rR + r;
+                rB = (i == 0) ? b : rB + b;
+                const double v = r + b;
+#pragma unroll
+                for (int k = 0; k < Q; ++k) f[k] = (k == i) ? v : f[k];   // f[i] = v, f kept in registers
             }
-            if (i == 0) {
-                rR = r;
-                rB = b;
-            } else {
-                rR = rR + r;
-                rB = rB + b;
-            }
-            f[i] = r + b;
         }
     }
     // the site's own pulled populations stored raw (a site the collision leaves untouched); a rolled
@@ -940,40 +954,30 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
     double rRn, rBn;
     double ph[3], hl[3], hr[3];   // colour field of the rows behind, at and ahead of the current one
     CPop hp;
-    // terms of the stored rows s0 - 2d .. s0 (the senders of pulled row s0 - d, behind the first row)
-    P.scal_load(s0 - 2 * d, sc, hsc);
-    bool Wm = P.rc_write(s0 - 2 * d, sc, hsc);
-    P.scal_load(s0 - d, sc, hsc);
-    bool W0 = P.rc_write(s0 - d, sc, hsc);
-    P.scal_load(s0, sc, hsc);
-    bool Wp = P.rc_write(s0, sc, hsc);
-    lds_order();
-    P.load_tot(s0 - d, tot);
-    hp = cpop_load(t.in, halo_e0(s0 - d), px, h.i);
-    P.pull_sums(s0 - d, tot, Wm || W0 || Wp, fn, rRn, rBn);
-    ph[0] = phase(rRn, rBn);
-    halo_phase(s0 - d, hp, hl[0], hr[0]);
-    // pulled row s0 (senders in stored rows s0 - d .. s0 + d)
-    P.scal_load(s0 + d, sc, hsc);
-    Wm = W0; W0 = Wp;
-    lds_order();   // the ring slot of stored row s0 - 2d is written again
-    Wp = P.rc_write(s0 + d, sc, hsc);
-    lds_order();
-    P.load_tot(s0, tot);
-    hp = cpop_load(t.in, halo_e0(s0), px, h.i);
-    P.pull_sums(s0, tot, Wm || W0 || Wp, fn, rRn, rBn);
-    ph[1] = phase(rRn, rBn);
-    halo_phase(s0, hp, hl[1], hr[1]);
-
-    for (int k = 0, y = s0; k < t.y1 - t.y0; ++k, y += d) {
+    bool Wm, W0, Wp;   // a raw site in the ring's stored rows (oldest, middle, newest in march order)
+    // The loop's first two passes (k = -2, -1) only pull rows s0 - d and s0, the rows the first
+    // collision needs besides its own next one: one copy of the pulling code serves them all.
+    // Stored rows s0 - 2d and s0 - d go into the ring first; every pass adds the next one.
+    {
+        Scal sa, ha;
+        P.scal_load(s0 - 2 * d, sc, hsc);
+        P.scal_load(s0 - d, sa, ha);
+        Wm = false;
+        W0 = P.rc_write(s0 - 2 * d, sc, hsc);
+        Wp = P.rc_write(s0 - d, sa, ha);
+    }
+    for (int k = -2, y = s0 - 2 * d; k < t.y1 - t.y0; ++k, y += d) {
         // the next row's loads go out first (its pulled totals, the scalars of stored row y + 2d and the
         // halo pulls); the gradient-independent part of this row's collision runs while they are in flight
         P.load_tot(y + d, tot);
         P.scal_load(y + 2 * d, sc, hsc);
         hp = cpop_load(t.in, halo_e0(y + d), px, h.i);
+        const bool collide = k >= 0;
         Site s;
-        site_sums_f(s, fn, rRn, rBn);
-        if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
+        if (collide) {
+            site_sums_f(s, fn, rRn, rBn);
+            if constexpr (!F3) bgk<false>(cc, s, 0.0, 0.0, 0.0);
+        }
 
         Wm = W0; W0 = Wp;
         lds_order();   // the ring slot of stored row y - d is written again
@@ -982,6 +986,12 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
         P.pull_sums(y + d, tot, Wm || W0 || Wp, fn, rRn, rBn);
         ph[2] = phase(rRn, rBn);
         halo_phase(y + d, hp, hl[2], hr[2]);
+        if (!collide) {
+            ph[0] = ph[1]; ph[1] = ph[2];
+            hl[0] = hl[1]; hl[1] = hl[2];
+            hr[0] = hr[1]; hr[1] = hr[2];
+            continue;
+        }
         // rows y - 1, y, y + 1
         const double pr[3] = {d > 0 ? ph[0] : ph[2], ph[1], d > 0 ? ph[2] : ph[0]};
         const double lr[3] = {d > 0 ? hl[0] : hl[2], hl[1], d > 0 ? hl[2] : hl[0]};
@@ -1604,6 +1614,45 @@ __global__ void __launch_bounds__(256) f64c_rows_to_soa(const double* __restrict
     }
 }
 }  // namespace
+
+namespace {
+// One case's state from the full FP64 form to the compact one (every site raw) or back (each site's 38
+// populations formed from its stored values): the populations are unchanged, only their layout.
+__global__ void __launch_bounds__(256) f64_convert(const double* __restrict__ in, double* __restrict__ out,
+                                                   int to_compact, double beta, int nx, int px, long long n)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+        const long long y = p / nx, x = p - y * nx;
+        const long long f0 = y * (2LL * Q * nx) + x;   // full form: row blocks [y][colour][19][nx]
+        const long long c0 = y * cblock_doubles(px) + x;
+        if (to_compact) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                out[c0 + (long long)i * px] = in[f0 + (long long)i * nx];
+                out[c0 + (long long)cs_raw_b(i) * px] = in[f0 + (long long)(Q + i) * nx];
+            }
+            out[c0 + (long long)CS_RHO * px] = CS_RAW;
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const CPop c = cpop_load((gcdouble*)in, c0, px, i);
+                double r, b;
+                cpop_split((gcdouble*)in, c0, px, beta, i, c, r, b);
+                out[f0 + (long long)i * nx] = r;
+                out[f0 + (long long)(Q + i) * nx] = b;
+            }
+        }
+    }
+}
+}  // namespace
+
+hipError_t launch_f64_convert(const double* in, double* out, bool to_compact, double beta, int nx, int px, long long n,
+                              hipStream_t stream)
+{
+    const int nb = (int)std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(f64_convert, dim3(nb), dim3(256), 0, stream, in, out, to_compact ? 1 : 0, beta, nx, px, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, int px, long long n,
                                   long long S, double* soa, hipStream_t stream)

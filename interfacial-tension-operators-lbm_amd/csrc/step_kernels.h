@@ -101,6 +101,11 @@ struct GeoLaunch {
 };
 hipError_t launch_step_geo(int kind, const GeoLaunch& a, hipStream_t stream);
 
+// One case's FP64 state from the full form to the compact one (every site raw) or back, from `in` to
+// `out` (the case's other state buffer); the 38 populations of every site are unchanged
+hipError_t launch_f64_convert(const double* in, double* out, bool to_compact, double beta, int nx, int px, long long n,
+                              hipStream_t stream);
+
 // FP64 state in row blocks [y][colour][19][nx], or in the compact form [y][CSLOTS][px] (recolouring
 // factor beta, slot pitch px), -> [colour][19][S]: every site's post-collision populations at the site
 hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, int nx, int px, long long n,

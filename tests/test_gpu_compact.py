@@ -2,10 +2,15 @@
 
 In the compact form a site stores its 19 post-collision totals and the five scalars its recolouring
 is formed from (cR, rho, the mediator gradient); the site that pulls a population forms its two colour
-parts with the sender's own expressions.  These tests check, through the C ABI:
+parts with the sender's own expressions.  A launch covering many sites runs its cases in the compact
+form, a small one in the full form; ITOLBM_F64_COMPACT=1 / 0 fixes either.  These tests check, through
+the C ABI:
 
-- the bytes per lattice update each case reports (lbm_stored_bytes);
-- the full FP64 form (ITOLBM_F64_COMPACT=0) still equals the oracle bit for bit;
+- the bytes per lattice update each case reports (lbm_stored_bytes), and the form each launch size
+  chooses;
+- drops of every P/F3 operator in the compact form equal the oracle bit for bit, and so does the full
+  form;
+- cases moved between the forms by launches of different sizes step exactly as in one form;
 - raw sites (near-empty sites the collision leaves untouched, Operadores_tensao.cpp:347) placed on
   strip edges, on the box's periodic seam and across segment boundaries, where the kernel falls back
   to per-population loads, give the oracle's bits;
@@ -59,6 +64,10 @@ def test_stored_bytes(monkeypatch):
     m = _pkg.load()
     descs = [m.case_desc(40, 30, op, 1.0, A, 0.8) for op, A in OPS7]
     expect = {"P1": 384, "P3": 384, "F3": 384, "F1": 608, "F2": 608}
+    ctx = m.Context(descs)   # small launches: the full form
+    assert [ctx.stored_bytes(case=i) for i in range(len(descs))] == [608] * len(descs)
+    ctx.close()
+    monkeypatch.setenv("ITOLBM_F64_COMPACT", "1")
     ctx = m.Context(descs)
     got = [ctx.stored_bytes(case=i) for i in range(len(descs))]
     ctx.close()
@@ -101,6 +110,7 @@ def test_compact_raw_sites_at_strip_edges(op, A, monkeypatch):
     and across 4-row segment boundaries; rows pulling from them take the per-population path."""
     say(f"[gpu] compact raw sites 130x40 {op}")
     monkeypatch.setenv("ITOLBM_LY", "4")
+    monkeypatch.setenv("ITOLBM_F64_COMPACT", "1")
     m = _pkg.load()
     nx, ny = 130, 40
     sim = O.Sim(O.make_case(nx, ny, op, 1.0, A, 0.8))
@@ -119,6 +129,55 @@ def test_compact_raw_sites_at_strip_edges(op, A, monkeypatch):
     ctx.upload(sim.fR, sim.fB)
     oracle_steps(m, sim, ctx, [1, 2, 3, 6])
     ctx.close()
+
+
+@pytest.mark.parametrize("op,A", [OPS7[0], OPS7[1], OPS7[2], OPS7[3], OPS7[6]])
+def test_compact_drop_bitwise(op, A, monkeypatch):
+    """A diffuse drop and a sharp translating one in the compact form, against the oracle."""
+    say(f"[gpu] compact form parity {op}")
+    monkeypatch.setenv("ITOLBM_F64_COMPACT", "1")
+    m = _pkg.load()
+    for nx, ny, R, w, v0, steps in ((70, 45, 11.0, 3.45, 0.0, [1, 2, 7, 20]), (131, 33, 9.0, -1.0, 0.05, [1, 10])):
+        sim = O.Sim(O.make_case(nx, ny, op, 0.8, A, 0.8))
+        ctx = m.Context(m.case_desc(nx, ny, op, 0.8, A, 0.8))
+        assert ctx.stored_bytes() == 384
+        sim.init_drop(R, width=w, v0=v0)
+        ctx.upload(sim.fR, sim.fB)
+        oracle_steps(m, sim, ctx, steps)
+        ctx.close()
+
+
+def test_form_follows_launch_size(monkeypatch):
+    """Eight 192^2 cases (295 k sites per lbm_step launch: the compact form) stepped by lbm_step, then
+    one of them by lbm_step_case (a 37 k-site launch: the full form, converted on the device), then
+    all by lbm_step again: the same bits as the same steps in the full form throughout."""
+    say("[gpu] storage form by launch size, 8 x 192^2")
+    m = _pkg.load()
+    descs = [m.case_desc(192, 192, "P3 0.5", tau, 0.225, 0.8) for tau in (0.6, 1.0) * 4]
+    out = {}
+    for form in (None, "0"):
+        if form is None:
+            monkeypatch.delenv("ITOLBM_F64_COMPACT", raising=False)
+        else:
+            monkeypatch.setenv("ITOLBM_F64_COMPACT", form)
+        ctx = m.Context(descs)
+        for i in range(len(descs)):
+            ctx.init_drop(40.0, width=-1.0, v0=0.02, case=i)
+        ctx.step(12)
+        b1 = [ctx.stored_bytes(case=i) for i in range(len(descs))]
+        ctx.step(9, case=3)
+        b2 = ctx.stored_bytes(case=3)
+        ctx.step(6, record=True)
+        b3 = ctx.stored_bytes(case=3)
+        if form is None:
+            assert b1 == [384] * 8 and b2 == 608 and b3 == 384, (b1, b2, b3)
+        else:
+            assert b1 == [608] * 8 and b2 == 608 and b3 == 608
+        out[form] = [ctx.download(m.STATE_POST_COLLISION, case=i) for i in range(len(descs))]
+        ctx.close()
+    for i in range(len(descs)):
+        assert_same(out[None][i][0], out["0"][i][0], f"case {i} red")
+        assert_same(out[None][i][1], out["0"][i][1], f"case {i} blue")
 
 
 def test_compact_equals_full_batch(monkeypatch):
