@@ -138,10 +138,12 @@ int lbm_download_state(lbm_ctx* ctx, int case_id, int which, double* fR, double*
 
 /* Advances every case by n_steps time steps (Imm_OPS.cpp:246-287 + swap :368-371), stream-ordered
  * and asynchronous.  flags: LBM_STEP_RECORD_AUX keeps the last step's mediator momentum and
- * class-F force for the diagnostics below. */
+ * class-F force for the diagnostics below.  LBM_PREC_F64: the P1/P3 and F3 cases of a family launch
+ * covering 196,608 sites or more are stepped in the compact form (see lbm_stored_bytes), the others
+ * in the full form; a case in the other form is converted first, exactly, on the device. */
 int lbm_step(lbm_ctx* ctx, int n_steps, int flags);
 
-/* Advances one case only. */
+/* Advances one case only (the form chosen by the case's own size, as for lbm_step's launches). */
 int lbm_step_case(lbm_ctx* ctx, int case_id, int n_steps, int flags);
 
 /* As lbm_step_case, allocating nothing.  No counterpart in the reference: it is for callers that
@@ -153,7 +155,9 @@ int lbm_step_case(lbm_ctx* ctx, int case_id, int n_steps, int flags);
  * more than 2^20 tiles (64-column strips x row segments) is refused with LBM_E_ARG;
  * LBM_STEP_RECORD_AUX needs the aux fields that a first recording lbm_step / lbm_step_case allocates
  * (LBM_E_STATE otherwise).  Every call on one GPU shares that GPU's array, so calls that overlap on
- * one GPU (other streams, other host threads) race; calls queued on one stream do not. */
+ * one GPU (other streams, other host threads) race; calls queued on one stream do not.  An FP64 case
+ * whose storage form must change (see lbm_step) is first converted by one kernel launch, nothing
+ * allocated; under a graph capture that is refused (LBM_E_STATE). */
 int lbm_step_case_noalloc(lbm_ctx* ctx, int case_id, int n_steps, int flags);
 
 /* Waits for all queued work. */

@@ -3,6 +3,8 @@
 
   * ITOLBM_PAD: a stride pad that is negative or not a whole number is refused by lbm_create, and
     the state buffers' size does not depend on the pad;
+  * ITOLBM_CPAD: likewise for the slot-pitch pad of the FP64 compact form, whose state buffers hold
+    39 doubles per site of the padded pitch (either FP64 form fits them);
   * graph capture: while the context's stream is capturing (a caller's graph), every entry point
     that would allocate, free, copy from host memory or synchronise returns LBM_E_STATE and does none
     of it; stepping a prepared context queues kernel launches only, and needs an even step count.
@@ -76,6 +78,20 @@ def test_stride_pad_checked(mock, pad, monkeypatch):
     if ok:
         # two FP32 state buffers of 2 x 19 floats per site, whatever the pad
         assert L.lbm_device_bytes(h) == 2 * (4 * 2 * 19 * 40 * 30)
+        assert L.lbm_destroy(h) == LBM_OK
+
+
+@pytest.mark.parametrize("pad", ["-8", "abc", "3.5", "", "0", "64", "4097"])
+def test_compact_pad_checked(mock, pad, monkeypatch):
+    L, m = mock
+    monkeypatch.setenv("ITOLBM_CPAD", pad)
+    rc, h = create(L, m, [m.case_desc(40, 30, "P3 0.5")], prec=0)
+    ok = pad in ("", "0", "64")
+    assert rc == (LBM_OK if ok else LBM_E_ARG), (pad, rc)
+    if ok:
+        px = 40 + (int(pad) if pad else 0)
+        # two FP64 state buffers of 39 doubles per site of the padded pitch
+        assert L.lbm_device_bytes(h) == 2 * (8 * 39 * px * 30)
         assert L.lbm_destroy(h) == LBM_OK
 
 
