@@ -209,13 +209,12 @@ long long stride_pad()
 // case 13.6 against 7.3 us.
 constexpr long long COMPACT_MIN_SITES = 196608;
 
-// Elements added to the slot pitch of the FP64 compact form (px = nx + pad).  ITOLBM_CPAD sets it: a
-// whole number in [0, 4096] (otherwise lbm_create fails); the default is 0 (padding the pitch measured
-// no faster on the 512^2 batch: 187.2 us per P-family launch unpadded, 187.6-191.8 us with 8-128).
+// Elements added to the slot pitch of the FP64 compact form (px = nx + pad), from ITOLBM_CPAD: a whole
+// number in [0, 4096] (-1: refused, lbm_create fails), or -2 when unset (compact_pad_of decides).
 long long compact_pad()
 {
     const char* e = std::getenv("ITOLBM_CPAD");
-    if (!e || !*e) return 0;
+    if (!e || !*e) return -2;
     char* end = nullptr;
     errno = 0;
     const long long v = std::strtoll(e, &end, 10);
@@ -224,6 +223,18 @@ long long compact_pad()
         return -1;
     }
     return v;
+}
+
+// The pad of an nx-column case: ITOLBM_CPAD if set, else 64 elements when the pitch is a multiple of
+// 64 KiB (nx a multiple of 8192), 0 otherwise.  With such a pitch every slot of a row block falls on
+// the same offset of a 64 KiB window: the 8192^2 case ran 5.67-5.95 ms per step unpadded and
+// 5.06-5.37 ms with the pad (8- to 128-row segments), while 512^2 (187.2 us per eight-case launch
+// unpadded, 187.6-191.8 us padded by 8-128), 2048^2 (0.316 against 0.333 ms) and 4096^2 (1.256 against
+// 1.289 ms) ran best unpadded.
+int compact_pad_of(long long env_pad, int nx)
+{
+    if (env_pad >= 0) return (int)env_pad;
+    return (nx % 8192 == 0) ? 64 : 0;
 }
 
 bool want_compact(const lbm_ctx* c, long long launch_sites)
@@ -488,7 +499,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         return LBM_E_HIP;
     }
     const long long pad = stride_pad(), cpad = compact_pad();
-    if (pad < 0 || cpad < 0) {
+    if (pad < 0 || cpad == -1) {
         lbm_destroy(c);
         return LBM_E_ARG;
     }
@@ -502,7 +513,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         if (d.nx < 1 || d.ny < 1 || !valid_op(d.op) || !(d.tau > 0.0) || !std::isfinite(d.A) ||
             !std::isfinite(d.beta) || (d.op == OP_F3 && !(d.xi != 0.0)) ||
             (long long)d.nx * d.ny > (1LL << 31) / Q ||
-            ((long long)d.nx + cpad) * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
+            ((long long)d.nx + compact_pad_of(cpad, d.nx)) * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
                                                                                        // offsets are 32-bit
             lbm_destroy(c);
             return LBM_E_ARG;
@@ -517,7 +528,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.sym = sym;
         k.nslot = n_slots(sym);
         k.compact = false;   // the form of the first launch is set below
-        k.cc.px = d.nx + (int)cpad;
+        k.cc.px = d.nx + compact_pad_of(cpad, d.nx);
         // state buffers hold 2 x nslot values per site in row blocks (the step kernels index them by
         // nx and ny only), so their size follows from n and not from the padded stride S; an FP64
         // buffer holds either form (F64_SLOTS doubles per site of the padded pitch px), so a case can

@@ -4,7 +4,8 @@
   * ITOLBM_PAD: a stride pad that is negative or not a whole number is refused by lbm_create, and
     the state buffers' size does not depend on the pad;
   * ITOLBM_CPAD: likewise for the slot-pitch pad of the FP64 compact form, whose state buffers hold
-    39 doubles per site of the padded pitch (either FP64 form fits them);
+    39 doubles per site of the padded pitch (either FP64 form fits them); unset, widths that make the
+    pitch a multiple of 64 KiB are padded by 64 elements;
   * graph capture: while the context's stream is capturing (a caller's graph), every entry point
     that would allocate, free, copy from host memory or synchronise returns LBM_E_STATE and does none
     of it; stepping a prepared context queues kernel launches only, and needs an even step count.
@@ -93,6 +94,17 @@ def test_compact_pad_checked(mock, pad, monkeypatch):
         # two FP64 state buffers of 39 doubles per site of the padded pitch
         assert L.lbm_device_bytes(h) == 2 * (8 * 39 * px * 30)
         assert L.lbm_destroy(h) == LBM_OK
+
+
+@pytest.mark.parametrize("nx,pad", [(8192, 64), (16384, 64), (4096, 0), (8200, 0)])
+def test_compact_pad_default(mock, nx, pad, monkeypatch):
+    """Unset ITOLBM_CPAD: box widths that make the slot pitch a multiple of 64 KiB are padded by 64."""
+    L, m = mock
+    monkeypatch.delenv("ITOLBM_CPAD", raising=False)
+    rc, h = create(L, m, [m.case_desc(nx, 2, "P3 0.5")], prec=0)
+    assert rc == LBM_OK
+    assert L.lbm_device_bytes(h) == 2 * (8 * 39 * (nx + pad) * 2)
+    assert L.lbm_destroy(h) == LBM_OK
 
 
 def test_capture_guard(mock):
