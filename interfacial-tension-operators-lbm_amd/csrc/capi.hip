@@ -225,16 +225,17 @@ long long compact_pad()
     return v;
 }
 
-// The pad of an nx-column case: ITOLBM_CPAD if set, else 64 elements when the pitch is a multiple of
-// 64 KiB (nx a multiple of 8192), 0 otherwise.  With such a pitch every slot of a row block falls on
-// the same offset of a 64 KiB window: the 8192^2 case ran 5.67-5.95 ms per step unpadded and
-// 5.06-5.37 ms with the pad (8- to 128-row segments), while 512^2 (187.2 us per eight-case launch
-// unpadded, 187.6-191.8 us padded by 8-128), 2048^2 (0.316 against 0.333 ms) and 4096^2 (1.256 against
-// 1.289 ms) ran best unpadded.
-int compact_pad_of(long long env_pad, int nx)
+// The pad of the slot pitch of an nx-column case whose slots hold `slot_bytes`-byte elements (FP64
+// compact and FP32 forms 8, FP16 4): ITOLBM_CPAD if set, else 64 elements when the pitch is a multiple
+// of 64 KiB, 0 otherwise.  With such a pitch every slot of a row block falls on the same offset of a
+// 64 KiB window: the 8192^2 FP64 compact case ran 5.67-5.95 ms per step unpadded and 5.06-5.37 ms with
+// the pad (8- to 128-row segments), while 512^2 (187.2 us per eight-case launch unpadded, 187.6-191.8 us
+// padded by 8-128), 2048^2 (0.316 against 0.333 ms) and 4096^2 (1.256 against 1.289 ms) ran best
+// unpadded.
+int compact_pad_of(long long env_pad, int nx, int slot_bytes)
 {
     if (env_pad >= 0) return (int)env_pad;
-    return (nx % 8192 == 0) ? 64 : 0;
+    return ((long long)nx * slot_bytes % 65536 == 0) ? 64 : 0;
 }
 
 bool want_compact(const lbm_ctx* c, long long launch_sites)
@@ -386,9 +387,9 @@ long long gcdll(long long a, long long b)
 // minority fraction copysign((1 - |e|) / 2, e) as a half (the device encoding of the same pair).
 int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
 {
-    const int nx = k.d.nx, ns = k.nslot;
+    const int nx = k.d.nx, ns = k.nslot, px = k.cc.px;
     const LpRef rf = make_lp_ref(k.cc.ref_mid, k.cc.ref_half);
-    std::vector<float> a(2 * (size_t)ns * k.n);
+    std::vector<float> a(2 * (size_t)ns * px * k.d.ny, 0.0f);   // rows of the padded pitch px
     // z-symmetric storage: a state whose mirror populations differ beyond rounding cannot be held
     if (k.sym)
         for (int i = 0; i < Q; ++i) {
@@ -417,7 +418,7 @@ int upload_lp(lbm_ctx* c, CaseState& k, const std::vector<double>& h)
                     }
             }
             const double f = r + b;
-            const size_t j = 2 * (size_t)((y * ns + slot_of(k.sym, i)) * nx + x);
+            const size_t j = 2 * (size_t)((y * ns + slot_of(k.sym, i)) * px + x);
             const float e = (float)((f != 0.0) ? (r - b) / f : 0.0);
             a[j] = (float)((f - w) * (double)SCL[cl]) - std::fma(rf.half(cl), e, rf.mid(cl));   // as the device
             a[j + 1] = e;
@@ -454,7 +455,7 @@ int state_f64(lbm_ctx* c, CaseState& k, const double** out)
         HIPCHK(c, launch_f64_rows_to_soa((const double*)k.pop[k.cur], k.compact, k.cc.beta, k.d.nx, k.cc.px, k.n, k.S,
                                          k.f64, c->stream));
     } else {
-        HIPCHK(c, launch_lp_to_f64(k.elem, k.sym, k.pop[k.cur], k.d.nx, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
+        HIPCHK(c, launch_lp_to_f64(k.elem, k.sym, k.pop[k.cur], k.d.nx, k.cc.px, k.S, k.n, k.cc.ref_mid, k.cc.ref_half, k.f64,
                                    k.f64 + Q * k.S, c->stream));
     }
     *out = k.f64;
@@ -513,7 +514,7 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         if (d.nx < 1 || d.ny < 1 || !valid_op(d.op) || !(d.tau > 0.0) || !std::isfinite(d.A) ||
             !std::isfinite(d.beta) || (d.op == OP_F3 && !(d.xi != 0.0)) ||
             (long long)d.nx * d.ny > (1LL << 31) / Q ||
-            ((long long)d.nx + compact_pad_of(cpad, d.nx)) * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
+            ((long long)d.nx + compact_pad_of(cpad, d.nx, 8)) * F64_SLOTS * (long long)sizeof(double) >= (1LL << 31)) {   // a row block's byte
                                                                                        // offsets are 32-bit
             lbm_destroy(c);
             return LBM_E_ARG;
@@ -528,13 +529,14 @@ int lbm_create(const lbm_case_desc* cases, int n_cases, int precision, lbm_ctx**
         k.sym = sym;
         k.nslot = n_slots(sym);
         k.compact = false;   // the form of the first launch is set below
-        k.cc.px = d.nx + compact_pad_of(cpad, d.nx);
+        // slot pitch of the FP64 compact form (8-byte slots) and of the FP32 / FP16 forms (8- / 4-byte pairs)
+        k.cc.px = d.nx + compact_pad_of(cpad, d.nx, elem == 2 ? 4 : 8);
         // state buffers hold 2 x nslot values per site in row blocks (the step kernels index them by
         // nx and ny only), so their size follows from n and not from the padded stride S; an FP64
         // buffer holds either form (F64_SLOTS doubles per site of the padded pitch px), so a case can
         // change form in place
         const size_t pop_bytes = elem == 8 ? sizeof(double) * F64_SLOTS * (size_t)k.cc.px * (size_t)d.ny
-                                           : (size_t)elem * 2 * k.nslot * (size_t)k.n;
+                                           : (size_t)elem * 2 * k.nslot * (size_t)k.cc.px * (size_t)d.ny;
         for (int b = 0; b < 2; ++b) {
             if (dev_alloc(c, (void**)&k.pop[b], pop_bytes) || hipMemsetAsync(k.pop[b], 0, pop_bytes, c->stream) != hipSuccess) {
                 lbm_destroy(c);
@@ -715,7 +717,7 @@ int lbm_init_drop(lbm_ctx* c, int id, double R, double width, int x0, int y0, do
         // reduced-precision forms: computed on the device in the stored form (no host arrays, so
         // boxes of 10^8 sites initialise in milliseconds); FP64 keeps the host path below, whose
         // libm exp gives the reference's bits
-        HIPCHK(c, launch_lp_init_drop(k.elem, k.sym, k.pop[k.cur], nx, ny, R, k_perfil, x0, y0, v0, k.cc.ref_mid,
+        HIPCHK(c, launch_lp_init_drop(k.elem, k.sym, k.pop[k.cur], nx, k.cc.px, ny, R, k_perfil, x0, y0, v0, k.cc.ref_mid,
                                       k.cc.ref_half, c->stream));
         k.aux_valid = false;
         return LBM_OK;
@@ -1739,7 +1741,7 @@ int lbm_set_reference(lbm_ctx* c, int id, double rho_red, double rho_blue)
     CaseState& k = c->cases[id];
     if (k.elem == 8) return LBM_OK;   // FP64 stores the populations themselves
     const double mid = 0.5 * (rho_red + rho_blue) - 1.0, half = 0.5 * (rho_red - rho_blue);
-    HIPCHK(c, launch_lp_recentre(k.elem, k.sym, k.pop[k.cur], (long long)k.nslot * k.n, k.d.nx, k.cc.ref_mid, k.cc.ref_half, mid, half,
+    HIPCHK(c, launch_lp_recentre(k.elem, k.sym, k.pop[k.cur], (long long)k.nslot * k.cc.px * k.d.ny, k.cc.px, k.cc.ref_mid, k.cc.ref_half, mid, half,
                                  c->stream));
     k.cc.ref_mid = mid;
     k.cc.ref_half = half;

@@ -112,18 +112,18 @@ hipError_t launch_f64_rows_to_soa(const double* pop, bool compact, double beta, 
                                   long long S, double* soa, hipStream_t stream);
 
 // FP32-compute forms; storage FP32 (elem = 4) or FP16 (elem = 2): per population the pair
-// (s_i / w_i, e_i), s_i = fR_i + fB_i - w_i, e_i = (fR_i - fB_i) / (fR_i + fB_i), at ((y * NS + slot) nx + x)
+// (s_i / w_i, e_i), s_i = fR_i + fB_i - w_i, e_i = (fR_i - fB_i) / (fR_i + fB_i), at ((y * NS + slot) px + x)
 // with NS = 19 slots, or NS = 14 for the z-symmetric storage (sym; slot_of in lattice.h)
 hipError_t launch_step_lp(int elem, bool sym, int kind, const StepLaunch& a, int ntiles, hipStream_t stream);
 
 // low-precision state -> FP64 colour populations fR, fB [19][S]
-hipError_t launch_lp_to_f64(int elem, bool sym, const void* pop, int nx, long long S, long long n, double rmid,
+hipError_t launch_lp_to_f64(int elem, bool sym, const void* pop, int nx, int px, long long S, long long n, double rmid,
                             double rhalf, double* fR, double* fB, hipStream_t stream);
 // drop initial condition written on the device in a reduced-precision form
-hipError_t launch_lp_init_drop(int elem, bool sym, void* pop, int nx, int ny, double R, double kp, int x0, int y0,
+hipError_t launch_lp_init_drop(int elem, bool sym, void* pop, int nx, int px, int ny, double R, double kp, int x0, int y0,
                                double v0, double rmid, double rhalf, hipStream_t stream);
 // in-place change of the reference density deviation r(e) = mid + half e of a reduced-precision state
-hipError_t launch_lp_recentre(int elem, bool sym, void* pop, long long npairs, int nx, double mid0, double half0,
+hipError_t launch_lp_recentre(int elem, bool sym, void* pop, long long npairs, int px, double mid0, double half0,
                               double mid1, double half1, hipStream_t stream);
 
 }  // namespace itolbm

@@ -2,7 +2,8 @@
 // kernels (pull-stream, colour field, mediator gradient, tension operator, BGK + Guo, recolouring),
 // reformulated in deviation variables so that single and half precision keep their digits.
 //
-// Storage: one pair (h_i, e_i) per site and velocity i, in rows: pair(i, x, y) at ((y * 19 + i) nx + x).
+// Storage: one pair (h_i, e_i) per site and velocity i, in rows: pair(i, x, y) at ((y * 19 + i) px + x), px
+// >= nx the slot pitch (CaseConst::px).
 // A row of one velocity is contiguous (coalesced pulls and stores) and the 19 velocities of a row sit
 // together (one buffer resource per row block, few pages per wavefront):
 //   s_i = fR_i + fB_i - w_i                         deviation of the total population from w_i (rho0 = 1)
@@ -106,11 +107,11 @@ struct Row {
 // so the compiler's waits for the current row (vector memory counts retire in issue order) do not
 // also wait for the rows ahead.
 template <typename ST, bool SYM>
-__device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restrict__ in, int nx, int ny, int y,
+__device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restrict__ in, int px, int ny, int y,
                                          const Lane& L, typename PairOf<ST>::raw r[Q], bool live = true)
 {
     using R = typename PairOf<ST>::raw;
-    const long long pitch = (long long)n_slots(SYM) * nx;
+    const long long pitch = (long long)n_slots(SYM) * px;   // pairs per row block (px: the slot pitch)
     const unsigned bytes = live ? (unsigned)(pitch * (long long)sizeof(R)) : 0u;
     const rsrc_t rowm = make_rsrc(in + wrapi(y - 1, ny) * pitch, bytes);
     const rsrc_t row0 = make_rsrc(in + wrapi(y, ny) * pitch, bytes);
@@ -123,7 +124,7 @@ __device__ __forceinline__ void load_raw(gcst<typename PairOf<ST>::raw>* __restr
         if (!is_stored(SYM, i)) continue;
         const unsigned xs = CX[i] > 0 ? om : (CX[i] < 0 ? op : o0);
         r[i] = pair_load<R>(CY[i] > 0 ? rowm : (CY[i] < 0 ? rowp : row0), xs,
-                            (unsigned)(slot_of(SYM, i) * nx * (int)sizeof(R)));
+                            (unsigned)(slot_of(SYM, i) * px * (int)sizeof(R)));
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i)
@@ -551,19 +552,19 @@ struct Halo2LP {
     using R = typename PairOf<ST>::raw;
     unsigned long long off[2];   // element offset of (velocity i, column x) within a row block
     int dy[2];
-    __device__ __forceinline__ void init(int nx, int strip)
+    __device__ __forceinline__ void init(int nx, int px, int strip)
     {
         const int c0 = strip * 64;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const HaloLaneLP h = s ? halo_lane_lp(nx, c0 + 64, c0 + 65) : halo_lane_lp(nx, c0 - 2, c0 - 1);
-            off[s] = (unsigned long long)slot_of(SYM, h.i) * (unsigned)nx + h.x;
+            off[s] = (unsigned long long)slot_of(SYM, h.i) * (unsigned)px + h.x;
             dy[s] = h.dy;
         }
     }
-    __device__ __forceinline__ void load(gcst<R>* in, int nx, int ny, int y, R& a, R& b) const
+    __device__ __forceinline__ void load(gcst<R>* in, int px, int ny, int y, R& a, R& b) const
     {
-        const unsigned long long pitch = (unsigned long long)n_slots(SYM) * (unsigned)nx;
+        const unsigned long long pitch = (unsigned long long)n_slots(SYM) * (unsigned)px;
         a = in[(unsigned long long)wrap_row3(y, dy[0], ny) * pitch + off[0]];
         b = in[(unsigned long long)wrap_row3(y, dy[1], ny) * pitch + off[1]];
     }
@@ -588,17 +589,17 @@ __device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& 
 {
     using R = typename PairOf<ST>::raw;
     if (!L.interior) return;
-    const int nx = t.cc.nx;
+    const int nx = t.cc.nx, px = t.cc.px;
     Row row;
     decode<ST>(cur, row, rf);
     float ds, rho;
     row_density(row, ds, rho);
     constexpr int NS = n_slots(SYM);
-    const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)NS * nx), (unsigned)(NS * nx * (int)sizeof(R)));
+    const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)NS * px), (unsigned)(NS * px * (int)sizeof(R)));
     const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
     auto st = [&](int i, float s, float e) {
         if (!is_stored(SYM, i)) return;   // equal to its z mirror's
-        pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(slot_of(SYM, i) * nx * (int)sizeof(R)));
+        pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(slot_of(SYM, i) * px * (int)sizeof(R)));
     };
     if (rho > 1.0e-30f) {
         collide_lp<FORCED, SYM>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
@@ -628,14 +629,14 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     constexpr int NA = DEPTH - 1;         // rows loaded ahead of the iteration that uses them
     constexpr int NU = NA > 0 ? NA : 1;   // unroll of the row loop
     using R = typename PairOf<ST>::raw;
-    const int nx = t.cc.nx, ny = t.cc.ny;
+    const int nx = t.cc.nx, ny = t.cc.ny, px = t.cc.px;
     const ConstF k = to_float(t.cc);
     const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
     Halo2LP<ST, SYM> h;
-    h.init(nx, t.strip);
+    h.init(nx, px, t.strip);
     const int y0 = t.y0, y1 = t.y1, n = y1 - y0;
     auto row = [&](int kk) { return REV ? y1 - 1 - kk : y0 + kk; };
     auto G = [](int r) { return REV ? 2 - r : r; };
@@ -646,9 +647,9 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     float P[5], HW[5][4];   // rows row(k - 2) .. row(k + 2)
     double pd[3];   // FP64 colour field of rows row(k), row(k + 1), row(k + 2) (for their collision)
     auto phi_row = [&](int y, float (&hw)[4]) {
-        load_raw<ST, SYM>(in, nx, ny, y, L, cur);
+        load_raw<ST, SYM>(in, px, ny, y, L, cur);
         R va, vb;
-        h.load(in, nx, ny, y, va, vb);
+        h.load(in, px, ny, y, va, vb);
         halo2_phase<ST>(va, vb, rf, hw);
         return raw_phase<ST>(cur, rf);
     };
@@ -687,8 +688,8 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     hl[1] = aux_halo(1, 0); hr[1] = aux_halo(1, 1);
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
-        load_raw<ST, SYM>(in, nx, ny, row(2 + r), L, ahead[r]);
-        h.load(in, nx, ny, row(2 + r), ha[r], hb[r]);
+        load_raw<ST, SYM>(in, px, ny, row(2 + r), L, ahead[r]);
+        h.load(in, px, ny, row(2 + r), ha[r], hb[r]);
     }
 
     for (int k0 = 0; k0 < n; k0 += NU) {
@@ -700,19 +701,19 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
             pd[0] = pd[1];
             pd[1] = pd[2];
             if constexpr (NA > 0) {
-                load_raw<ST, SYM>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
+                load_raw<ST, SYM>(in, px, ny, yy, L, cur);   // row yy again (from L2), for its collision
                 __builtin_amdgcn_sched_barrier(0);      // issued before the wait on row(kk + 2)
                 pd[2] = raw_phase<ST>(ahead[j], rf);    // row(kk + 2) (waits for its loads)
                 halo2_phase<ST>(ha[j], hb[j], rf, HW[4]);
                 // row(kk + 2 + NA), used NA iterations on; past the segment's last row the loads are
                 // issued empty, so every path leaves the same loads outstanding
                 const bool live = kk + NA < n;
-                load_raw<ST, SYM>(in, nx, ny, row(kk + 2 + NA), L, ahead[j], live);
-                h.load(in, nx, ny, live ? row(kk + 2 + NA) : yy, ha[j], hb[j]);
+                load_raw<ST, SYM>(in, px, ny, row(kk + 2 + NA), L, ahead[j], live);
+                h.load(in, px, ny, live ? row(kk + 2 + NA) : yy, ha[j], hb[j]);
                 __builtin_amdgcn_sched_barrier(0);
             } else {
                 pd[2] = phi_row(row(kk + 2), HW[4]);
-                load_raw<ST, SYM>(in, nx, ny, yy, L, cur);   // row yy again (from L2), for its collision
+                load_raw<ST, SYM>(in, px, ny, yy, L, cur);   // row yy again (from L2), for its collision
             }
             P[4] = (float)pd[2];
             const AuxF ap = aux_lanes(2);
@@ -763,14 +764,14 @@ template <typename ST, int KIND, bool SYM, bool REV>
 __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
 {
     using R = typename PairOf<ST>::raw;
-    const int nx = t.cc.nx, ny = t.cc.ny;
+    const int nx = t.cc.nx, ny = t.cc.ny, px = t.cc.px;
     const ConstF k = to_float(t.cc);
     const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
     Halo2LP<ST, SYM> h;
-    h.init(nx, t.strip);
+    h.init(nx, px, t.strip);
     const int y0 = t.y0, y1 = t.y1, n = y1 - y0;
     auto row = [&](int kk) { return REV ? y1 - 1 - kk : y0 + kk; };
     auto G = [](int r) { return REV ? 2 - r : r; };
@@ -779,9 +780,9 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
     float P[5], HW[5][4];   // rows row(k - 2) .. row(k + 2)
     double pd[3];           // FP64 colour field of rows row(k), row(k + 1), row(k + 2) (for their collision)
     auto phi_row = [&](int y, R (&dst)[Q], float (&hw)[4]) {
-        load_raw<ST, SYM>(in, nx, ny, y, L, dst);
+        load_raw<ST, SYM>(in, px, ny, y, L, dst);
         R va, vb;
-        h.load(in, nx, ny, y, va, vb);
+        h.load(in, px, ny, y, va, vb);
         halo2_phase<ST>(va, vb, rf, hw);
         return raw_phase<ST>(dst, rf);
     };
@@ -899,18 +900,18 @@ struct HaloLP {
     unsigned long long off;   // element offset of (velocity i, column x) within a row block
     int dy;                   // -c_iy
     bool on;
-    __device__ __forceinline__ void init(int nx, int strip)
+    __device__ __forceinline__ void init(int nx, int px, int strip)
     {
         const HaloLaneLP h = halo_lane_lp(nx, strip * 64 - 1, strip * 64 + 64);
-        off = (unsigned long long)slot_of(SYM, h.i) * (unsigned)nx + h.x;
+        off = (unsigned long long)slot_of(SYM, h.i) * (unsigned)px + h.x;
         dy = h.dy;
         on = h.on;
     }
-    __device__ __forceinline__ R load(gcst<R>* in, int nx, int ny, int y) const
+    __device__ __forceinline__ R load(gcst<R>* in, int px, int ny, int y) const
     {
         const int yy = wrap_row3(y, dy, ny);
         // every lane loads (the unused lanes re-read an in-range value): no branch
-        return in[(unsigned long long)yy * ((unsigned long long)n_slots(SYM) * (unsigned)nx) + off];
+        return in[(unsigned long long)yy * ((unsigned long long)n_slots(SYM) * (unsigned)px) + off];
     }
 };
 
@@ -957,14 +958,14 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     using R = typename PairOf<ST>::raw;
     TileCtxLP t;
     if (!tile_ctx(launch, t)) return;
-    const int nx = t.cc.nx, ny = t.cc.ny;
+    const int nx = t.cc.nx, ny = t.cc.ny, px = t.cc.px;
     const ConstF k = to_float(t.cc);
     const Ref rf = make_lp_ref(t.cc.ref_mid, t.cc.ref_half);
     const Lane L = make_lane_aligned(nx, t.strip);
     gcst<R>* in = (gcst<R>*)t.in;
     gst<R>* out = (gst<R>*)t.out;
     HaloLP<ST, SYM> h;
-    h.init(nx, t.strip);
+    h.init(nx, px, t.strip);
     const int y0 = t.y0, y1 = t.y1, n = y1 - y0;
     // Segments march in alternating directions: even ones from y0 to y1 - 1, odd ones from y1 - 1 to
     // y0.  Two adjacent segments then reach the rows they share (each reads one row beyond its own)
@@ -979,12 +980,12 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
     R buf[NB][Q];
     R hv[NB];
     float p[3], hl[3], hr[3];   // colour field of rows row(k-1), row(k), row(k+1)
-    load_raw<ST, SYM>(in, nx, ny, row(-1), L, buf[NB - 1]);
-    hv[NB - 1] = h.load(in, nx, ny, row(-1));
+    load_raw<ST, SYM>(in, px, ny, row(-1), L, buf[NB - 1]);
+    hv[NB - 1] = h.load(in, px, ny, row(-1));
 #pragma unroll
     for (int r = 0; r < DEPTH; ++r) {
-        load_raw<ST, SYM>(in, nx, ny, row(r), L, buf[r]);
-        hv[r] = h.load(in, nx, ny, row(r));
+        load_raw<ST, SYM>(in, px, ny, row(r), L, buf[r]);
+        hv[r] = h.load(in, px, ny, row(r));
     }
     p[1] = (float)raw_phase<ST>(buf[NB - 1], rf);
     halo_phase<ST>(hv[NB - 1], rf, hl[1], hr[1]);
@@ -1002,8 +1003,8 @@ __global__ void __launch_bounds__(64) lp_pipe(StepLaunch launch)
                 // rows up to row(n) feed the gradients; past it the loads are issued empty, so every
                 // path leaves the same loads outstanding
                 const bool live = kk + DEPTH <= n;
-                load_raw<ST, SYM>(in, nx, ny, row(kk + DEPTH), L, far, live);
-                hv[(j + DEPTH) % NB] = h.load(in, nx, ny, live ? row(kk + DEPTH) : row(kk));
+                load_raw<ST, SYM>(in, px, ny, row(kk + DEPTH), L, far, live);
+                hv[(j + DEPTH) % NB] = h.load(in, px, ny, live ? row(kk + DEPTH) : row(kk));
                 __builtin_amdgcn_sched_barrier(0);
                 p[0] = p[1]; p[1] = p[2];
                 hl[0] = hl[1]; hl[1] = hl[2];
@@ -1065,14 +1066,15 @@ hipError_t launch_typed(int kind, const StepLaunch& a, int ntiles, hipStream_t s
 // (h, e) pairs in rows -> fR, fB in FP64, [19][S] each (own site, post-collision); s is decoded as
 // the step kernels decode it, f = w + s in FP64 (a z mirror of the symmetric storage from its slot)
 template <typename ST, bool SYM>
-__global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw* __restrict__ pop, int nx, long long S,
-                                                 long long n, Ref rf, double* __restrict__ fR, double* __restrict__ fB)
+__global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw* __restrict__ pop, int nx, int px,
+                                                 long long S, long long n, Ref rf, double* __restrict__ fR,
+                                                 double* __restrict__ fB)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
         const long long y = p / nx, x = p - y * nx;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const auto r = pop[(y * n_slots(SYM) + slot_of(SYM, i)) * nx + x];
+            const auto r = pop[(y * n_slots(SYM) + slot_of(SYM, i)) * px + x];
             const float e = PairOf<ST>::e(r);
             const float s = decode_s(PairOf<ST>::a(r), e, vclass(i), rf);
             const double f = (double)WF[i] + (double)s;
@@ -1083,14 +1085,15 @@ __global__ void __launch_bounds__(256) lp_to_f64(const typename PairOf<ST>::raw*
 }
 
 // Re-express a stored state against a new reference density, in place:
-// h' = h + c_k r_old(e) - c_k r_new(e).  Pair p is slot (p / nx) mod (19 or 14) of its row block.
+// h' = h + c_k r_old(e) - c_k r_new(e).  Pair p is slot (p / px) mod (19 or 14) of its row block (the
+// pitch's padding pairs, never read, are re-expressed too).
 template <typename ST, bool SYM>
 __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __restrict__ pop, long long npairs,
-                                                   int nx, Ref from, Ref to)
+                                                   int px, Ref from, Ref to)
 {
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npairs;
          p += (long long)gridDim.x * blockDim.x) {
-        const int sl = (int)((p / nx) % n_slots(SYM));
+        const int sl = (int)((p / px) % n_slots(SYM));
         const int i = SYM ? VEL_SYM[sl] : sl;
         const int k = vclass(i);
         const auto r = pop[p];
@@ -1105,8 +1108,8 @@ __global__ void __launch_bounds__(256) lp_recentre(typename PairOf<ST>::raw* __r
 // (v0, 0, 0)), f_B = f^eq((1 - phi) rho, (v0, 0, 0)).  The stored value at site x for velocity i is
 // the population that streams into x, i.e. the initial value at x + c_i (the kernel pulls).
 template <typename ST, bool SYM>
-__global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __restrict__ pop, int nx, int ny,
-                                                    double R, double kp, int x0, int y0, double v0, Ref rf)
+__global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __restrict__ pop, int nx, int px,
+                                                    int ny, double R, double kp, int x0, int y0, double v0, Ref rf)
 {
     const long long n = (long long)nx * ny;
     for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
@@ -1131,7 +1134,7 @@ __global__ void __launch_bounds__(256) lp_init_drop(typename PairOf<ST>::raw* __
             const float e = (float)((f != 0.0) ? (fR - fB) / f : 0.0);
             const int k = vclass(i);
             const float h = (float)((f - w) * (double)SCL[k]) - __builtin_fmaf(rf.half(k), e, rf.mid(k));
-            pop[((long long)y * n_slots(SYM) + slot_of(SYM, i)) * nx + x] = PairOf<ST>::make(h, e);
+            pop[((long long)y * n_slots(SYM) + slot_of(SYM, i)) * px + x] = PairOf<ST>::make(h, e);
         }
     }
 }
@@ -1147,39 +1150,39 @@ hipError_t launch_step_lp(int elem, bool sym, int kind, const StepLaunch& a, int
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_lp_to_f64(int elem, bool sym, const void* pop, int nx, long long S, long long n, double rmid,
+hipError_t launch_lp_to_f64(int elem, bool sym, const void* pop, int nx, int px, long long S, long long n, double rmid,
                             double rhalf, double* fR, double* fB, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((n + 255) / 256, 4096);
     const Ref rf = make_lp_ref(rmid, rhalf);
     const dim3 g(nb), b(256);
     if (elem == 4) {
-        if (sym) hipLaunchKernelGGL((lp_to_f64<float, true>), g, b, 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
-        else hipLaunchKernelGGL((lp_to_f64<float, false>), g, b, 0, stream, (const f32x2*)pop, nx, S, n, rf, fR, fB);
+        if (sym) hipLaunchKernelGGL((lp_to_f64<float, true>), g, b, 0, stream, (const f32x2*)pop, nx, px, S, n, rf, fR, fB);
+        else hipLaunchKernelGGL((lp_to_f64<float, false>), g, b, 0, stream, (const f32x2*)pop, nx, px, S, n, rf, fR, fB);
     } else {
-        if (sym) hipLaunchKernelGGL((lp_to_f64<_Float16, true>), g, b, 0, stream, (const unsigned*)pop, nx, S, n, rf, fR, fB);
-        else hipLaunchKernelGGL((lp_to_f64<_Float16, false>), g, b, 0, stream, (const unsigned*)pop, nx, S, n, rf, fR, fB);
+        if (sym) hipLaunchKernelGGL((lp_to_f64<_Float16, true>), g, b, 0, stream, (const unsigned*)pop, nx, px, S, n, rf, fR, fB);
+        else hipLaunchKernelGGL((lp_to_f64<_Float16, false>), g, b, 0, stream, (const unsigned*)pop, nx, px, S, n, rf, fR, fB);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_lp_recentre(int elem, bool sym, void* pop, long long npairs, int nx, double mid0, double half0,
+hipError_t launch_lp_recentre(int elem, bool sym, void* pop, long long npairs, int px, double mid0, double half0,
                               double mid1, double half1, hipStream_t stream)
 {
     const int nb = (int)std::min<long long>((npairs + 255) / 256, 8192);
     const Ref from = make_lp_ref(mid0, half0), to = make_lp_ref(mid1, half1);
     const dim3 g(nb), b(256);
     if (elem == 4) {
-        if (sym) hipLaunchKernelGGL((lp_recentre<float, true>), g, b, 0, stream, (f32x2*)pop, npairs, nx, from, to);
-        else hipLaunchKernelGGL((lp_recentre<float, false>), g, b, 0, stream, (f32x2*)pop, npairs, nx, from, to);
+        if (sym) hipLaunchKernelGGL((lp_recentre<float, true>), g, b, 0, stream, (f32x2*)pop, npairs, px, from, to);
+        else hipLaunchKernelGGL((lp_recentre<float, false>), g, b, 0, stream, (f32x2*)pop, npairs, px, from, to);
     } else {
-        if (sym) hipLaunchKernelGGL((lp_recentre<_Float16, true>), g, b, 0, stream, (unsigned*)pop, npairs, nx, from, to);
-        else hipLaunchKernelGGL((lp_recentre<_Float16, false>), g, b, 0, stream, (unsigned*)pop, npairs, nx, from, to);
+        if (sym) hipLaunchKernelGGL((lp_recentre<_Float16, true>), g, b, 0, stream, (unsigned*)pop, npairs, px, from, to);
+        else hipLaunchKernelGGL((lp_recentre<_Float16, false>), g, b, 0, stream, (unsigned*)pop, npairs, px, from, to);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_lp_init_drop(int elem, bool sym, void* pop, int nx, int ny, double R, double kp, int x0, int y0,
+hipError_t launch_lp_init_drop(int elem, bool sym, void* pop, int nx, int px, int ny, double R, double kp, int x0, int y0,
                                double v0, double rmid, double rhalf, hipStream_t stream)
 {
     const long long n = (long long)nx * ny;
@@ -1187,13 +1190,13 @@ hipError_t launch_lp_init_drop(int elem, bool sym, void* pop, int nx, int ny, do
     const Ref rf = make_lp_ref(rmid, rhalf);
     const dim3 g(nb), b(256);
     if (elem == 4) {
-        if (sym) hipLaunchKernelGGL((lp_init_drop<float, true>), g, b, 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0, v0, rf);
-        else hipLaunchKernelGGL((lp_init_drop<float, false>), g, b, 0, stream, (f32x2*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+        if (sym) hipLaunchKernelGGL((lp_init_drop<float, true>), g, b, 0, stream, (f32x2*)pop, nx, px, ny, R, kp, x0, y0, v0, rf);
+        else hipLaunchKernelGGL((lp_init_drop<float, false>), g, b, 0, stream, (f32x2*)pop, nx, px, ny, R, kp, x0, y0, v0, rf);
     } else {
         if (sym)
-            hipLaunchKernelGGL((lp_init_drop<_Float16, true>), g, b, 0, stream, (unsigned*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+            hipLaunchKernelGGL((lp_init_drop<_Float16, true>), g, b, 0, stream, (unsigned*)pop, nx, px, ny, R, kp, x0, y0, v0, rf);
         else
-            hipLaunchKernelGGL((lp_init_drop<_Float16, false>), g, b, 0, stream, (unsigned*)pop, nx, ny, R, kp, x0, y0, v0, rf);
+            hipLaunchKernelGGL((lp_init_drop<_Float16, false>), g, b, 0, stream, (unsigned*)pop, nx, px, ny, R, kp, x0, y0, v0, rf);
     }
     return hipGetLastError();
 }

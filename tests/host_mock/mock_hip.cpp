@@ -163,9 +163,9 @@ int step_f64_fused_capacity(bool) { return 2048; }
 hipError_t launch_step_lp(int, bool, int, const StepLaunch&, int, hipStream_t) MOCK_LAUNCH
 hipError_t launch_f64_rows_to_soa(const double*, bool, double, int, int, long long, long long, double*, hipStream_t) MOCK_LAUNCH
 hipError_t launch_f64_convert(const double*, double*, bool, double, int, int, long long, hipStream_t) MOCK_LAUNCH
-hipError_t launch_lp_to_f64(int, bool, const void*, int, long long, long long, double, double, double*, double*,
+hipError_t launch_lp_to_f64(int, bool, const void*, int, int, long long, long long, double, double, double*, double*,
                             hipStream_t) MOCK_LAUNCH
-hipError_t launch_lp_init_drop(int, bool, void*, int, int, double, double, int, int, double, double, double,
+hipError_t launch_lp_init_drop(int, bool, void*, int, int, int, double, double, int, int, double, double, double,
                                hipStream_t) MOCK_LAUNCH
 hipError_t launch_lp_recentre(int, bool, void*, long long, int, double, double, double, double, hipStream_t) MOCK_LAUNCH
 hipError_t launch_drop_pass1(const DropPass1Args&, int, hipStream_t) MOCK_LAUNCH
