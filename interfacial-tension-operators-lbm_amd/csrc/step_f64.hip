@@ -53,10 +53,10 @@ __device__ __forceinline__ unsigned vrow_off(int colour, int i, int nx)
 
 // Pull the 38 populations of site (col, y): f_i(x) = f*_i(x - c_i).
 __device__ __forceinline__ void load_row(gcdouble* __restrict__ in, int nx, int ny, int y, const Lane& L,
-                                         double fR[Q], double fB[Q])
+                                         double fR[Q], double fB[Q], bool live = true)
 {
     const long long rb = block_doubles(nx);
-    const unsigned bytes = block_bytes(nx);
+    const unsigned bytes = live ? block_bytes(nx) : 0u;
     const rsrc_t rm = make_rsrc(in + wrapi(y - 1, ny) * rb, bytes);
     const rsrc_t r0 = make_rsrc(in + wrapi(y, ny) * rb, bytes);
     const rsrc_t rp = make_rsrc(in + wrapi(y + 1, ny) * rb, bytes);
@@ -1096,11 +1096,21 @@ struct Ring2Lds {
     double hbuf[2][4 * Q];   // pulled populations of the four halo sites, by colour
 };
 
-// CACHED: the populations pulled for a row's colour field are kept in registers (a ring of three
-// rows) until the row is collided, two iterations later, instead of being read again; 114 more
-// doubles per lane, so the kernel runs one wavefront per SIMD.
-template <int OP, bool CACHED>
-__device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
+// LDS of the cached form (step_ring2c): Ring2Lds and the totals and colour sums of two rows,
+// [slot][value][lane] (values 0 .. Q - 1 the totals f_i, Q and Q + 1 the colour sums); each lane reads
+// back only what it wrote.
+template <int OP>
+struct Ring2cLds : Ring2Lds<OP> {
+    double kept[2][Q + 2][64];
+};
+
+// CACHED: what the collision of a row needs from its pulled populations (the totals f_i = fR_i + fB_i
+// and the colour sums) is kept from its colour field until its collision, two iterations later,
+// instead of reading the populations again: rows y and y + 1 in LDS (lds.kept), row y + 2 in registers
+// until row y's slot is free.  The loads of row y + 3 are issued before row y is collided, so they are
+// in flight over the collision.  This takes one wavefront per SIMD.
+template <int OP, bool CACHED, class Lds>
+__device__ __forceinline__ void ring2_tile(const TileCtx& t, Lds& lds)
 {
     constexpr int NA = Ring2Lds<OP>::NA;
     constexpr int NP = 68;
@@ -1120,17 +1130,29 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
     // and this keeps the compiler from moving reads above writes (it waits on LDS only)
     auto lds_order = [] { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); };
 
-    // colour field of row yy into its ring slot: own lanes, and the four halo columns.  The halo sites'
-    // 76 pulled values (two loads per colour, value g = 19 s + i in lane g of load A or g - 64 of load B)
-    // go through LDS to lanes 0 .. 3, each of which sums one site's values in index order, as dens().
-    auto phi_row_into = [&](int yy, double (&fR)[Q], double (&fB)[Q]) {
-        load_row(t.in, nx, ny, yy, L, fR, fB);
+    // a row's pulled populations: the own lanes' 38 and this lane's values of the two halo loads
+    struct Pulled { double fR[Q], fB[Q], ra, ba, rb, bb; };
+    // with live = false the own lanes' loads read nothing (empty resources) and the halo loads read row
+    // yy's halo values (valid addresses)
+    auto issue = [&](int yy, Pulled& p, bool live) {
+        load_row(t.in, nx, ny, yy, L, p.fR, p.fB, live);
         const unsigned long long e0 = elem_index(nx, wrap_row3(yy, h.dy[0], ny), 0, h.i[0], h.x[0]);
         const unsigned long long e1 = elem_index(nx, wrap_row3(yy, h.dy[1], ny), 0, h.i[1], h.x[1]);
         const unsigned cb = (unsigned)(Q * nx);
-        const double ra = t.in[e0], ba = t.in[e0 + cb], rb = t.in[e1], bb = t.in[e1 + cb];
+        p.ra = t.in[e0]; p.ba = t.in[e0 + cb]; p.rb = t.in[e1]; p.bb = t.in[e1 + cb];
+    };
+    // colour field of row yy into its ring slot: own lanes, and the four halo columns.  The halo sites'
+    // 76 pulled values (two loads per colour, value g = 19 s + i in lane g of load A or g - 64 of load B)
+    // go through LDS to lanes 0 .. 3, each of which sums one site's values in index order, as dens().
+    // rR / rB receive the own lanes' colour sums.
+    auto consume = [&](int yy, const Pulled& p, double& rR, double& rB) {
+        const double (&fR)[Q] = p.fR;
+        const double (&fB)[Q] = p.fB;
+        const double ra = p.ra, ba = p.ba, rb = p.rb, bb = p.bb;
         double* row = phir[pslot(yy)];
-        row[lane + 2] = phase(dens(fR), dens(fB));
+        rR = dens(fR);
+        rB = dens(fB);
+        row[lane + 2] = phase(rR, rB);
         hbuf[0][lane] = ra;
         hbuf[1][lane] = ba;
         if (lane < 4 * Q - 64) {
@@ -1152,8 +1174,35 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
         lds_order();   // hbuf is written again by the next row
     };
     auto phi_row = [&](int yy) {
-        double fR[Q], fB[Q];
-        phi_row_into(yy, fR, fB);
+        Pulled p;
+        double rR, rB;
+        issue(yy, p, true);
+        consume(yy, p, rR, rB);
+    };
+    // CACHED: the row's totals f_i = fR_i + fB_i and colour sums, all its collision needs
+    auto keep = [&](int yy, const Pulled& p, double (&f)[Q], double& rR, double& rB) {
+        consume(yy, p, rR, rB);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = p.fR[i] + p.fB[i];
+    };
+    // kept values of row yy in and out of its LDS slot
+    auto put = [&](int yy, const double (&f)[Q], double rR, double rB) {
+        if constexpr (CACHED) {
+            auto& k = lds.kept[(yy - t.y0) & 1];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) k[i][lane] = f[i];
+            k[Q][lane] = rR;
+            k[Q + 1][lane] = rB;
+        }
+    };
+    auto get = [&](int yy, double (&f)[Q], double& rR, double& rB) {
+        if constexpr (CACHED) {
+            const auto& k = lds.kept[(yy - t.y0) & 1];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) f[i] = k[i][lane];
+            rR = k[Q][lane];
+            rB = k[Q + 1][lane];
+        }
     };
     // 3 x 3 colour field around ring column c (colour-field ring coordinates) of row yy
     auto phi9 = [&](int yy, int c) {
@@ -1184,31 +1233,40 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
         }
     };
 
-    // raw rows kept for the collision (CACHED): rows y, y + 1, y + 2 in slots (y - y0) mod 3
-    double rawR[CACHED ? 3 : 1][Q], rawB[CACHED ? 3 : 1][Q];
+    Pulled ahead;   // CACHED: row y + 2's pulled populations, loaded during row y - 1's iteration
     // one row of populations in flight at a time (the scheduler would otherwise issue all four rows'
     // loads together and size the kernel's registers for them)
     for (int k = -2; k <= 1; ++k) {
-        if constexpr (CACHED) {
-            if (k == 0) phi_row_into(t.y0, rawR[0], rawB[0]);
-            else if (k == 1) phi_row_into(t.y0 + 1, rawR[1], rawB[1]);
-            else phi_row(t.y0 + k);
+        if (CACHED && k >= 0) {
+            Pulled p;
+            double f[Q], rR, rB;
+            issue(t.y0 + k, p, true);
+            keep(t.y0 + k, p, f, rR, rB);
+            put(t.y0 + k, f, rR, rB);
         } else {
             phi_row(t.y0 + k);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
+    if constexpr (CACHED) issue(t.y0 + 2, ahead, true);
     lds_order();
     double g00, g01, g02;   // colour-field gradient at row y (F1 scales it by the curvature)
     nt_row(t.y0 - 1, g00, g01, g02);
     nt_row(t.y0, g00, g01, g02);
 
-    // one row: the colour field of row y + 2 (its populations into nR / nB when CACHED), n / T of row
-    // y + 1, then the force, mediator gradient and collision of row y (its populations from cR / cB
-    // when CACHED, read again otherwise)
-    auto row_step = [&](int y, double (&cR)[Q], double (&cB)[Q], double (&nR)[Q], double (&nB)[Q]) {
-        if constexpr (CACHED) phi_row_into(y + 2, nR, nB);
-        else phi_row(y + 2);
+    // one row: the colour field of row y + 2 (its totals and colour sums kept when CACHED), n / T of
+    // row y + 1, then the force, mediator gradient and collision of row y (from its kept values when
+    // CACHED, its populations read again otherwise)
+    auto row_step = [&](int y) {
+        double nf[Q], nr, nb;   // row y + 2's kept values (CACHED)
+        if constexpr (CACHED) {
+            keep(y + 2, ahead, nf, nr, nb);
+            __builtin_amdgcn_sched_barrier(0);
+            issue(y + 3, ahead, y + 1 < t.y1);   // in flight over the rest of this row
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            phi_row(y + 2);
+        }
         lds_order();
         double gp0, gp1, gp2;
         nt_row(y + 1, gp0, gp1, gp2);
@@ -1254,50 +1312,36 @@ __device__ __forceinline__ void ring2_tile(const TileCtx& t, Ring2Lds<OP>& lds)
         // once the force is formed: the scheduler would otherwise hoist them above the force's ring
         // reads and hold both at once
         __builtin_amdgcn_sched_barrier(0);
-        double fR[Q], fB[Q];
+        Site s;
         if constexpr (CACHED) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) { fR[i] = cR[i]; fB[i] = cB[i]; }
+            double cf[Q], cr, cb;
+            get(y, cf, cr, cb);
+            site_sums_f(s, cf, cr, cb);
         } else {
+            double fR[Q], fB[Q];
             load_row(t.in, nx, ny, y, L, fR, fB);
+            site_sums(s, fR, fB, dens(fR), dens(fB));
         }
 
         if (L.interior) {
             const unsigned site = (unsigned)(y * nx + L.col);
             const rsrc_t ro = make_rsrc(t.out + (long long)y * block_doubles(nx), block_bytes(nx));
             const unsigned xoff = (unsigned)L.col * 8u;
-            Site s;
-            site_sums(s, fR, fB, dens(fR), dens(fB));
             if (!s.skip) {
                 bgk<true>(cc, s, Fx, Fy, Fz);
                 recolour(cc, s, gx, gy, gz, mod, [=](int i, double r, double b) { store_pop(ro, nx, xoff, i, r, b); });
             } else {   // left untouched (rare): the raw populations again
                 double uR[Q], uB[Q];
-                if constexpr (CACHED) {
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) { uR[i] = cR[i]; uB[i] = cB[i]; }
-                } else {
-                    load_row(t.in, nx, ny, y, L, uR, uB);
-                }
+                load_row(t.in, nx, ny, y, L, uR, uB);
                 store_row(ro, nx, xoff, uR, uB);
             }
             if (t.aux) store_aux(t.aux, t.S, site, gx, gy, gz, Fx, Fy, Fz);
         }
         g00 = gp0; g01 = gp1; g02 = gp2;
+        put(y + 2, nf, nr, nb);   // into row y's slot
     };
 
-    if constexpr (CACHED) {
-        // unrolled by three so that every raw-row slot is a fixed register range
-        for (int y = t.y0; y < t.y1; y += 3) {
-            row_step(y, rawR[0], rawB[0], rawR[2], rawB[2]);
-            if (y + 1 >= t.y1) break;
-            row_step(y + 1, rawR[1], rawB[1], rawR[0], rawB[0]);
-            if (y + 2 >= t.y1) break;
-            row_step(y + 2, rawR[2], rawB[2], rawR[1], rawB[1]);
-        }
-    } else {
-        for (int y = t.y0; y < t.y1; ++y) row_step(y, rawR[0], rawB[0], rawR[0], rawB[0]);
-    }
+    for (int y = t.y0; y < t.y1; ++y) row_step(y);
 }
 
 template <int OP>
@@ -1309,11 +1353,11 @@ __global__ void __launch_bounds__(64) step_ring2(StepLaunch launch)
     ring2_tile<OP, false>(t, lds);
 }
 
-// the raw rows kept in registers: one wavefront per SIMD (the register file's 512 per lane)
+// the rows' totals and colour sums kept: one wavefront per SIMD (the register file's 512 per lane)
 template <int OP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) step_ring2c(StepLaunch launch)
 {
-    __shared__ Ring2Lds<OP> lds;
+    __shared__ Ring2cLds<OP> lds;
     TileCtx t;
     if (!tile_ctx(launch, t)) return;
     ring2_tile<OP, true>(t, lds);

@@ -103,14 +103,19 @@ def test_full_form_bitwise(op, A, depth, monkeypatch):
     ctx.close()
 
 
-@pytest.mark.parametrize("op,A", [OPS7[0], OPS7[2], OPS7[6]])
-def test_compact_raw_sites_at_strip_edges(op, A, monkeypatch):
+@pytest.mark.parametrize("op,A,depth", [OPS7[0] + (None,), OPS7[2] + (None,), OPS7[6] + (None,),
+                                        OPS7[4] + ("2",), OPS7[5] + ("2",), OPS7[5] + ("1",)])
+def test_compact_raw_sites_at_strip_edges(op, A, depth, monkeypatch):
     """3 x 3 blocks of sites scaled to a density of ~1e-33 (left uncollided, so stored raw) on the
     edge between two 64-column strips, on the periodic seam x = 0 / nx - 1, on a partial last strip
-    and across 4-row segment boundaries; rows pulling from them take the per-population path."""
-    say(f"[gpu] compact raw sites 130x40 {op}")
+    and across 4-row segment boundaries; rows pulling from them take the per-population path.  F1 / F2
+    (full form) with the kernel that keeps a row's totals for its collision (depth 2), which reads an
+    uncollided site's populations again, and with the one that reads every row again (depth 1)."""
+    say(f"[gpu] compact raw sites 130x40 {op} depth {depth}")
     monkeypatch.setenv("ITOLBM_LY", "4")
     monkeypatch.setenv("ITOLBM_F64_COMPACT", "1")
+    if depth:
+        monkeypatch.setenv("ITOLBM_DEPTH", depth)
     m = _pkg.load()
     nx, ny = 130, 40
     sim = O.Sim(O.make_case(nx, ny, op, 1.0, A, 0.8))
