@@ -610,6 +610,41 @@ __device__ __forceinline__ void collide_store(const TileCtxLP& t, const ConstF& 
     if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
 }
 
+// collide_store for a row whose deviations s_i were kept from its colour field (the collision uses
+// only them); a site without fluid reads its pairs again to store them unchanged
+template <typename ST, bool SYM>
+__device__ __forceinline__ void collide_store_kept(const TileCtxLP& t, const ConstF& k, const Ref& rf, const Lane& L,
+                                                   gcst<typename PairOf<ST>::raw>* in, gst<typename PairOf<ST>::raw>* out,
+                                                   int y, const float (&sv)[Q], double phi, float gx, float gy,
+                                                   float Fx, float Fy)
+{
+    using R = typename PairOf<ST>::raw;
+    if (!L.interior) return;
+    const int nx = t.cc.nx, ny = t.cc.ny, px = t.cc.px;
+    Row row;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) row.s[i] = sv[i];
+    float ds, rho;
+    row_density(row, ds, rho);
+    constexpr int NS = n_slots(SYM);
+    const rsrc_t orow = make_rsrc(out + (long long)y * ((long long)NS * px), (unsigned)(NS * px * (int)sizeof(R)));
+    const unsigned ocol = (unsigned)L.col * (unsigned)sizeof(R);
+    auto st = [&](int i, float s, float e) {
+        if (!is_stored(SYM, i)) return;   // equal to its z mirror's
+        pair_store<R>(encode<ST>(i, s, e, rf), orow, ocol, (unsigned)(slot_of(SYM, i) * px * (int)sizeof(R)));
+    };
+    if (rho > 1.0e-30f) {
+        collide_lp<true, SYM>(k, row, ds, rho, phi, gx, gy, Fx, Fy, st);
+    } else {
+        R cur[Q];
+        load_raw<ST, SYM>(in, px, ny, y, L, cur);
+        decode<ST>(cur, row, rf);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st(i, row.s[i], row.e[i]);
+    }
+    if (t.aux) store_aux(t.aux, t.S, (unsigned)(y * nx + L.col), gx, gy, Fx, Fy);
+}
+
 // Aligned 64-column strip kernel for the two-row stencils (F1, F2): rows y-2 .. y+2 of the colour
 // field are kept for the own lanes (P) and the four halo columns (HW); the gradient and the n / T
 // field at the halo columns c0-1 and c0+64 are evaluated on the wavefront's uniform values with the
@@ -754,12 +789,13 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     }
 }
 
-// Depth-1 form of f12_rows without the re-read: the raw row loaded for a row's colour field is kept in
-// registers until that row is collided two iterations later, in three rotating buffers (the row loop
-// unrolled three times, so every buffer role is a compile-time constant).  The re-read of f12_rows
-// mostly missed L2 (fetch 1.8-2.8x the algorithmic reads over eight 512^2 cases): the rows between a
-// strip's two reads of a row exceed an XCD's 4 MB L2 at the wavefronts in flight.  Two more raw rows
-// cost 28 (FP32, z-symmetric) to 76 (FP32) VGPRs.  Same arithmetic per site as f12_rows.
+// Depth-1 form of f12_rows without the re-read: the deviations s_i decoded for a row's colour field
+// (all its collision uses) are kept in registers until that row is collided two iterations later, in
+// three rotating buffers (the row loop unrolled three times, so every buffer role is a compile-time
+// constant), and the next row's pairs are loaded one iteration ahead, in flight over the collision.
+// The re-read of f12_rows mostly missed L2 (fetch 1.8-2.8x the algorithmic reads over eight 512^2
+// cases): the rows between a strip's two reads of a row exceed an XCD's 4 MB L2 at the wavefronts in
+// flight.  Same arithmetic per site as f12_rows.
 template <typename ST, int KIND, bool SYM, bool REV>
 __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
 {
@@ -776,15 +812,25 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
     auto row = [&](int kk) { return REV ? y1 - 1 - kk : y0 + kk; };
     auto G = [](int r) { return REV ? 2 - r : r; };
 
-    R buf[3][Q];            // raw row row(kk) in buffer kk mod 3, from its colour field to its collision
+    float sk[3][Q];         // deviations of row row(kk) in buffer kk mod 3, from its colour field to its collision
     float P[5], HW[5][4];   // rows row(k - 2) .. row(k + 2)
     double pd[3];           // FP64 colour field of rows row(k), row(k + 1), row(k + 2) (for their collision)
-    auto phi_row = [&](int y, R (&dst)[Q], float (&hw)[4]) {
-        load_raw<ST, SYM>(in, px, ny, y, L, dst);
-        R va, vb;
-        h.load(in, px, ny, y, va, vb);
-        halo2_phase<ST>(va, vb, rf, hw);
-        return raw_phase<ST>(dst, rf);
+    R ahead[Q], aha, ahb;   // pairs of the next row to consume (own lanes, halo loads)
+    // with live = false the own lanes' loads read nothing; the halo loads read valid (wrapped) rows
+    auto issue = [&](int y, bool live) {
+        load_raw<ST, SYM>(in, px, ny, y, L, ahead, live);
+        h.load(in, px, ny, y, aha, ahb);
+    };
+    auto consume = [&](float (&sv)[Q], float (&hw)[4]) {
+        halo2_phase<ST>(aha, ahb, rf, hw);
+        Row row;
+        decode<ST>(ahead, row, rf);
+        float ds, rho;
+        double phi;
+        row_phase(row, ds, rho, phi);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) sv[i] = row.s[i];
+        return phi;
     };
     auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
         const float hl[3] = {HW[j + G(0)][1], HW[j + 1][1], HW[j + G(2)][1]};
@@ -809,16 +855,23 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
         return aux_of<KIND>(k, nn);
     };
     {
-        R tmp[Q];
-        P[0] = (float)phi_row(row(-2), tmp, HW[0]);
+        float tmp[Q];
+        issue(row(-2), true);
+        P[0] = (float)consume(tmp, HW[0]);
         __builtin_amdgcn_sched_barrier(0);
-        P[1] = (float)phi_row(row(-1), tmp, HW[1]);
+        issue(row(-1), true);
+        P[1] = (float)consume(tmp, HW[1]);
         __builtin_amdgcn_sched_barrier(0);
     }
-    pd[1] = phi_row(row(0), buf[0], HW[2]);
+    issue(row(0), true);
+    pd[1] = consume(sk[0], HW[2]);
     P[2] = (float)pd[1];
-    pd[2] = phi_row(row(1), buf[1], HW[3]);
+    __builtin_amdgcn_sched_barrier(0);
+    issue(row(1), true);
+    pd[2] = consume(sk[1], HW[3]);
     P[3] = (float)pd[2];
+    __builtin_amdgcn_sched_barrier(0);
+    issue(row(2), true);
     AuxF am = aux_lanes(0), a0 = aux_lanes(1);
     // n / T at the halo columns of rows row(k-1) and row(k) (carried along; row(k+1)'s is new each row)
     AuxF hl[3], hr[3];
@@ -833,7 +886,10 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
             const int yy = row(kk);
             pd[0] = pd[1];
             pd[1] = pd[2];
-            pd[2] = phi_row(row(kk + 2), buf[(j + 2) % 3], HW[4]);
+            pd[2] = consume(sk[(j + 2) % 3], HW[4]);
+            __builtin_amdgcn_sched_barrier(0);
+            issue(row(kk + 3), kk + 1 < n);   // in flight over the rest of this row
+            __builtin_amdgcn_sched_barrier(0);
             P[4] = (float)pd[2];
             const AuxF ap = aux_lanes(2);
             hl[2] = aux_halo(2, 0);
@@ -858,7 +914,7 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
             const float hr0[3] = {HW[1 + G(0)][2], HW[2][2], HW[1 + G(2)][2]};
             const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0, L.lane);
             const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
-            collide_store<ST, true, SYM>(t, k, rf, L, out, yy, buf[j], pd[0], gx, gy, Fx, Fy);
+            collide_store_kept<ST, SYM>(t, k, rf, L, in, out, yy, sk[j], pd[0], gx, gy, Fx, Fy);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 P[r] = P[r + 1];
