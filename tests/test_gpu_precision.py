@@ -117,6 +117,29 @@ def test_lp_step_close_to_oracle(prec, op, A):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", [1, 2], ids=["f32", "f16"])
+@pytest.mark.parametrize("op,A", [OPS7[2], OPS7[4]])
+def test_lp_wide_case_close_to_oracle(prec, op, A, monkeypatch):
+    """A case 4096 wide (64 strips, a multiple of 32: its tiles are dealt in bands of 32 strips) and
+    24 rows in 8-row segments, a drop on the band edge x = 2048: 5 steps against the oracle."""
+    say(f"[gpu] precision {prec} step 4096x24 {op}")
+    monkeypatch.setenv("ITOLBM_LY", "8")
+    m = _pkg.load()
+    nx, ny = 4096, 24
+    sim = O.Sim(O.make_case(nx, ny, op, 0.8, A, 0.8))
+    sim.init_drop(9.0, width=3.45, x0=2048, y0=12, v0=0.03)
+    ctx = m.Context(m.case_desc(nx, ny, op, 0.8, A, 0.8), precision=prec)
+    ctx.upload(sim.fR, sim.fB)
+    for _ in range(5):
+        sim.step()
+    ctx.step(5)
+    gR, gB = ctx.download(m.STATE_POST_COLLISION)
+    err = max(np.abs(gR - sim.post_R).max(), np.abs(gB - sim.post_B).max())
+    say(f"    max |df| = {err:.3e}")
+    assert err <= POP_TOL[prec]
+    ctx.close()
+
+
 RANDOM_BOXES = [(5, 3, "F2", 0.1), (7, 1, "F1", 0.1), (6, 2, "F2", 0.1), (9, 2, "P3 0.5", 0.225),
                 (4, 1, "F3 1.569988", 0.1), (3, 4, "F1", 0.1), (64, 40, "F2", 0.1), (70, 45, "P3 0.5", 0.225)]
 RANDOM_TOL = {1: 5e-7, 2: 2e-4}
