@@ -37,6 +37,9 @@ Also reported:
                  kernel reaches on this GPU
   large          the 8192 x 8192 translating drop (R = 1000, v0 = 0.1, P3 chi = 0.5) in FP32 (all 19
                  populations, 304 B per update), FP32 z-symmetric and FP64 storage
+  shard_8gpu     the same forms on an 8192 x 1024 box (R = 300): one GPU's y-slab when the 8192^2 box
+                 is split over 8 GPUs, the kernel at that shape (the slabs' halo-row exchange is not
+                 part of it: this build runs on one GPU)
   sweep          the 168-case static-drop sweep (tau x sigma x 7 operators, R ~ U{10..50} from
                  numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
                  batched reduced-precision form vs the plain FP64 one-launch-per-case path over
@@ -252,11 +255,13 @@ def dp_agreement(dp, ref):
             "within_1e-4": sum(1 for e in err if e <= 1e-4)}
 
 
-def bench_large(m, timer, stream, form, steps):
-    n = 8192
-    ctx = m.Context(m.case_desc(n, n, "P3 0.5", 1.0, amplitude("P", 1.0), 0.8), precision=PREC_ID[form])
+def bench_large(m, timer, stream, form, steps, nx=8192, ny=8192, R=1000.0):
+    """One large case: the 8192^2 translating drop, or (nx = 8192, ny = 1024) one GPU's y-slab of it
+    when the box is split over 8 GPUs (a periodic 8192 x 1024 box, R = 300: the kernel at the slab's
+    shape; the slab's two halo rows and their exchange are not part of it)."""
+    ctx = m.Context(m.case_desc(nx, ny, "P3 0.5", 1.0, amplitude("P", 1.0), 0.8), precision=PREC_ID[form])
     ctx.set_stream(stream.cuda_stream)
-    ctx.init_drop(1000.0, width=-1.0, v0=0.1)
+    ctx.init_drop(R, width=-1.0, v0=0.1)
     ctx.step(LARGE_TUNE)   # untimed: the library tunes the tile shape on these steps
     ctx.sync()
     rows, depth, tuned = ctx.tile_shape()
@@ -266,12 +271,13 @@ def bench_large(m, timer, stream, form, steps):
     blu = ctx.stored_bytes()
     ctx.close()
     avg = gpu_s / steps
-    return {"workload": f"{n}x{n}x1 translating sharp drop R=1000 v0=0.1, P3 chi=0.5, tau=1, {form} storage",
+    n = nx * ny
+    return {"workload": f"{nx}x{ny}x1 translating sharp drop R={R:g} v0=0.1, P3 chi=0.5, tau=1, {form} storage",
             "steps": steps, "tile_shape": {"rows_per_segment": rows, "depth": depth, "tuned": tuned},
             "launch_form": mode,
-            "MLUPS": n * n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
+            "MLUPS": n * steps / wall_s / 1e6, "kernel_avg_launch_us": avg * 1e6,
             "bytes_per_lattice_update": blu,
-            "achieved_GBs": blu * n * n / avg / 1e9, "frac": blu * n * n / avg / 1e9 / PEAK_HBM_GBS}
+            "achieved_GBs": blu * n / avg / 1e9, "frac": blu * n / avg / 1e9 / PEAK_HBM_GBS}
 
 
 def stream_ceiling(m, timer, stream, reps=10):
@@ -465,6 +471,8 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=20000, help="0 skips the 168-case sweep")
     ap.add_argument("--large-steps", type=int, default=20, help="0 skips the 8192^2 cases")
     ap.add_argument("--large-forms", default="f32z,f32,f64")
+    ap.add_argument("--shard-steps", type=int, default=20,
+                    help="timed steps of the 8192 x 1024 slab in the --large-forms (0 skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--copy-only", action="store_true",
@@ -542,6 +550,12 @@ def main():
         for form in [f for f in a.large_forms.split(",") if f]:
             say(f"bench: 8192^2 {form}, {LARGE_TUNE} tuning + {a.large_steps} timed steps")
             out["large"][form] = bench_large(m, timer, stream, form, a.large_steps)
+    if a.shard_steps > 0:
+        out["shard_8gpu"] = {}
+        for form in [f for f in a.large_forms.split(",") if f]:
+            say(f"bench: 8192 x 1024 (one y-slab of 8192^2 over 8 GPUs) {form}, {LARGE_TUNE} tuning + "
+                f"{a.shard_steps} timed steps")
+            out["shard_8gpu"][form] = bench_large(m, timer, stream, form, a.shard_steps, ny=1024, R=300.0)
     if a.sweep_steps > 0:
         sform = a.prec if a.prec != "f64" else "f32z"
         say(f"bench: 168-case sweep, {a.sweep_steps} steps, f64 per case vs {sform} batched")

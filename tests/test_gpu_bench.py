@@ -1,6 +1,6 @@
 """GPU: bench.py's output contract on a short run: one JSON line with the metric fields, the roofline
 of the dominant kernel, the tuned tile shapes, the other forms' delta_p against FP64 and the CPU
-baseline (sized down: a few timed steps, no sweep, no 8192^2 case)."""
+baseline (sized down: a few timed steps, no sweep, no 8192^2 case or slab)."""
 import json
 import os
 import subprocess
@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_bench_json_line():
     print("[gpu] bench.py --steps 20 --warmup 5 (f64 headline, f32z beside it)", flush=True)
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "20", "--warmup", "5", "--forms", "f32z",
-                        "--sweep-steps", "0", "--large-steps", "0"],
+                        "--sweep-steps", "0", "--large-steps", "0", "--shard-steps", "0"],
                        cwd=ROOT, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [x for x in r.stdout.strip().splitlines() if x.strip()]
