@@ -209,3 +209,54 @@ def test_compact_equals_full_batch(monkeypatch):
         assert_same(out["1"][0][i][0], out["0"][0][i][0], f"case {i} red")
         assert_same(out["1"][0][i][1], out["0"][0][i][1], f"case {i} blue")
     assert out["1"][1] == out["0"][1]
+
+
+def test_compact_graph_replay_and_form_switch(monkeypatch):
+    """Eight 192^2 cases with the library's step graphs (4 steps per graph): runs of lbm_step (the six
+    P1/P3 cases' launch, 221 k sites, in the compact form, the two F3 cases' launch, 74 k sites, in the
+    full form, replayed from graphs), lbm_step_case on one case (the full form: converted on the
+    device, the graph dropped), lbm_step again (converted back, the graph captured again), recording
+    aux on some calls; the same bits as eager steps in the full form after every call."""
+    say("[gpu] compact form under graph replay with form switches, 8 x 192^2")
+    m = _pkg.load()
+    descs = [m.case_desc(192, 192, op, tau, A, 0.8) for (op, A) in (OPS7[0], OPS7[2], OPS7[6], OPS7[3])
+             for tau in (0.6, 1.0)]
+
+    def make(form, graph):
+        if form is None:
+            monkeypatch.delenv("ITOLBM_F64_COMPACT", raising=False)
+        else:
+            monkeypatch.setenv("ITOLBM_F64_COMPACT", form)
+        ctx = m.Context(descs)
+        ctx.set_graph(graph)
+        for i in range(len(descs)):
+            ctx.init_drop(40.0, width=-1.0, v0=0.02 * (i % 3), case=i)
+        return ctx
+
+    ops = [op for (op, A) in (OPS7[0], OPS7[2], OPS7[6], OPS7[3]) for tau in (0.6, 1.0)]
+    family_bytes = [608 if op.startswith("F3") else 384 for op in ops]
+    a = make(None, 4)    # form chosen by launch size, graphs
+    b = make("0", 0)     # full form, eager
+    try:
+        plan = [(9, None, False), (12, None, True), (6, 2, False), (8, None, False), (4, 5, True), (10, None, True)]
+        for call, (n, case, rec) in enumerate(plan):
+            a.step(n, record=rec, case=case)
+            b.step(n, record=rec, case=case)
+            if case is None:
+                assert [a.stored_bytes(case=i) for i in range(len(descs))] == family_bytes, call
+            else:
+                assert a.stored_bytes(case=case) == 608, call
+            for i in range(len(descs)):
+                ra, ba = a.download(m.STATE_POST_COLLISION, case=i)
+                rb, bb = b.download(m.STATE_POST_COLLISION, case=i)
+                assert_same(ra, rb, f"call {call} case {i} red")
+                assert_same(ba, bb, f"call {call} case {i} blue")
+            if rec:
+                k = case if case is not None else 0
+                ga, Fa = a.aux(case=k)
+                gb, Fb = b.aux(case=k)
+                assert_same(ga, gb, f"call {call} mediator momentum")
+                assert_same(Fa, Fb, f"call {call} force")
+    finally:
+        a.close()
+        b.close()
