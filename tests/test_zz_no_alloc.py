@@ -38,11 +38,17 @@ def same_state(a, b, case, what):
     assert np.array_equal(ba, bb), f"{what}: blue differs, max |d| {np.abs(ba - bb).max()}"
 
 
-@pytest.mark.parametrize("prec", [0, 3], ids=["f64", "f32z"])
-def test_noalloc_step_equals_step_case(prec):
+@pytest.mark.parametrize("prec,compact", [(0, None), (0, "1"), (3, None)], ids=["f64", "f64-compact", "f32z"])
+def test_noalloc_step_equals_step_case(prec, compact, monkeypatch):
+    """f64-compact: the FP64 P1/P3 case held in the compact form (ITOLBM_F64_COMPACT=1), so the
+    allocation-free step runs the compact kernel too (the F2 case keeps the full form)."""
+    if compact:
+        monkeypatch.setenv("ITOLBM_F64_COMPACT", compact)
     m = _pkg.load()
-    say(f"[gpu] lbm_step_case_noalloc vs lbm_step_case, precision {prec}: 200x150 F2 then 70x45 P3")
+    say(f"[gpu] lbm_step_case_noalloc vs lbm_step_case, precision {prec}, compact {compact}: 200x150 F2 then 70x45 P3")
     ref, na = make(m, prec), make(m, prec)
+    if compact:
+        assert na.stored_bytes(case=1) == 384 and na.stored_bytes(case=0) == 608
     def noalloc(n, case, record=False):
         before = na.device_bytes
         na.step_noalloc(n, record=record, case=case)
