@@ -44,6 +44,7 @@ Also reported:
                  numpy default_rng(2026), box 4R, initial width 3.45 as the reference programs'):
                  batched reduced-precision form vs the plain FP64 one-launch-per-case path over
                  --sweep-steps steps in all (the first 500 untimed), delta_p agreement at the end;
+                 FP64 batched the same way (the same bits as one launch per case: delta_p equal);
                  FP16 z-symmetric storage batched the same way, for its delta_p agreement
   cpu_baseline   the reference program Imm_OPS (built from the reference's sources by oracle/Makefile)
                  on this host's cores, over a bounded sample of the same workload
@@ -341,7 +342,7 @@ def bench_sweep(m, timer, stream, form, steps):
     sites = sum(d.nx * d.ny for d in descs)
     timed = steps - SWEEP_WARMUP
     res = {}
-    runs = [("f64_per_case", 0), (f"{form}_batched", PREC_ID[form])]
+    runs = [("f64_per_case", 0), ("f64_batched", 0), (f"{form}_batched", PREC_ID[form])]
     if form != "f16z":
         runs.append(("f16z_batched", PREC_ID["f16z"]))   # FP16 storage, for its delta_p agreement
     for label, prec in runs:
@@ -350,13 +351,14 @@ def bench_sweep(m, timer, stream, form, steps):
         ctx.set_stream(stream.cuda_stream)
         for i, (d, r, _) in enumerate(cases):
             ctx.init_drop(r, width=3.45, case=i)
-        if prec == 0:
+        per_case = label == "f64_per_case"
+        if per_case:
             for i in range(len(descs)):
                 ctx.step(SWEEP_WARMUP, case=i)
         else:
             ctx.step(SWEEP_WARMUP)
         ctx.sync()
-        if prec == 0:
+        if per_case:
             # one launch per case and step (cases are independent, so case by case is the same work)
             def run():
                 for i in range(len(descs)):
@@ -382,6 +384,9 @@ def bench_sweep(m, timer, stream, form, steps):
     labels = [lab for (_, _, lab), g in zip(cases, good) if g]
     per_hour = lambda ms: 168 * 3600.0 / (ms * 1e-3 * 20000)
     over = [[labels[j], float(err[j])] for j in np.argsort(err)[::-1] if err[j] > 1e-4]
+    fb = res["f64_batched"]
+    f64_batched = {"ms_per_step": fb["ms_per_step"], "speedup_vs_per_case": a["ms_per_step"] / fb["ms_per_step"],
+                   "dp_equal_to_per_case": bool(np.array_equal(fb["dp"], a["dp"], equal_nan=True))}
     fp16 = None
     if "f16z_batched" in res:
         g16, e16 = agreement(res["f16z_batched"])
@@ -389,7 +394,7 @@ def bench_sweep(m, timer, stream, form, steps):
                 "dp_rel_err_median": float(np.median(e16)) if e16.size else None,
                 "dp_within_1e-4": int((e16 <= 1e-4).sum())}
     return {"cases": len(descs), "sites": sites, "steps": steps, "untimed_steps": SWEEP_WARMUP, "form": form,
-            "f16z": fp16,
+            "f16z": fp16, "f64_batched": f64_batched,
             "f64_per_case_ms_per_step": a["ms_per_step"], "batched_ms_per_step": b["ms_per_step"],
             "speedup": a["ms_per_step"] / b["ms_per_step"],
             "cases_per_hour_20k_steps": {"f64_per_case": per_hour(a["ms_per_step"]),
