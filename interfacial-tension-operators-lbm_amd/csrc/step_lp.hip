@@ -173,7 +173,8 @@ __device__ __forceinline__ void row_phase(const Row& r, float& ds, float& rho, d
     for (int i = 0; i < Q; ++i) {
         const float f = r.s[i] + WF[i];
         a += r.s[i];
-        D += (double)r.e[i] * (double)f;
+        // the product of two floats is exact in FP64, so the fused form rounds as product-then-sum
+        D = __builtin_fma((double)r.e[i], (double)f, D);
         F += (double)f;
     }
     ds = a;
