@@ -177,7 +177,7 @@ int lbm_set_graph(lbm_ctx* ctx, int steps_per_graph);
 
 /* Tile shape lbm_step uses for the kernel family of case case_id: rows per wavefront segment, the
  * pipeline depth (FP32/FP16: rows in flight ahead, 0 = the kernel's default; FP64 F1 / F2: 2 when a
- * row's populations stay in registers until its collision) and whether the shape was tuned (1) or is
+ * row's totals and colour sums are kept until its collision) and whether the shape was tuned (1) or is
  * the default (0).  No reference counterpart (the reference has no tiles); for reports and benchmarks. */
 int lbm_tile_shape(const lbm_ctx* ctx, int case_id, int* rows, int* depth, int* tuned);
 

@@ -1040,12 +1040,12 @@ int queue_steps(lbm_ctx* c, const std::vector<Group*>& act, int n_steps, int fla
 // repeated on one buffer pair re-read an input that the last-level cache then partly holds, which
 // favoured the wrong shapes), and all active families run together, as they do in a step, so a
 // family's shape is chosen in the mix.  Coordinate search, one family at a time with the others
-// fixed: the pipeline depth (FP32/FP16 kernels; FP64 F1 / F2: rows read again or kept in registers)
-// at the current rows per segment, then the rows per segment (1 .. 64) at that depth, then the other
-// depths at the best rows per segment, each candidate over W untimed + T timed steps; the two fastest
-// are timed again over 2T steps.  Only families without a tuned shape are searched (a family whose
-// members changed).  The steps used are subtracted from `left`; nothing is done unless `left`
-// covers the whole search.
+// fixed: the pipeline depth (FP32/FP16 kernels; FP64 F1 / F2: rows read again, or their totals and
+// colour sums kept) at the current rows per segment, then the rows per segment (1 .. 64) at that depth,
+// then the other depths at the best rows per segment, each candidate over W untimed + T timed steps; the
+// two fastest are timed again over 2T steps.  Only families without a tuned shape are searched (a
+// family whose members changed).  The steps used are subtracted from `left`; nothing is done unless
+// `left` covers the whole search.
 int online_tune(lbm_ctx* c, const std::vector<Group*>& act, int& left)
 {
     constexpr int W = 1, T = 6;
