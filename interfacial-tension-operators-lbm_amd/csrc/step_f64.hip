@@ -85,8 +85,6 @@ __device__ __forceinline__ double dens(const double f[Q])
     return r;
 }
 
-__device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }
-
 // c_i . a with the reference's left-to-right order, zero-coefficient terms left out
 template <int i>
 __device__ __forceinline__ double cdot(double a0, double a1, double a2)
@@ -127,15 +125,14 @@ struct Nine {
 // 3 x 3 neighbourhood of each lane's site from the lanes of rows y-1, y, y+1 (a row per lane), with the
 // columns left of lane 0 and right of lane 63 taken from the strip's halo values (hl, hr per row), for
 // aligned 64-column strips.
-__device__ __forceinline__ Nine gather9h(const double r[3], const double hl[3], const double hr[3], int lane)
+__device__ __forceinline__ Nine gather9h(const double r[3], const double hl[3], const double hr[3])
 {
     Nine n;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double lf = shfl(r[k], lane - 1), rt = shfl(r[k], lane + 1);
-        n.v[k][0] = (lane == 0) ? hl[k] : lf;
+        n.v[k][0] = from_left(r[k], hl[k]);
         n.v[k][1] = r[k];
-        n.v[k][2] = (lane == 63) ? hr[k] : rt;
+        n.v[k][2] = from_right(r[k], hr[k]);
     }
     return n;
 }
@@ -477,7 +474,7 @@ __device__ __forceinline__ void ring1_tile(const TileCtx& t, double (&hbuf)[2][2
         const double rRn = dens(nR), rBn = dens(nB);
         ph[2] = phase(rRn, rBn);
         halo_phase(vr, vb, hl[2], hr[2]);
-        const Nine p9 = gather9h(ph, hl, hr, L.lane);
+        const Nine p9 = gather9h(ph, hl, hr);
         double gx, gy, gz;
         mediator_gradient(p9, gx, gy, gz);
         const double mod2 = gx * gx + gy * gy + gz * gz;
@@ -996,7 +993,7 @@ __device__ __forceinline__ void ring1c_tile(const TileCtx& t, Ring1cLds& lds)
         const double pr[3] = {d > 0 ? ph[0] : ph[2], ph[1], d > 0 ? ph[2] : ph[0]};
         const double lr[3] = {d > 0 ? hl[0] : hl[2], hl[1], d > 0 ? hl[2] : hl[0]};
         const double rr[3] = {d > 0 ? hr[0] : hr[2], hr[1], d > 0 ? hr[2] : hr[0]};
-        const Nine p9 = gather9h(pr, lr, rr, L.lane);
+        const Nine p9 = gather9h(pr, lr, rr);
         double gx, gy, gz;
         mediator_gradient(p9, gx, gy, gz);
         const double mod2 = gx * gx + gy * gy + gz * gz;

@@ -204,8 +204,6 @@ __device__ __forceinline__ void row_density(const Row& r, float& ds, float& rho)
     rho = 1.f + a;
 }
 
-__device__ __forceinline__ float shf(float v, int src) { return __shfl(v, src, 64); }
-
 // 64-bit value of the lane that DPP control CTRL selects (rows outside ROWS give 0)
 template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ double dpp_value(double v)
@@ -288,16 +286,15 @@ struct Nine {
 // 3 x 3 neighbourhood of each lane's site from the lanes of rows y-1, y, y+1 (a row per lane), with the
 // columns left of lane 0 and right of lane 63 taken from the strip's halo values (hl, hr per row), for
 // aligned 64-column strips.
-__device__ __forceinline__ Nine gather9h(float rm, float r0, float rp, const float hl[3], const float hr[3], int lane)
+__device__ __forceinline__ Nine gather9h(float rm, float r0, float rp, const float hl[3], const float hr[3])
 {
     Nine n;
     const float rows[3] = {rm, r0, rp};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float lf = shf(rows[k], lane - 1), rt = shf(rows[k], lane + 1);
-        n.v[k][0] = (lane == 0) ? hl[k] : lf;
+        n.v[k][0] = from_left(rows[k], hl[k]);
         n.v[k][1] = rows[k];
-        n.v[k][2] = (lane == 63) ? hr[k] : rt;
+        n.v[k][2] = from_right(rows[k], hr[k]);
     }
     return n;
 }
@@ -706,7 +703,7 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
     auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
         const float hl[3] = {HW[j + G(0)][1], HW[j + 1][1], HW[j + G(2)][1]};
         const float hr[3] = {HW[j + G(0)][2], HW[j + 1][2], HW[j + G(2)][2]};
-        return aux_of<KIND>(k, gather9h(P[j + G(0)], P[j + 1], P[j + G(2)], hl, hr, L.lane));
+        return aux_of<KIND>(k, gather9h(P[j + G(0)], P[j + 1], P[j + G(2)], hl, hr));
     };
     auto aux_halo = [&](int j, int side) {   // n / T at c0-1 (side 0) or c0+64 (side 1)
         Nine nn;
@@ -772,8 +769,8 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
             auto field = [&](float AuxF::*m) {
                 const float l3[3] = {hl[G(0)].*m, hl[1].*m, hl[G(2)].*m};
                 const float r3[3] = {hr[G(0)].*m, hr[1].*m, hr[G(2)].*m};
-                return REV ? gather9h(ap.*m, a0.*m, am.*m, l3, r3, L.lane)
-                          : gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
+                return REV ? gather9h(ap.*m, a0.*m, am.*m, l3, r3)
+                          : gather9h(am.*m, a0.*m, ap.*m, l3, r3);
             };
             float Fx, Fy;
             if constexpr (KIND == KIND_F1) {
@@ -787,7 +784,7 @@ __device__ __forceinline__ void f12_rows(const TileCtxLP& t)
             }
             const float hl1[3] = {HW[1 + G(0)][1], HW[2][1], HW[1 + G(2)][1]};
             const float hr0[3] = {HW[1 + G(0)][2], HW[2][2], HW[1 + G(2)][2]};
-            const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0, L.lane);
+            const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0);
             const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
             collide_store<ST, true, SYM>(t, k, rf, L, out, yy, cur, pd[0], gx, gy, Fx, Fy);
 #pragma unroll
@@ -850,7 +847,7 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
     auto aux_lanes = [&](int j) {   // n / T at the centre row of window rows j .. j+2
         const float hl[3] = {HW[j + G(0)][1], HW[j + 1][1], HW[j + G(2)][1]};
         const float hr[3] = {HW[j + G(0)][2], HW[j + 1][2], HW[j + G(2)][2]};
-        return aux_of<KIND>(k, gather9h(P[j + G(0)], P[j + 1], P[j + G(2)], hl, hr, L.lane));
+        return aux_of<KIND>(k, gather9h(P[j + G(0)], P[j + 1], P[j + G(2)], hl, hr));
     };
     auto aux_halo = [&](int j, int side) {   // n / T at c0-1 (side 0) or c0+64 (side 1)
         Nine nn;
@@ -912,8 +909,8 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
             auto field = [&](float AuxF::*m) {
                 const float l3[3] = {hl[G(0)].*m, hl[1].*m, hl[G(2)].*m};
                 const float r3[3] = {hr[G(0)].*m, hr[1].*m, hr[G(2)].*m};
-                return REV ? gather9h(ap.*m, a0.*m, am.*m, l3, r3, L.lane)
-                          : gather9h(am.*m, a0.*m, ap.*m, l3, r3, L.lane);
+                return REV ? gather9h(ap.*m, a0.*m, am.*m, l3, r3)
+                          : gather9h(am.*m, a0.*m, ap.*m, l3, r3);
             };
             float Fx, Fy;
             if constexpr (KIND == KIND_F1) {
@@ -927,7 +924,7 @@ __device__ __forceinline__ void f12_rows_cached(const TileCtxLP& t)
             }
             const float hl1[3] = {HW[1 + G(0)][1], HW[2][1], HW[1 + G(2)][1]};
             const float hr0[3] = {HW[1 + G(0)][2], HW[2][2], HW[1 + G(2)][2]};
-            const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0, L.lane);
+            const Nine p9 = gather9h(P[1 + G(0)], P[2], P[1 + G(2)], hl1, hr0);
             const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
             collide_store_kept<ST, SYM>(t, k, rf, L, in, out, yy, sk[j], pd[0], gx, gy, Fx, Fy);
 #pragma unroll
@@ -994,7 +991,7 @@ __device__ __forceinline__ void collide_row(const TileCtxLP& t, const ConstF& k,
                                             const typename PairOf<ST>::raw (&cur)[Q], const float (&p)[3],
                                             const float (&hl)[3], const float (&hr)[3], double phi)
 {
-    const Nine p9 = gather9h(p[0], p[1], p[2], hl, hr, L.lane);
+    const Nine p9 = gather9h(p[0], p[1], p[2], hl, hr);
     const float gx = -dx_iso(p9), gy = -dy_iso(p9);   // mediator momentum = -grad(phi)
     float Fx = 0.f, Fy = 0.f;
     if constexpr (KIND == KIND_F3) {

@@ -187,4 +187,34 @@ __device__ __forceinline__ double lane_value(double v, int src)
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// The value of lane - 1 (from_left) or lane + 1 (from_right) of the wavefront, and `edge` in lane 0
+// (from_left) or lane 63 (from_right), whose neighbour lies outside: one whole-wavefront DPP move
+// (wave_shr:1 / wave_shl:1; the lane without a source keeps the `old` operand, bound_ctrl off).
+__device__ __forceinline__ float from_left(float v, float edge)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                 __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v, float edge)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                 __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double dpp_pair(double v, double edge, int ctrl_left)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned long long e = __builtin_bit_cast(unsigned long long, edge);
+    unsigned lo, hi;
+    if (ctrl_left) {
+        lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)b, 0x138, 0xF, 0xF, false);
+        hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(e >> 32), (int)(unsigned)(b >> 32), 0x138, 0xF, 0xF, false);
+    } else {
+        lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)e, (int)(unsigned)b, 0x130, 0xF, 0xF, false);
+        hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(e >> 32), (int)(unsigned)(b >> 32), 0x130, 0xF, 0xF, false);
+    }
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double from_left(double v, double edge) { return dpp_pair(v, edge, 1); }
+__device__ __forceinline__ double from_right(double v, double edge) { return dpp_pair(v, edge, 0); }
+
 }  // namespace itolbm
