@@ -59,27 +59,39 @@ def c2_cases(m):
     return out
 
 
-def test_c2_fp64_bitwise_5_steps():
+def test_c2_fp64_bitwise_5_steps(monkeypatch):
+    """At the library's default shapes, and at the shapes the tuner picks for the benchmark (32-row
+    segments; F1 / F2 with the kernel that keeps a row's totals and colour sums, depth 2)."""
     m = _pkg.load()
     cases = c2_cases(m)
-    say("[gpu] C2: 14 x 512^2 sharp drops R=100, FP64 batched, 5 steps vs oracle")
-    ctx = m.Context([c[0] for c in cases])
-    for k in range(len(cases)):
-        ctx.init_drop(100.0, width=-1.0, case=k)
-    ctx.step(5, record=True)
+    say("[gpu] C2: 14 x 512^2 sharp drops R=100, FP64 batched, 5 steps vs oracle (default and tuned shapes)")
+    ctxs = []
+    for shape in (None, ("32", "2")):
+        if shape:
+            monkeypatch.setenv("ITOLBM_LY", shape[0])
+            monkeypatch.setenv("ITOLBM_DEPTH", shape[1])
+        ctx = m.Context([c[0] for c in cases])
+        for k in range(len(cases)):
+            ctx.init_drop(100.0, width=-1.0, case=k)
+        ctx.step(5, record=True)
+        ctxs.append((shape, ctx))
     for k, (_, op, tau, A) in enumerate(cases):
         sim = O.Sim(O.make_case(512, 512, op, tau, A, 0.8))
         sim.init_drop(100.0, width=-1.0)
         for _ in range(5):
             sim.step()
-        gR, gB = ctx.download(m.STATE_POST_COLLISION, case=k)
-        assert_same(gR, sim.post_R, f"C2 {op} tau={tau} red")
-        assert_same(gB, sim.post_B, f"C2 {op} tau={tau} blue")
-        gm, F = ctx.aux(case=k)
-        assert_same(gm, sim.gm, f"C2 {op} tau={tau} mediator momentum")
-        if sim.F is not None:
-            assert_same(F, sim.F, f"C2 {op} tau={tau} force")
-    ctx.close()
+        for shape, ctx in ctxs:
+            what = f"C2 {op} tau={tau} shape {shape}"
+            gR, gB = ctx.download(m.STATE_POST_COLLISION, case=k)
+            assert_same(gR, sim.post_R, f"{what} red")
+            assert_same(gB, sim.post_B, f"{what} blue")
+            gm, F = ctx.aux(case=k)
+            assert_same(gm, sim.gm, f"{what} mediator momentum")
+            if sim.F is not None:
+                assert_same(F, sim.F, f"{what} force")
+    for _, ctx in ctxs:
+        assert ctx.stored_bytes(case=0) == 384   # the P1 cases' launch: the compact form
+        ctx.close()
 
 
 C2_DP = {1: 2e-4, 3: 2e-4, 2: 3e-2, 4: 3e-2}
