@@ -137,30 +137,6 @@ __device__ __forceinline__ Nine gather9h(const double r[3], const double hl[3], 
     return n;
 }
 
-// density of the 19 values held by lanes base .. base+18, summed in index order as dens()
-__device__ __forceinline__ double lane_dens(double v, int base)
-{
-    double r = lane_value(v, base);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) r = r + lane_value(v, base + i);
-    return r;
-}
-
-// value g (0 .. 75) of a two-load halo (Halo2)
-__device__ __forceinline__ double halo2_value(double a, double b, int g)
-{
-    return (g < 64) ? lane_value(a, g) : lane_value(b, g - 64);
-}
-
-// density of halo site s of a two-load halo, summed in index order as dens()
-__device__ __forceinline__ double halo2_dens(double a, double b, int s)
-{
-    double r = halo2_value(a, b, Q * s);
-#pragma unroll
-    for (int i = 1; i < Q; ++i) r = r + halo2_value(a, b, Q * s + i);
-    return r;
-}
-
 // Mediator momentum gm = sum_i c_i M_i, M_i(x) = w_i (3 phi(x - c_i)):
 // emite_mediadores + propag_site_med + momentum (LBM_functions.cpp:1286-1337, 1056-1087).
 __device__ __forceinline__ void mediator_gradient(const Nine& phi, double& gx, double& gy, double& gz)
@@ -670,25 +646,6 @@ __device__ __forceinline__ void cpop_split(gcdouble* __restrict__ in, long long 
     } else {
         r = cR * p.f - (-t);
         b = cB * p.f + (-t);
-    }
-}
-
-// element of slot 0 of the site that site (col, Y) pulls velocity i from
-__device__ __forceinline__ long long csender(int px, int ny, const Lane& L, int Y, int i)
-{
-    const int xs = CX[i] > 0 ? L.xm : (CX[i] < 0 ? L.xp : L.col);
-    return (long long)wrapi(Y - CY[i], ny) * cblock_doubles(px) + xs;
-}
-
-// the 38 populations pulled by site (col, Y), each from its sender's stored values (any sender raw)
-__device__ __forceinline__ void cpull_site(gcdouble* __restrict__ in, int px, int ny, double beta, const Lane& L,
-                                           int Y, double fR[Q], double fB[Q])
-{
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const long long e0 = csender(px, ny, L, Y, i);
-        const CPop p = cpop_load(in, e0, px, i);
-        cpop_split(in, e0, px, beta, i, p, fR[i], fB[i]);
     }
 }
 
