@@ -29,19 +29,6 @@ struct Lane {
     bool interior;        // owns (stores) its column
 };
 
-__device__ __forceinline__ Lane make_lane(int nx, int strip, int H)
-{
-    Lane L;
-    L.lane = threadIdx.x;
-    const int WI = 64 - 2 * H;
-    const int xr = strip * WI + L.lane - H;
-    L.col = wrapi(xr, nx);
-    L.xm = (L.col == 0) ? nx - 1 : L.col - 1;
-    L.xp = (L.col == nx - 1) ? 0 : L.col + 1;
-    L.interior = (L.lane >= H) && (L.lane < 64 - H) && (xr < nx);
-    return L;
-}
-
 // Aligned strip (one-row stencils): the 64 lanes own the 64 columns strip*64 .. strip*64+63, so
 // every wavefront-wide load of a column row and every store starts on a 64-element boundary.  The
 // columns just outside the strip (its halo) are handled by Halo1.
